@@ -1,0 +1,390 @@
+"""icbc_amd -- MI355X (gfx950) BC1 encoder, a drop-in for castano/icbc v1.05.
+
+Python mirror of the reference's public API (icbc.h:10-52) over the C ABI of
+``libicbc_amd.so`` (include/icbc_gpu.h):
+
+    import icbc_amd as icbc
+    icbc.init(icbc.Decoder.D3D10)
+    err = icbc.compress_bc1(icbc.Quality.Level8, colors64, weights16, (1, 1, 1),
+                            True, True, out8)
+
+plus batched entry points over numpy arrays or device buffers.  Every call
+runs on the GPU; there is no CPU encoder in this package.  If the shared
+library is missing or no gfx950 device is present the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libicbc_amd.so")
+
+
+class Decoder(enum.IntEnum):  # icbc.h:12-17
+    D3D10 = 0
+    NVIDIA = 1
+    AMD = 2
+    Intel = 3
+
+
+class Quality(enum.IntEnum):  # icbc.h:21-36
+    Level0 = 0
+    Level1 = 1
+    Level2 = 2
+    Level3 = 3
+    Level4 = 4
+    Level5 = 5
+    Level6 = 6
+    Level7 = 7
+    Level8 = 8
+    Level9 = 9
+    Fast = 1
+    Default = 8
+    Max = 9
+
+
+OK = 0
+ERR_INVALID_ARGUMENT = -1
+ERR_NOT_INITIALIZED = -2
+ERR_OUT_OF_MEMORY = -3
+ERR_DEVICE = -4
+ERR_NO_DEVICE = -5
+ERR_BUDGET = -6
+ERR_CANCELLED = -7
+
+JOB_HOST_MEMORY = 1
+SCHEDULE_GRID = 0
+SCHEDULE_PERSISTENT = 1
+
+
+class IcbcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"icbc_amd error {code}: {msg}")
+        self.code = code
+
+
+class Job(C.Structure):
+    """icbc_bc1_job (include/icbc_gpu.h)."""
+    _fields_ = [("level", C.c_int32), ("three_color_mode", C.c_int32),
+                ("three_color_black", C.c_int32), ("flags", C.c_uint32),
+                ("color_weights", C.c_float * 3), ("reserved", C.c_uint32),
+                ("block_count", C.c_uint64), ("colors", C.c_void_p), ("weights", C.c_void_p),
+                ("codes", C.c_void_p), ("errors", C.c_void_p), ("stats", C.c_void_p)]
+
+
+class PlanOptions(C.Structure):
+    _fields_ = [("schedule", C.c_int32), ("use_graph", C.c_int32),
+                ("workgroups_per_cu", C.c_int32), ("order_by_cost", C.c_int32)]
+
+
+# exported symbols of the C ABI (include/icbc_gpu.h), with ctypes signatures
+_SIGNATURES = {
+    "icbc_gpu_init": (C.c_int, [C.c_int]),
+    "icbc_gpu_compress_bc1_batch": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p]),
+    "icbc_gpu_compress_bc1_plain": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p]),
+    "icbc_gpu_compress_bc1": (C.c_float, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                          C.c_int, C.c_void_p]),
+    "icbc_gpu_plan_create": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,
+                                       C.POINTER(C.c_void_p)]),
+    "icbc_gpu_plan_execute": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "icbc_gpu_plan_destroy": (C.c_int, [C.c_void_p]),
+    "icbc_gpu_plan_tiles": (C.c_uint64, [C.c_void_p]),
+    "icbc_gpu_decode_bc1_batch": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, C.c_void_p]),
+    "icbc_gpu_evaluate_bc1_error_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                                    C.c_int, C.c_void_p]),
+    "icbc_gpu_last_error": (C.c_char_p, []),
+    "icbc_gpu_device_name": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "icbc_gpu_version": (C.c_int, []),
+    "icbc_gpu_malloc": (C.c_void_p, [C.c_size_t]),
+    "icbc_gpu_free": (C.c_int, [C.c_void_p]),
+    "icbc_gpu_memcpy_htod": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "icbc_gpu_memcpy_dtoh": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "icbc_gpu_memset": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t]),
+    "icbc_gpu_stream_create": (C.c_void_p, []),
+    "icbc_gpu_stream_destroy": (C.c_int, [C.c_void_p]),
+    "icbc_gpu_stream_synchronize": (C.c_int, [C.c_void_p]),
+    "icbc_gpu_event_create": (C.c_void_p, []),
+    "icbc_gpu_event_destroy": (C.c_int, [C.c_void_p]),
+    "icbc_gpu_event_record": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "icbc_gpu_event_synchronize": (C.c_int, [C.c_void_p]),
+    "icbc_gpu_event_elapsed_ms": (C.c_float, [C.c_void_p, C.c_void_p]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded C ABI library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise IcbcError(ERR_NO_DEVICE, f"{LIB_PATH} is not built; run python -m icbc_amd.build")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().icbc_gpu_last_error().decode()
+
+
+def check(code):
+    if code != OK:
+        raise IcbcError(code, last_error())
+    return code
+
+
+# ---------------------------------------------------------------------------
+# the reference's API
+
+
+def init(decoder=Decoder.D3D10):
+    """icbc::init (icbc.h:19, 3813)."""
+    check(lib().icbc_gpu_init(int(decoder)))
+
+
+def _ptr(a):
+    return a.ctypes.data
+
+
+def compress_bc1(level, input_colors, input_weights, color_weights, three_color_mode,
+                 three_color_black, output):
+    """icbc::compress_bc1 (icbc.h:48, 3846) for one block.
+
+    ``input_colors``: 64 floats (16 texels x RGBA); ``input_weights``: 16
+    floats; ``color_weights``: 3 floats; ``output``: a writable 8-byte buffer
+    (numpy uint8 array / bytearray).  Returns the block error; FLT_MAX with
+    ``output`` untouched for an invalid level.
+    """
+    c = np.ascontiguousarray(input_colors, dtype=np.float32).reshape(1, 16, 4)
+    w = np.ascontiguousarray(input_weights, dtype=np.float32).reshape(1, 16)
+    out = np.frombuffer(output, dtype=np.uint8) if not isinstance(output, np.ndarray) else output
+    if out.nbytes < 8 or not out.flags.c_contiguous or not out.flags.writeable:
+        raise ValueError("output must be a writable contiguous 8-byte buffer")
+    codes, errors, _ = compress_many(level, c, w, color_weights, three_color_mode, three_color_black,
+                                     prior=out.view(np.uint8)[:8].reshape(1, 8))
+    out.view(np.uint8)[:8] = codes[0]
+    return float(errors[0])
+
+
+def compress_bc3(level, input_colors, input_weights, color_weights, six_alpha_mode, output):
+    """icbc::compress_bc3 (icbc.h:49, 3850): a stub upstream that encodes nothing."""
+    return 0.0
+
+
+# ---------------------------------------------------------------------------
+# device buffers and streams (plumbing for callers without a HIP runtime)
+
+
+class DeviceBuffer:
+    """Device allocation owned by Python; freed on close()/GC."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = lib().icbc_gpu_malloc(max(1, self.nbytes))
+        if not p:
+            raise IcbcError(ERR_OUT_OF_MEMORY, last_error())
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a, offset=0):
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        check(lib().icbc_gpu_memcpy_htod(self.ptr + offset, _ptr(a), a.nbytes))
+
+    def download(self, dtype, shape, offset=0):
+        out = np.empty(shape, dtype=dtype)
+        assert offset + out.nbytes <= self.nbytes
+        check(lib().icbc_gpu_memcpy_dtoh(_ptr(out), self.ptr + offset, out.nbytes))
+        return out
+
+    def fill(self, byte):
+        check(lib().icbc_gpu_memset(self.ptr, int(byte), self.nbytes))
+
+    def close(self):
+        if self.ptr:
+            lib().icbc_gpu_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        self.handle = lib().icbc_gpu_stream_create()
+        if not self.handle:
+            raise IcbcError(ERR_DEVICE, last_error())
+
+    def synchronize(self):
+        check(lib().icbc_gpu_stream_synchronize(self.handle))
+
+    def close(self):
+        if self.handle:
+            lib().icbc_gpu_stream_destroy(self.handle)
+            self.handle = None
+
+
+class Event:
+    def __init__(self):
+        self.handle = lib().icbc_gpu_event_create()
+        if not self.handle:
+            raise IcbcError(ERR_DEVICE, last_error())
+
+    def record(self, stream=None):
+        check(lib().icbc_gpu_event_record(self.handle, stream.handle if stream else None))
+
+    def synchronize(self):
+        check(lib().icbc_gpu_event_synchronize(self.handle))
+
+    def elapsed_ms(self, end):
+        return float(lib().icbc_gpu_event_elapsed_ms(self.handle, end.handle))
+
+    def close(self):
+        if self.handle:
+            lib().icbc_gpu_event_destroy(self.handle)
+            self.handle = None
+
+
+def device_name():
+    buf = C.create_string_buffer(256)
+    check(lib().icbc_gpu_device_name(buf, 256))
+    return buf.value.decode()
+
+
+# ---------------------------------------------------------------------------
+# batches
+
+
+def make_job(level, block_count, colors, codes, weights=None, errors=None, stats=None,
+             color_weights=(1.0, 1.0, 1.0), three_color_mode=False, three_color_black=False,
+             host=False):
+    """Builds an icbc_bc1_job; pointer arguments are ints (device) or numpy arrays (host)."""
+    def addr(x):
+        if x is None:
+            return None
+        if isinstance(x, np.ndarray):
+            return x.ctypes.data
+        if isinstance(x, DeviceBuffer):
+            return x.ptr
+        return int(x)
+    j = Job()
+    j.level = int(level)
+    j.three_color_mode = int(bool(three_color_mode))
+    j.three_color_black = int(bool(three_color_black))
+    j.flags = JOB_HOST_MEMORY if host else 0
+    for k in range(3):
+        j.color_weights[k] = float(color_weights[k])
+    j.reserved = 0
+    j.block_count = int(block_count)
+    j.colors = addr(colors)
+    j.weights = addr(weights)
+    j.codes = addr(codes)
+    j.errors = addr(errors)
+    j.stats = addr(stats)
+    return j
+
+
+def _job_array(jobs):
+    arr = (Job * len(jobs))(*jobs)
+    return arr
+
+
+def compress_batch(jobs, stream=None, plain=False):
+    """icbc_gpu_compress_bc1_batch / _plain over a list of Job."""
+    arr = _job_array(jobs)
+    fn = lib().icbc_gpu_compress_bc1_plain if plain else lib().icbc_gpu_compress_bc1_batch
+    check(fn(C.addressof(arr), len(jobs), stream.handle if stream else None))
+
+
+def compress_many(level, colors, weights=None, color_weights=(1.0, 1.0, 1.0),
+                  three_color_mode=False, three_color_black=False, prior=None, with_stats=False,
+                  plain=False):
+    """Encodes block-major host arrays: colors (n, 16, 4) float32, weights (n, 16)
+    or None.  Returns (codes uint8[n, 8], errors float32[n], stats int32[n, 4] | None)."""
+    colors = np.ascontiguousarray(colors, dtype=np.float32)
+    n = colors.shape[0]
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    codes = np.zeros((n, 8), np.uint8) if prior is None else np.array(prior, np.uint8, copy=True)
+    errors = np.zeros(n, np.float32)
+    stats = np.zeros((n, 4), np.int32) if with_stats else None
+    job = make_job(level, n, colors, codes, w, errors, stats, color_weights, three_color_mode,
+                   three_color_black, host=True)
+    compress_batch([job], plain=plain)
+    return codes, errors, stats
+
+
+class Plan:
+    """Persistent batched plan over device-memory jobs (icbc_gpu_plan_*)."""
+
+    def __init__(self, jobs, schedule=SCHEDULE_PERSISTENT, use_graph=True, workgroups_per_cu=0,
+                 order_by_cost=True, stream=None):
+        self._jobs = _job_array(jobs)
+        opt = PlanOptions(schedule, int(bool(use_graph)), workgroups_per_cu, int(bool(order_by_cost)))
+        h = C.c_void_p()
+        check(lib().icbc_gpu_plan_create(C.addressof(self._jobs), len(jobs), C.byref(opt),
+                                         stream.handle if stream else None, C.byref(h)))
+        self.handle = h
+
+    @property
+    def tiles(self):
+        return int(lib().icbc_gpu_plan_tiles(self.handle))
+
+    def execute(self, stream=None):
+        check(lib().icbc_gpu_plan_execute(self.handle, stream.handle if stream else None))
+
+    def close(self):
+        if self.handle:
+            lib().icbc_gpu_plan_destroy(self.handle)
+            self.handle = None
+
+
+def decode_bc1_batch(codes_dev, block_count, rgba_dev, decoder=Decoder.D3D10, stream=None):
+    check(lib().icbc_gpu_decode_bc1_batch(int(codes_dev), int(block_count), int(rgba_dev), int(decoder),
+                                          stream.handle if stream else None))
+
+
+def evaluate_bc1_error_batch(rgba_dev, codes_dev, block_count, errors_dev, decoder=Decoder.D3D10,
+                             stream=None):
+    check(lib().icbc_gpu_evaluate_bc1_error_batch(int(rgba_dev), int(codes_dev), int(block_count),
+                                                  int(errors_dev), int(decoder),
+                                                  stream.handle if stream else None))
+
+
+def decode_bc1(block, decoder=Decoder.D3D10):
+    """icbc::decode_bc1 (icbc.h:38, 3820) for one block -> 64 RGBA8 bytes."""
+    b = np.ascontiguousarray(np.frombuffer(bytes(block), np.uint8)[:8])
+    d = DeviceBuffer(128)
+    d.upload(b)
+    decode_bc1_batch(d.ptr, 1, d.ptr + 64, decoder)
+    out = d.download(np.uint8, 64, offset=64)
+    d.close()
+    return out
+
+
+def evaluate_bc1_error(rgba_block, block, decoder=Decoder.D3D10):
+    """icbc::evaluate_bc1_error (icbc.h:43, 3837) for one block."""
+    d = DeviceBuffer(128)
+    d.upload(np.ascontiguousarray(rgba_block, np.uint8).reshape(64))
+    d.upload(np.ascontiguousarray(np.frombuffer(bytes(block), np.uint8)[:8]), offset=64)
+    evaluate_bc1_error_batch(d.ptr, d.ptr + 64, 1, d.ptr + 80, decoder)
+    e = float(d.download(np.float32, 1, offset=80)[0])
+    d.close()
+    return e

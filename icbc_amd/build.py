@@ -1,0 +1,53 @@
+"""Builds libicbc_amd.so (HIP kernels + C ABI + icbc:: API) in-tree for gfx950.
+
+    python -m icbc_amd.build          # or: python icbc_amd/build.py
+
+Compiler flags that the encoder's bit-exactness depends on:
+  -ffp-contract=off                    no fused multiply-add contraction
+  -fno-fast-math                       IEEE semantics (NaN ordering, no reassociation)
+  -fno-gpu-flush-denormals-to-zero     fp32 denormals kept, as on the host
+  (HIP's default correctly-rounded fp32 division is left on)
+Packed-fp32 instruction selection is disabled for device code.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libicbc_amd.so")
+SOURCES = ["bc1_encoder.hip", "icbc_runtime.cpp"]
+ARCH = "gfx950"
+
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fno-fast-math", "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def build(verbose=False):
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, *srcs]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,655 @@
+// Device-side building blocks of the BC1 encoder (CDNA4 / gfx950).
+//
+// Every function reproduces the arithmetic of the reference encoder's scalar
+// build (castano/icbc v1.05, icbc.h with ICBC_SIMD=0) operation by operation,
+// so that, compiled with -ffp-contract=off and IEEE fp32 division and
+// denormals, the 8-byte block codes and float errors are bit-identical:
+//   max(a,b) = (b < a) ? a : b, min(a,b) = (a < b) ? a : b   (icbc.h:207-213)
+//   round01(a) = float(int(a + 0.5f))                        (icbc.h:417)
+//   dot(a,b) = (a.x*b.x + a.y*b.y) + a.z*b.z                 (icbc.h:305, 1533)
+// Sums over texels and colours run left to right, as in the reference.
+//
+// The functions here are per-lane ("one lane owns one 4x4 block"); the
+// wave-cooperative candidate search lives in bc1_encoder.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+namespace icbc_gpu {
+
+struct f3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+__device__ __forceinline__ float max_r(float a, float b) { return (b < a) ? a : b; }
+__device__ __forceinline__ float min_r(float a, float b) { return (a < b) ? a : b; }
+__device__ __forceinline__ float sat01(float a) { return min_r(max_r(a, 0.0f), 1.0f); }
+__device__ __forceinline__ f3 sat3(f3 a) { return mk(sat01(a.x), sat01(a.y), sat01(a.z)); }
+__device__ __forceinline__ float round01(float a) { return (float)(int)(a + 0.5f); }
+
+// ---------------------------------------------------------------------------
+// Tables written once by the host at init (icbc.h:1560-1570, 1947-1950,
+// 3374-3375, 3495-3515) -- see upload_tables() in bc1_encoder.hip.
+
+struct DecoderWeights {
+    float w1_3, w2_3, w1_9, w4_9, w2_9;
+};
+
+extern __constant__ float k_u8_to_unit[256];   // i / 255.0f
+extern __constant__ float k_mid5[32];          // 565 rounding midpoints
+extern __constant__ float k_mid6[64];
+extern __constant__ uint32_t k_four[976];      // c0 | c1 << 8 | c2 << 16
+extern __constant__ uint32_t k_three[160];     // c0 | c1 << 8
+extern __constant__ int k_four_total[16];
+extern __constant__ int k_three_total[16];
+extern __constant__ uint16_t k_match5[4][256]; // max | min << 8
+extern __constant__ uint16_t k_match6[4][256];
+extern __constant__ DecoderWeights k_dweights[4];
+
+// Encoded block: col0 | col1 << 16, indices.
+struct Code {
+    uint32_t cols;
+    uint32_t idx;
+};
+
+__device__ __forceinline__ uint32_t col0(Code c) { return c.cols & 0xffffu; }
+__device__ __forceinline__ uint32_t col1(Code c) { return c.cols >> 16; }
+__device__ __forceinline__ Code make_code(uint32_t c0, uint32_t c1, uint32_t idx) {
+    return Code{c0 | (c1 << 16), idx};
+}
+
+// ---------------------------------------------------------------------------
+// 565 conversion (icbc.h:1588-1667)
+
+__device__ __forceinline__ float q5(float x) { return round01(x * 31.0f) * (1.0f / 31.0f); }
+__device__ __forceinline__ float q6(float x) { return round01(x * 63.0f) * (1.0f / 63.0f); }
+__device__ __forceinline__ f3 quantize_endpoint(f3 v) {
+    return mk(q5(sat01(v.x)), q6(sat01(v.y)), q5(sat01(v.z)));
+}
+
+__device__ __forceinline__ uint32_t to565(f3 v) {
+    uint32_t r = (uint32_t)min_r(max_r(v.x * 31.0f, 0.0f), 31.0f);
+    uint32_t g = (uint32_t)min_r(max_r(v.y * 63.0f, 0.0f), 63.0f);
+    uint32_t b = (uint32_t)min_r(max_r(v.z * 31.0f, 0.0f), 31.0f);
+    r += (v.x > k_mid5[r]) ? 1u : 0u;
+    g += (v.y > k_mid6[g]) ? 1u : 0u;
+    b += (v.z > k_mid5[b]) ? 1u : 0u;
+    return (r << 11) | (g << 5) | b;
+}
+
+__device__ __forceinline__ uint32_t to_u8(float v) { return (uint32_t)(uint8_t)(sat01(v) * 255 + 0.5f); }
+
+// ---------------------------------------------------------------------------
+// Palettes (icbc.h:2751-2900).  Channels are kept as ints 0..255.
+
+struct Pal8 {
+    int r[4], g[4], b[4];
+};
+
+__device__ __forceinline__ void expand565(uint32_t c, int& r, int& g, int& b) {
+    int r5 = (c >> 11) & 31, g6 = (c >> 5) & 63, b5 = c & 31;
+    r = (r5 << 3) | (r5 >> 2);
+    g = (g6 << 2) | (g6 >> 4);
+    b = (b5 << 3) | (b5 >> 2);
+}
+
+__device__ __forceinline__ int interp_ch(int decoder, bool four, int p0, int p1) {
+    // third palette entry of one channel for D3D10 / AMD / Intel
+    if (four) {
+        if (decoder == 2) return (uint8_t)((43 * p0 + 21 * p1 + 32) >> 6);
+        if (decoder == 3) return (uint8_t)((171 * p0 + 85 * p1 + 128) >> 8);
+        return (2 * p0 + p1) / 3;
+    }
+    if (decoder == 2) return (uint8_t)((p0 + p1 + 1) / 2);
+    if (decoder == 3) return (uint8_t)((128 * p0 + 128 * p1 + 128) >> 8);
+    return (p0 + p1) / 2;
+}
+
+__device__ __forceinline__ void palette8(uint32_t c0, uint32_t c1, int decoder, Pal8& p) {
+    expand565(c0, p.r[0], p.g[0], p.b[0]);
+    expand565(c1, p.r[1], p.g[1], p.b[1]);
+    const bool four = c0 > c1;
+    if (decoder == 1) {
+        // NVIDIA: red and blue interpolate the 5-bit values, green the 8-bit ones.
+        int r0 = (c0 >> 11) & 31, b0 = c0 & 31, r1 = (c1 >> 11) & 31, b1 = c1 & 31;
+        int gd = p.g[1] - p.g[0];
+        if (four) {
+            p.r[2] = (uint8_t)(((2 * r0 + r1) * 22) / 8);
+            p.g[2] = (uint8_t)((256 * p.g[0] + gd / 4 + 128 + gd * 80) / 256);
+            p.b[2] = (uint8_t)(((2 * b0 + b1) * 22) / 8);
+            p.r[3] = (uint8_t)(((2 * r1 + r0) * 22) / 8);
+            p.g[3] = (uint8_t)((256 * p.g[1] - gd / 4 + 128 - gd * 80) / 256);
+            p.b[3] = (uint8_t)(((2 * b1 + b0) * 22) / 8);
+        } else {
+            p.r[2] = (uint8_t)(((r0 + r1) * 33) / 8);
+            p.g[2] = (uint8_t)((256 * p.g[0] + gd / 4 + 128 + gd * 128) / 256);
+            p.b[2] = (uint8_t)(((b0 + b1) * 33) / 8);
+            p.r[3] = p.g[3] = p.b[3] = 0;
+        }
+        return;
+    }
+    p.r[2] = interp_ch(decoder, four, p.r[0], p.r[1]);
+    p.g[2] = interp_ch(decoder, four, p.g[0], p.g[1]);
+    p.b[2] = interp_ch(decoder, four, p.b[0], p.b[1]);
+    if (four) {
+        p.r[3] = interp_ch(decoder, true, p.r[1], p.r[0]);
+        p.g[3] = interp_ch(decoder, true, p.g[1], p.g[0]);
+        p.b[3] = interp_ch(decoder, true, p.b[1], p.b[0]);
+    } else {
+        p.r[3] = p.g[3] = p.b[3] = 0;
+    }
+}
+
+// palette as floats: channel / 255.0f (icbc.h:1656, 2902-2909), from a table
+// holding exactly those quotients.
+__device__ __forceinline__ void palette_f(uint32_t c0, uint32_t c1, int decoder, f3 pal[4]) {
+    Pal8 p;
+    palette8(c0, c1, decoder, p);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        pal[i] = mk(k_u8_to_unit[p.r[i]], k_u8_to_unit[p.g[i]], k_u8_to_unit[p.b[i]]);
+}
+
+// ---------------------------------------------------------------------------
+// Error of a block for a palette and index word (icbc.h:2955-2958, 3020-3028).
+
+__device__ __forceinline__ float texel_error(f3 p, f3 c, f3 w) {
+    f3 d = ((p - c) * w) * 255.0f;
+    return dot3(d, d);
+}
+
+__device__ __forceinline__ f3 pick(const f3 pal[4], uint32_t k) {
+    f3 lo = (k & 1) ? pal[1] : pal[0];
+    f3 hi = (k & 1) ? pal[3] : pal[2];
+    return (k & 2) ? hi : lo;
+}
+
+__device__ __forceinline__ float palette_error(const f3 tex[16], const float wt[16], f3 cw,
+                                               const f3 pal[4], uint32_t indices) {
+    float e = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 16; i++) e += wt[i] * texel_error(pick(pal, (indices >> (2 * i)) & 3), tex[i], cw);
+    return e;
+}
+
+__device__ __forceinline__ float code_error(const f3 tex[16], const float wt[16], f3 cw, int decoder,
+                                            Code c) {
+    f3 pal[4];
+    palette_f(col0(c), col1(c), decoder, pal);
+    return palette_error(tex, wt, cw, pal, c.idx);
+}
+
+// ---------------------------------------------------------------------------
+// Index selection (icbc.h:3097-3235).  Returned words hold texel i's 2-bit
+// index at bits 2i..2i+1.
+
+__device__ __forceinline__ uint32_t indices4(const f3 tex[16], f3 w, const f3 pal[4]) {
+    f3 p0 = pal[0] * w, p1 = pal[1] * w, p2 = pal[2] * w, p3 = pal[3] * w;
+    uint32_t out = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        f3 c = tex[i] * w;
+        f3 e0 = c - p0, e1 = c - p1, e2 = c - p2, e3 = c - p3;
+        float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2), d3 = dot3(e3, e3);
+        uint32_t hi = ((d1 > d2) & (d0 > d2)) | ((d0 > d3) & (d1 > d3));
+        uint32_t lo = (d0 > d3) & (d2 > d3);
+        out |= (lo | (hi << 1)) << (2 * i);
+    }
+    return out;
+}
+
+__device__ __forceinline__ uint32_t indices3(const f3 tex[16], f3 w, bool allow_black, const f3 pal[4]) {
+    f3 p0 = pal[0] * w, p1 = pal[1] * w, p2 = pal[2] * w;
+    uint32_t out = 0;
+    if (allow_black) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            f3 c = tex[i] * w;
+            f3 e0 = p0 - c, e1 = p1 - c, e2 = p2 - c;
+            float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2), d3 = dot3(c, c);
+            uint32_t i1 = d1 < d2;
+            uint32_t i2 = (d2 <= d0) & (d2 <= d1);
+            uint32_t i3 = (d3 <= d0) & (d3 <= d1) & (d3 <= d2);
+            out |= ((i1 | i3) | ((i2 | i3) << 1)) << (2 * i);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            f3 c = tex[i] * w;
+            f3 e0 = c - p0, e1 = c - p1, e2 = c - p2;
+            float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2);
+            uint32_t i1 = d1 < d2;
+            uint32_t i2 = (d2 <= d0) & (d2 <= d1);
+            out |= (i1 | (i2 << 1)) << (2 * i);
+        }
+    }
+    return out;
+}
+
+__device__ __forceinline__ uint32_t indices_refine(const f3 tex[16], f3 w, const f3 pal[4]) {
+    f3 p0 = pal[0] * w, p1 = pal[1] * w, p2 = pal[2] * w, p3 = pal[3] * w;
+    uint32_t out = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        f3 c = tex[i] * w;
+        f3 e0 = c - p0, e1 = c - p1, e2 = c - p2, e3 = c - p3;
+        float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2), d3 = dot3(e3, e3);
+        uint32_t i1 = (d1 <= d0) & (d1 < d2) & (d1 < d3);
+        uint32_t i2 = (d2 <= d0) & (d2 <= d1) & (d2 < d3);
+        uint32_t i3 = (d3 <= d0) & (d3 <= d1) & (d3 <= d2);
+        out |= ((i1 | i3) | ((i2 | i3) << 1)) << (2 * i);
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Block output (icbc.h:3238-3274)
+
+__device__ __forceinline__ float emit4(const f3 tex[16], const float wt[16], f3 cw, int decoder, f3 a,
+                                       f3 b, Code& out) {
+    uint32_t c0 = to565(a), c1 = to565(b);
+    if (c0 < c1) {
+        uint32_t t = c0;
+        c0 = c1;
+        c1 = t;
+    }
+    f3 pal[4];
+    palette_f(c0, c1, decoder, pal);
+    uint32_t idx = indices4(tex, cw, pal);
+    out = make_code(c0, c1, idx);
+    return palette_error(tex, wt, cw, pal, idx);
+}
+
+__device__ __forceinline__ float emit3(const f3 tex[16], const float wt[16], f3 cw, int decoder,
+                                       bool allow_black, f3 a, f3 b, Code& out) {
+    uint32_t c0 = to565(a), c1 = to565(b);
+    if (c0 > c1) {
+        uint32_t t = c0;
+        c0 = c1;
+        c1 = t;
+    }
+    f3 pal[4];
+    palette_f(c0, c1, decoder, pal);
+    uint32_t idx = indices3(tex, cw, allow_black, pal);
+    out = make_code(c0, c1, idx);
+    return palette_error(tex, wt, cw, pal, idx);
+}
+
+// ---------------------------------------------------------------------------
+// Colour reduction (icbc.h:1673-1748)
+
+__device__ __forceinline__ bool is_black(f3 c) {
+    return c.x < 1.0f / 8 && c.y < 1.0f / 8 && c.z < 1.0f / 8;
+}
+
+__device__ __forceinline__ bool near3(f3 a, f3 b, float eps) {
+    return fabsf(a.x - b.x) < eps && fabsf(a.y - b.y) < eps && fabsf(a.z - b.z) < eps;
+}
+
+// Merge each weighted texel into the first earlier colour within `threshold`
+// (running weighted average), else append it.  Texels with weight <= 0 (or
+// NaN) are dropped.
+__device__ __forceinline__ int reduce_colors(const f3 tex[16], const float wt[16], float threshold,
+                                             f3 cols[16], float ws[16], bool& any_black) {
+    int n = 0;
+    any_black = false;
+    for (int i = 0; i < 16; i++) {
+        const float wi = wt[i];
+        if (!(wi > 0)) continue;
+        const f3 ci = tex[i];
+        int j = 0;
+        while (j < n && !near3(cols[j], ci, threshold)) j++;
+        if (j < n) {
+            const float wj = ws[j];
+            const f3 s = cols[j] * wj + ci * wi;
+            const float d = wj + wi;
+            cols[j] = mk(s.x / d, s.y / d, s.z / d);
+            ws[j] = wj + wi;
+        } else {
+            cols[n] = ci;
+            ws[n] = wi;
+            n++;
+        }
+        if (is_black(ci)) any_black = true;
+    }
+    return n;
+}
+
+__device__ __forceinline__ int drop_blacks(const f3 cols[16], const float ws[16], int n, f3 out[16],
+                                           float wout[16]) {
+    int m = 0;
+    for (int i = 0; i < n; i++) {
+        if (is_black(cols[i])) continue;
+        out[m] = cols[i];
+        wout[m] = ws[i];
+        m++;
+    }
+    return m;
+}
+
+// ---------------------------------------------------------------------------
+// Principal axis: weighted centroid, covariance, 6 power-method steps
+// (icbc.h:1752-1844).
+
+__device__ __forceinline__ f3 principal_axis(const f3 pts[16], const float ws[16], int n) {
+    float total = 0.0f;
+    f3 c = mk(0.0f, 0.0f, 0.0f);
+    for (int i = 0; i < n; i++) {
+        total += ws[i];
+        c = c + pts[i] * ws[i];
+    }
+    c = c * (1.0f / total);
+
+    float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f, m4 = 0.0f, m5 = 0.0f;
+    for (int i = 0; i < n; i++) {
+        f3 a = pts[i] - c;
+        f3 b = a * ws[i];
+        m0 += a.x * b.x;
+        m1 += a.x * b.y;
+        m2 += a.x * b.z;
+        m3 += a.y * b.y;
+        m4 += a.y * b.z;
+        m5 += a.z * b.z;
+    }
+    if (m0 == 0 && m3 == 0 && m5 == 0) return mk(0.0f, 0.0f, 0.0f);
+
+    f3 r0 = mk(m0, m1, m2), r1 = mk(m1, m3, m4), r2 = mk(m2, m4, m5);
+    float l0 = dot3(r0, r0), l1 = dot3(r1, r1), l2 = dot3(r2, r2);
+    f3 v = (l0 > l1 && l0 > l2) ? r0 : ((l1 > l2) ? r1 : r2);
+#pragma unroll
+    for (int it = 0; it < 6; it++) {
+        float x = v.x * m0 + v.y * m1 + v.z * m2;
+        float y = v.x * m1 + v.y * m3 + v.z * m4;
+        float z = v.x * m2 + v.y * m4 + v.z * m5;
+        float norm = max_r(max_r(x, y), z);
+        v = mk(x, y, z) * (1.0f / norm);
+    }
+    return v;
+}
+
+// Summed-area table of the colours sorted along the principal axis
+// (icbc.h:1857-1937).  Row k+1 of `sat` (float4 r,g,b,w) holds the prefix
+// over the first k+1 sorted colours; row 0 holds zeros so that a cluster
+// boundary of 0 reads a zero sum.  Only rows 0..n are written.
+__device__ __forceinline__ void build_sat(const f3 cols[16], const float ws[16], int n, float4* sat) {
+    f3 axis = principal_axis(cols, ws, n);
+    int order[16];
+    float key[16];
+    for (int i = 0; i < n; i++) {
+        order[i] = i;
+        key[i] = dot3(cols[i], axis);
+    }
+    // stable insertion sort, strict < (identical to the reference's loop even with NaN keys)
+    for (int i = 1; i < n; i++) {
+        for (int j = i; j > 0 && key[j] < key[j - 1]; j--) {
+            float tk = key[j];
+            key[j] = key[j - 1];
+            key[j - 1] = tk;
+            int to = order[j];
+            order[j] = order[j - 1];
+            order[j - 1] = to;
+        }
+    }
+    sat[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float r = 0.0f, g = 0.0f, b = 0.0f, w = 0.0f;
+    for (int i = 0; i < n; i++) {
+        const float wi = ws[order[i]];
+        const f3 c = cols[order[i]];
+        if (i == 0) {
+            r = c.x * wi;
+            g = c.y * wi;
+            b = c.z * wi;
+            w = wi;
+        } else {
+            r = r + c.x * wi;
+            g = g + c.y * wi;
+            b = b + c.z * wi;
+            w = w + wi;
+        }
+        sat[i + 1] = make_float4(r, g, b, w);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One cluster-fit candidate (icbc.h:2685-2742 four clusters, 2299-2344 three).
+// s0..s2 are SAT rows at the cluster boundaries, tot the full-sum row.
+// Returns the candidate's error (without the constant sum(w x^2) term) and the
+// snapped endpoints.
+
+__device__ __forceinline__ float candidate4(float4 s0, float4 s1, float4 s2, float4 tot, f3 metric,
+                                            const DecoderWeights& dw, f3& a, f3& b) {
+    f3 x0 = mk(s0.x, s0.y, s0.z), x1 = mk(s1.x, s1.y, s1.z), x2 = mk(s2.x, s2.y, s2.z);
+    float w0 = s0.w, w1 = s1.w, w2 = s2.w;
+    const float w3 = tot.w - w2;
+    x2 = x2 - x1;
+    x1 = x1 - x0;
+    w2 = w2 - w1;
+    w1 = w1 - w0;
+
+    const float alpha2 = w2 * dw.w1_9 + (w1 * dw.w4_9 + w0);
+    const float beta2 = w1 * dw.w1_9 + (w2 * dw.w4_9 + w3);
+    const float ab = (w1 + w2) * dw.w2_9;
+    const float factor = 1.0f / (alpha2 * beta2 - ab * ab);
+
+    const f3 ax = x2 * dw.w1_3 + (x1 * dw.w2_3 + x0);
+    const f3 bx = mk(tot.x, tot.y, tot.z) - ax;
+
+    a = quantize_endpoint((ax * beta2 - bx * ab) * factor);
+    b = quantize_endpoint((bx * alpha2 - ax * ab) * factor);
+
+    const f3 e2 = (a * (b * ab - ax) - b * bx) * 2.0f;
+    const f3 e1 = (a * a) * alpha2 + ((b * b) * beta2 + e2);
+    return dot3(e1, metric);
+}
+
+__device__ __forceinline__ float candidate3(float4 s0, float4 s1, float4 tot, f3 metric, f3& a, f3& b) {
+    f3 x0 = mk(s0.x, s0.y, s0.z), x1 = mk(s1.x, s1.y, s1.z);
+    float w0 = s0.w, w1 = s1.w;
+    const float w2 = tot.w - w1;
+    x1 = x1 - x0;
+    w1 = w1 - w0;
+
+    const float ab = w1 * 0.25f;
+    const float alpha2 = w0 + ab;
+    const float beta2 = w2 + ab;
+    const float factor = 1.0f / (alpha2 * beta2 - ab * ab);
+
+    const f3 ax = x0 + x1 * 0.5f;
+    const f3 bx = mk(tot.x, tot.y, tot.z) - ax;
+
+    a = quantize_endpoint((ax * beta2 - bx * ab) * factor);
+    b = quantize_endpoint((bx * alpha2 - ax * ab) * factor);
+
+    const f3 e2 = (a * (b * ab - ax) - b * bx) * 2.0f;
+    const f3 e1 = (a * a) * alpha2 + ((b * b) * beta2 + e2);
+    return dot3(e1, metric);
+}
+
+// ---------------------------------------------------------------------------
+// Least-squares endpoints for fixed indices (icbc.h:3278-3310) and box fit
+// (icbc.h:3313-3368).
+
+__device__ __forceinline__ bool lsq_endpoints(uint32_t indices, const f3 tex[16], f3& a, f3& b) {
+    float a2 = 0.0f, b2 = 0.0f, ab = 0.0f;
+    f3 ax = mk(0.0f, 0.0f, 0.0f), bx = mk(0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint32_t bits = indices >> (2 * i);
+        float beta = (float)(bits & 1);
+        if (bits & 2) beta = (1 + beta) / 3.0f;
+        const float alpha = 1.0f - beta;
+        a2 += alpha * alpha;
+        b2 += beta * beta;
+        ab += alpha * beta;
+        ax = ax + tex[i] * alpha;
+        bx = bx + tex[i] * beta;
+    }
+    const float denom = a2 * b2 - ab * ab;
+    if (fabsf(denom - 0.0f) < (float)0.0001) return false;
+    const float factor = 1.0f / denom;
+    a = sat3((ax * b2 - bx * ab) * factor);
+    b = sat3((bx * a2 - ax * ab) * factor);
+    return true;
+}
+
+__device__ __forceinline__ void box_fit(const f3 cols[16], int n, f3& c0, f3& c1) {
+    f3 hi = mk(0.0f, 0.0f, 0.0f), lo = mk(1.0f, 1.0f, 1.0f);
+    for (int i = 0; i < n; i++) {
+        hi = mk(max_r(hi.x, cols[i].x), max_r(hi.y, cols[i].y), max_r(hi.z, cols[i].z));
+        lo = mk(min_r(lo.x, cols[i].x), min_r(lo.y, cols[i].y), min_r(lo.z, cols[i].z));
+    }
+    const float bias = (8.0f / 255.0f) / 16.0f;
+    const f3 d = hi - lo;
+    const f3 inset = sat3(mk(d.x / 16.0f - bias, d.y / 16.0f - bias, d.z / 16.0f - bias));
+    hi = sat3(hi - inset);
+    lo = sat3(lo + inset);
+    const f3 center = (hi + lo) * 0.5f;
+    float cxz = 0.0f, cyz = 0.0f;
+    for (int i = 0; i < n; i++) {
+        const f3 t = cols[i] - center;
+        cxz += t.x * t.z;
+        cyz += t.y * t.z;
+    }
+    float x0 = hi.x, y0 = hi.y, x1 = lo.x, y1 = lo.y;
+    if (cxz < 0) {
+        float t = x0;
+        x0 = x1;
+        x1 = t;
+    }
+    if (cyz < 0) {
+        float t = y0;
+        y0 = y1;
+        y1 = t;
+    }
+    c0 = mk(x0, y0, hi.z);
+    c1 = mk(x1, y1, lo.z);
+}
+
+// ---------------------------------------------------------------------------
+// Single-colour block (icbc.h:3519-3534)
+
+__device__ __forceinline__ Code single_color(uint32_t r, uint32_t g, uint32_t b, int decoder) {
+    const uint32_t mr = k_match5[decoder][r], mg = k_match6[decoder][g], mb = k_match5[decoder][b];
+    uint32_t c0 = ((mr & 0xff) << 11) | ((mg & 0xff) << 5) | (mb & 0xff);
+    uint32_t c1 = ((mr >> 8) << 11) | ((mg >> 8) << 5) | (mb >> 8);
+    uint32_t idx = 0xaaaaaaaau;
+    if (c0 < c1) {
+        uint32_t t = c0;
+        c0 = c1;
+        c1 = t;
+        idx ^= 0x55555555u;
+    }
+    return make_code(c0, c1, idx);
+}
+
+// ---------------------------------------------------------------------------
+// Endpoint refinement (icbc.h:3574-3651)
+
+// add a signed step to each 565 field, wrapping inside the field
+__device__ __forceinline__ uint32_t nudge565(uint32_t c, int dr, int dg, int db) {
+    uint32_t r = (((c >> 11) & 31) + dr) & 31;
+    uint32_t g = (((c >> 5) & 63) + dg) & 63;
+    uint32_t b = ((c & 31) + db) & 31;
+    return (r << 11) | (g << 5) | b;
+}
+
+// The 16 step directions of the refinement walk, in the reference's order
+// (icbc.h:3584-3609): (1,0,0) (0,1,0) (0,0,1) (-1,0,0) (0,-1,0) (0,0,-1)
+// (1,1,0) (1,0,1) (0,1,1) (-1,-1,0) (-1,0,-1) (0,-1,-1) (-1,1,0) (1,-1,0)
+// (0,-1,1) (0,1,-1).  Each channel's 16 steps are packed as 2-bit fields
+// holding d + 1, so a step is decoded with shifts instead of a memory table.
+constexpr int8_t kStepR[16] = {1, 0, 0, -1, 0, 0, 1, 1, 0, -1, -1, 0, -1, 1, 0, 0};
+constexpr int8_t kStepG[16] = {0, 1, 0, 0, -1, 0, 1, 0, 1, -1, 0, -1, 1, -1, -1, 1};
+constexpr int8_t kStepB[16] = {0, 0, 1, 0, 0, -1, 0, 1, 1, 0, -1, -1, 0, 0, 1, -1};
+constexpr uint32_t pack_steps(const int8_t (&d)[16]) {
+    uint32_t v = 0;
+    for (int k = 0; k < 16; k++) v |= uint32_t(d[k] + 1) << (2 * k);
+    return v;
+}
+constexpr uint32_t kPackR = pack_steps(kStepR);
+constexpr uint32_t kPackG = pack_steps(kStepG);
+constexpr uint32_t kPackB = pack_steps(kStepB);
+
+__device__ __forceinline__ void refine_step(int k, int& dr, int& dg, int& db) {
+    dr = (int)((kPackR >> (2 * k)) & 3) - 1;
+    dg = (int)((kPackG >> (2 * k)) & 3) - 1;
+    db = (int)((kPackB >> (2 * k)) & 3) - 1;
+}
+
+__device__ __forceinline__ float refine(const f3 tex[16], const float wt[16], f3 cw, int decoder,
+                                        bool three_color_mode, float input_error, Code& out,
+                                        int& steps) {
+    float best = input_error;
+    int last = 0;
+    int i = 0;
+    for (; i < 256; i++) {
+        int dr, dg, db;
+        refine_step(i & 15, dr, dg, db);
+        uint32_t c0 = col0(out), c1 = col1(out);
+        if ((i >> 4) & 1) c0 = nudge565(c0, dr, dg, db);
+        else c1 = nudge565(c1, dr, dg, db);
+        if (!three_color_mode) {
+            if (c0 == c1) c1 = nudge565(c1, 0, 1, 0);
+            if (c0 < c1) {
+                uint32_t t = c0;
+                c0 = c1;
+                c1 = t;
+            }
+        }
+        f3 pal[4];
+        palette_f(col0(out), col1(out), decoder, pal);  // palette of the current best block
+        const uint32_t idx = indices_refine(tex, cw, pal);
+        const Code cand = make_code(c0, c1, idx);
+        const float e = code_error(tex, wt, cw, decoder, cand);
+        if (e < best) {
+            best = e;
+            out = cand;
+            last = i;
+        }
+        if (i - last > 32) break;
+    }
+    steps = (i < 256) ? i + 1 : 256;
+    return best;
+}
+
+// ---------------------------------------------------------------------------
+// Per-level stage selection (icbc.h:3653-3731)
+
+struct Plan {
+    float threshold;
+    bool box, lsq, cluster, fit3_always, fit3_black_only, refine;
+};
+
+__device__ __forceinline__ Plan make_plan(int level, bool tcm, bool tcb) {
+    Plan p{0.0f, false, false, false, false, false, false};
+    if (level < 0 || level > 9) return p;
+    p.box = level <= 3;
+    p.lsq = level == 1 || level == 2;
+    p.cluster = level >= 2;
+    p.fit3_always = level >= 8 && tcm;
+    p.fit3_black_only = level >= 2 && level <= 7 && tcm && tcb;
+    p.refine = level == 9;
+    switch (level) {
+        case 2: p.threshold = 1.0f / 24; break;
+        case 3: p.threshold = 1.0f / 32; break;
+        case 4: p.threshold = 1.0f / 48; break;
+        case 5: p.threshold = 1.0f / 64; break;
+        case 6: p.threshold = 1.0f / 96; break;
+        case 7: p.threshold = 1.0f / 128; break;
+        case 8:
+        case 9: p.threshold = 1.0f / 256; break;
+        default: break;
+    }
+    return p;
+}
+
+}  // namespace icbc_gpu

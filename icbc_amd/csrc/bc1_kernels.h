@@ -1,0 +1,54 @@
+// Host-visible launch interface of the BC1 kernels (bc1_encoder.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace icbc_gpu {
+
+constexpr int kTile = 64;     // blocks per tile = lanes per wavefront
+constexpr int kSatRows = 17;  // zero row + up to 16 prefix rows per block
+
+// One job as the kernels see it (device memory, built by the runtime).
+struct JobDev {
+    const float4* colors;   // 16 float4 per block
+    const float4* weights;  // 4 float4 per block, or null for all 1.0
+    uint2* codes;           // col0 | col1 << 16, indices
+    float* errors;          // or null
+    int4* stats;            // or null
+    uint64_t block_count;
+    int32_t level, tcm, tcb, decoder;
+    float cw0, cw1, cw2;
+    int32_t pad;
+};
+
+// Tables built by the host at init (all four decoders at once).
+struct HostTables {
+    float u8_to_unit[256];
+    float mid5[32];
+    float mid6[64];
+    uint32_t four[976];
+    uint32_t three[160];
+    int32_t four_total[16];
+    int32_t three_total[16];
+    uint16_t match5[4][256];
+    uint16_t match6[4][256];
+    float dweights[4][5];
+};
+
+hipError_t upload_tables(const HostTables& t, hipStream_t stream);
+
+// One workgroup per tile; tile k covers blocks [tiles[k].y, +64) of job tiles[k].x.
+hipError_t launch_tiles(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, hipStream_t stream);
+// One job: tile k covers blocks [64 k, 64 k + 64).
+hipError_t launch_job(const JobDev* job, uint32_t ntiles, hipStream_t stream);
+// Persistent workgroups pulling tiles from `counter` (reset by the same call).
+hipError_t launch_persistent(const JobDev* jobs, const uint2* tiles, uint32_t ntiles,
+                             uint32_t* counter, uint32_t grid, hipStream_t stream);
+
+hipError_t launch_decode(const uint2* codes, uint64_t n, uint32_t* rgba, int decoder,
+                         hipStream_t stream);
+hipError_t launch_evaluate(const uint32_t* rgba, const uint2* codes, uint64_t n, float* errors,
+                           int decoder, hipStream_t stream);
+
+}  // namespace icbc_gpu

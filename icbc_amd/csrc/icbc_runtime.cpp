@@ -1,0 +1,812 @@
+// Host runtime of the MI355X BC1 encoder: the C ABI of include/icbc_gpu.h and
+// the drop-in C++ API of include/icbc.h.
+//
+// Responsibilities: build the encoder's lookup tables at init and upload them
+// to each device that is used; validate jobs; stage host-memory jobs through
+// device buffers; turn jobs into 64-block tiles; launch the kernels of
+// bc1_encoder.hip (one launch per batch, one per job, or a persistent /
+// graph-captured plan).  There is no CPU encoding path: every encode, decode
+// and evaluate runs on the GPU, and a missing or failing device is reported.
+
+#include <hip/hip_runtime.h>
+
+#include <float.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/icbc.h"
+#include "../../include/icbc_gpu.h"
+#include "bc1_kernels.h"
+
+using namespace icbc_gpu;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// error reporting
+
+thread_local std::string t_last_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    t_last_error = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(e == hipErrorOutOfMemory ? ICBC_ERR_OUT_OF_MEMORY : ICBC_ERR_DEVICE, "%s: %s (%d)",
+                what, hipGetErrorString(e), (int)e);
+}
+
+#define HIP_TRY(expr)                                  \
+    do {                                               \
+        hipError_t e_ = (expr);                        \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// tables (built once per process, uploaded once per device)
+
+std::mutex g_mutex;
+bool g_tables_built = false;
+HostTables g_tables;
+int g_decoder = -1;  // -1 = init not called
+std::vector<int> g_uploaded;  // device ordinals holding the tables
+
+// 565 rounding midpoints, icbc.h:1560-1570: the reference stores the midpoint
+// of adjacent expanded levels as a 6-decimal literal.  The literal's text is
+// rebuilt from the exact midpoint k/510 and parsed with strtof, as the
+// compiler parsed the literal.
+void build_midpoints(float* out, int bits) {
+    const int levels = 1 << bits;
+    for (int i = 0; i < levels - 1; i++) {
+        const int e0 = bits == 5 ? ((i << 3) | (i >> 2)) : ((i << 2) | (i >> 4));
+        const int e1 = bits == 5 ? (((i + 1) << 3) | ((i + 1) >> 2)) : (((i + 1) << 2) | ((i + 1) >> 4));
+        const long k = e0 + e1;
+        const long micro = (k * 2000000L + 510L) / 1020L;  // round-half-up of k/510 * 1e6
+        char buf[32];
+        snprintf(buf, sizeof buf, "%ld.%06ld", micro / 1000000L, micro % 1000000L);
+        out[i] = strtof(buf, nullptr);
+    }
+    out[levels - 1] = FLT_MAX;
+}
+
+int expand_bits(int v, int bits) { return bits == 5 ? ((v << 3) | (v >> 2)) : ((v << 2) | (v >> 4)); }
+
+// Single-colour endpoint tables, icbc.h:3377-3485: exhaustive search over
+// (min, max) endpoint pairs, first pair with the smallest decoder error wins.
+// Stored as max | min << 8.
+void build_match(uint16_t* table, int bits, int decoder) {
+    const int levels = 1 << bits;
+    for (int target = 0; target < 256; target++) {
+        int best = 256 * 100;
+        uint16_t pick = 0;
+        for (int lo = 0; lo < levels; lo++) {
+            for (int hi = 0; hi < levels; hi++) {
+                const int elo = expand_bits(lo, bits), ehi = expand_bits(hi, bits);
+                const int amd = abs(((43 * ehi + 21 * elo + 32) >> 6) - target);
+                const int nv_val = bits == 5 ? ((2 * hi + lo) * 22) / 8
+                                             : (256 * elo + (ehi - elo) / 4 + 128 + (ehi - elo) * 80) / 256;
+                const int nv = abs(nv_val - target);
+                const int intel = abs(((171 * ehi + 85 * elo + 128) >> 8) - target);
+                int err;
+                if (decoder == ICBC_DECODER_D3D10) err = std::max(std::max(amd, nv), intel);
+                else if (decoder == ICBC_DECODER_NVIDIA) err = nv;
+                else if (decoder == ICBC_DECODER_INTEL) err = intel;
+                else err = amd;
+                if (err < best) {
+                    best = err;
+                    pick = (uint16_t)(hi | (lo << 8));
+                }
+            }
+        }
+        table[target] = pick;
+    }
+}
+
+void build_tables(HostTables& t) {
+    for (int i = 0; i < 256; i++) t.u8_to_unit[i] = i / 255.0f;
+    build_midpoints(t.mid5, 5);
+    build_midpoints(t.mid6, 6);
+    // Cluster partitions in the reference's total order (icbc.h:1965-2034):
+    // the partitions first reachable with t colours are (c0, c1, t) for
+    // 0 <= c0 <= c1 <= t, c0 then c1 ascending; the last entry pads to 976.
+    int i = 0;
+    for (int tt = 1; tt <= 16; tt++) {
+        for (int c0 = 0; c0 <= tt; c0++)
+            for (int c1 = c0; c1 <= tt; c1++) t.four[i++] = (uint32_t)c0 | ((uint32_t)c1 << 8) | ((uint32_t)tt << 16);
+        t.four_total[tt - 1] = i;
+    }
+    for (; i < 976; i++) t.four[i] = t.four[967];
+    i = 0;
+    for (int tt = 1; tt <= 16; tt++) {
+        for (int c0 = 0; c0 <= tt; c0++) t.three[i++] = (uint32_t)c0 | ((uint32_t)tt << 8);
+        t.three_total[tt - 1] = i;
+    }
+    for (; i < 160; i++) t.three[i] = t.three[151];
+    for (int d = 0; d < 4; d++) {
+        build_match(t.match5[d], 5, d);
+        build_match(t.match6[d], 6, d);
+        // decoder interpolation weights, icbc.h:3495-3515
+        float w1_3 = 1.0f / 3, w2_3 = 2.0f / 3;
+        if (d == ICBC_DECODER_AMD) {
+            w1_3 = 21.0f / 64;
+            w2_3 = 43.0f / 64;
+        } else if (d == ICBC_DECODER_INTEL) {
+            w1_3 = 85.0f / 256;
+            w2_3 = 171.0f / 256;
+        }
+        t.dweights[d][0] = w1_3;
+        t.dweights[d][1] = w2_3;
+        t.dweights[d][2] = w1_3 * w1_3;
+        t.dweights[d][3] = w2_3 * w2_3;
+        t.dweights[d][4] = w1_3 * w2_3;
+    }
+}
+
+// Makes sure the current device holds the tables; returns the decoder.
+int ensure_device_ready(int* decoder) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (g_decoder < 0) return fail(ICBC_ERR_NOT_INITIALIZED, "icbc_gpu_init has not been called");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (std::find(g_uploaded.begin(), g_uploaded.end(), dev) == g_uploaded.end()) {
+        hipDeviceProp_t prop;
+        HIP_TRY(hipGetDeviceProperties(&prop, dev));
+        if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            return fail(ICBC_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950", dev,
+                        prop.gcnArchName);
+        hipError_t e = upload_tables(g_tables, nullptr);
+        if (e != hipSuccess) return hip_fail(e, "upload_tables");
+        g_uploaded.push_back(dev);
+    }
+    *decoder = g_decoder;
+    return ICBC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// jobs
+
+int validate(const icbc_bc1_job& j, size_t k) {
+    if (j.reserved != 0 || (j.flags & ~ICBC_JOB_HOST_MEMORY) != 0)
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: unknown flags or nonzero reserved field", k);
+    if (j.block_count == 0) return ICBC_OK;
+    if (!j.colors || !j.codes) return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: null colors or codes", k);
+    if (!(j.flags & ICBC_JOB_HOST_MEMORY)) {
+        if (((uintptr_t)j.colors & 15) || ((uintptr_t)j.weights & 15))
+            return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: colors/weights must be 16-byte aligned", k);
+        if (((uintptr_t)j.codes & 7) || ((uintptr_t)j.errors & 3) || ((uintptr_t)j.stats & 15))
+            return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: misaligned codes/errors/stats", k);
+    }
+    if (j.block_count > (uint64_t)0xffffffffu * kTile)
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: too many blocks", k);
+    return ICBC_OK;
+}
+
+JobDev to_dev(const icbc_bc1_job& j, int decoder) {
+    JobDev d;
+    d.colors = reinterpret_cast<const float4*>(j.colors);
+    d.weights = reinterpret_cast<const float4*>(j.weights);
+    d.codes = reinterpret_cast<uint2*>(j.codes);
+    d.errors = j.errors;
+    d.stats = reinterpret_cast<int4*>(j.stats);
+    d.block_count = j.block_count;
+    d.level = j.level;
+    d.tcm = j.three_color_mode ? 1 : 0;
+    d.tcb = j.three_color_black ? 1 : 0;
+    d.decoder = decoder;
+    d.cw0 = j.color_weights[0];
+    d.cw1 = j.color_weights[1];
+    d.cw2 = j.color_weights[2];
+    d.pad = 0;
+    return d;
+}
+
+uint32_t tiles_of(uint64_t blocks) { return (uint32_t)((blocks + kTile - 1) / kTile); }
+
+// Host blob kept alive until the stream has consumed it.
+void free_host_blob(void* p) { free(p); }
+
+// Copies `bytes` from host into fresh stream-ordered device memory.
+int upload_blob(const void* src, size_t bytes, hipStream_t s, void** dptr) {
+    HIP_TRY(hipMallocAsync(dptr, bytes, s));
+    void* keep = malloc(bytes);
+    if (!keep) return fail(ICBC_ERR_OUT_OF_MEMORY, "host allocation of %zu bytes", bytes);
+    memcpy(keep, src, bytes);
+    hipError_t e = hipMemcpyAsync(*dptr, keep, bytes, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+        free(keep);
+        return hip_fail(e, "hipMemcpyAsync");
+    }
+    e = hipLaunchHostFunc(s, free_host_blob, keep);
+    if (e != hipSuccess) return hip_fail(e, "hipLaunchHostFunc");
+    return ICBC_OK;
+}
+
+// Host-memory jobs: device copies of the job's buffers.
+struct Staged {
+    void* colors = nullptr;
+    void* weights = nullptr;
+    void* codes = nullptr;
+    void* errors = nullptr;
+    void* stats = nullptr;
+};
+
+int stage_in(const icbc_bc1_job& j, hipStream_t s, Staged& st, icbc_bc1_job& dj) {
+    dj = j;
+    dj.flags &= ~ICBC_JOB_HOST_MEMORY;
+    const uint64_t n = j.block_count;
+    if (n == 0) return ICBC_OK;
+    HIP_TRY(hipMallocAsync(&st.colors, n * 256, s));
+    HIP_TRY(hipMemcpyAsync(st.colors, j.colors, n * 256, hipMemcpyHostToDevice, s));
+    dj.colors = (const float*)st.colors;
+    if (j.weights) {
+        HIP_TRY(hipMallocAsync(&st.weights, n * 64, s));
+        HIP_TRY(hipMemcpyAsync(st.weights, j.weights, n * 64, hipMemcpyHostToDevice, s));
+        dj.weights = (const float*)st.weights;
+    }
+    // codes go in too: blocks the reference leaves untouched keep their bytes
+    HIP_TRY(hipMallocAsync(&st.codes, n * 8, s));
+    HIP_TRY(hipMemcpyAsync(st.codes, j.codes, n * 8, hipMemcpyHostToDevice, s));
+    dj.codes = st.codes;
+    if (j.errors) {
+        HIP_TRY(hipMallocAsync(&st.errors, n * 4, s));
+        dj.errors = (float*)st.errors;
+    }
+    if (j.stats) {
+        HIP_TRY(hipMallocAsync(&st.stats, n * 16, s));
+        dj.stats = (int32_t*)st.stats;
+    }
+    return ICBC_OK;
+}
+
+int stage_out(const icbc_bc1_job& j, hipStream_t s, Staged& st) {
+    const uint64_t n = j.block_count;
+    if (n == 0) return ICBC_OK;
+    HIP_TRY(hipMemcpyAsync(j.codes, st.codes, n * 8, hipMemcpyDeviceToHost, s));
+    if (j.errors) HIP_TRY(hipMemcpyAsync(j.errors, st.errors, n * 4, hipMemcpyDeviceToHost, s));
+    if (j.stats) HIP_TRY(hipMemcpyAsync(j.stats, st.stats, n * 16, hipMemcpyDeviceToHost, s));
+    void* ptrs[5] = {st.colors, st.weights, st.codes, st.errors, st.stats};
+    for (void* p : ptrs)
+        if (p) HIP_TRY(hipFreeAsync(p, s));
+    return ICBC_OK;
+}
+
+// Shared front half of batch/plain: validate, stage host jobs, build JobDev list.
+struct Prepared {
+    std::vector<icbc_bc1_job> jobs;  // device-pointer view
+    std::vector<Staged> staged;      // per job (empty for device jobs)
+    std::vector<JobDev> dev;
+    bool any_host = false;
+};
+
+int prepare(const icbc_bc1_job* jobs, size_t n_jobs, hipStream_t s, int decoder, Prepared& p) {
+    if (n_jobs && !jobs) return fail(ICBC_ERR_INVALID_ARGUMENT, "null job array");
+    for (size_t k = 0; k < n_jobs; k++) {
+        int r = validate(jobs[k], k);
+        if (r) return r;
+    }
+    p.jobs.resize(n_jobs);
+    p.staged.resize(n_jobs);
+    p.dev.resize(n_jobs);
+    for (size_t k = 0; k < n_jobs; k++) {
+        if (jobs[k].flags & ICBC_JOB_HOST_MEMORY) {
+            p.any_host = true;
+            int r = stage_in(jobs[k], s, p.staged[k], p.jobs[k]);
+            if (r) return r;
+        } else {
+            p.jobs[k] = jobs[k];
+        }
+        p.dev[k] = to_dev(p.jobs[k], decoder);
+    }
+    return ICBC_OK;
+}
+
+int finish(const icbc_bc1_job* jobs, size_t n_jobs, hipStream_t s, Prepared& p) {
+    if (!p.any_host) return ICBC_OK;
+    for (size_t k = 0; k < n_jobs; k++)
+        if (jobs[k].flags & ICBC_JOB_HOST_MEMORY) {
+            int r = stage_out(jobs[k], s, p.staged[k]);
+            if (r) return r;
+        }
+    HIP_TRY(hipStreamSynchronize(s));
+    return ICBC_OK;
+}
+
+std::vector<uint2> make_tiles(const std::vector<JobDev>& dev) {
+    std::vector<uint2> tiles;
+    size_t total = 0;
+    for (const JobDev& d : dev) total += tiles_of(d.block_count);
+    tiles.reserve(total);
+    for (size_t k = 0; k < dev.size(); k++)
+        for (uint32_t t = 0; t < tiles_of(dev[k].block_count); t++)
+            tiles.push_back(make_uint2((uint32_t)k, t * (uint32_t)kTile));
+    return tiles;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+
+extern "C" {
+
+int icbc_gpu_init(int decoder) {
+    if (decoder < 0 || decoder > 3) return fail(ICBC_ERR_INVALID_ARGUMENT, "decoder %d", decoder);
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        if (!g_tables_built) {
+            build_tables(g_tables);
+            g_tables_built = true;
+        }
+        g_decoder = decoder;
+    }
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(ICBC_ERR_NO_DEVICE, "no HIP device visible");
+    int d;
+    return ensure_device_ready(&d);
+}
+
+int icbc_gpu_compress_bc1_batch(const icbc_bc1_job* jobs, size_t n_jobs, void* stream) {
+    int decoder;
+    int r = ensure_device_ready(&decoder);
+    if (r) return r;
+    hipStream_t s = (hipStream_t)stream;
+    Prepared p;
+    r = prepare(jobs, n_jobs, s, decoder, p);
+    if (r) return r;
+    std::vector<uint2> tiles = make_tiles(p.dev);
+    if (tiles.size() > 0xffffffffu) return fail(ICBC_ERR_INVALID_ARGUMENT, "batch too large");
+    if (!tiles.empty()) {
+        void *djobs = nullptr, *dtiles = nullptr;
+        r = upload_blob(p.dev.data(), p.dev.size() * sizeof(JobDev), s, &djobs);
+        if (r) return r;
+        r = upload_blob(tiles.data(), tiles.size() * sizeof(uint2), s, &dtiles);
+        if (r) return r;
+        HIP_TRY(launch_tiles((const JobDev*)djobs, (const uint2*)dtiles, (uint32_t)tiles.size(), s));
+        HIP_TRY(hipFreeAsync(djobs, s));
+        HIP_TRY(hipFreeAsync(dtiles, s));
+    }
+    return finish(jobs, n_jobs, s, p);
+}
+
+int icbc_gpu_compress_bc1_plain(const icbc_bc1_job* jobs, size_t n_jobs, void* stream) {
+    int decoder;
+    int r = ensure_device_ready(&decoder);
+    if (r) return r;
+    hipStream_t s = (hipStream_t)stream;
+    Prepared p;
+    r = prepare(jobs, n_jobs, s, decoder, p);
+    if (r) return r;
+    if (n_jobs) {
+        void* djobs = nullptr;
+        r = upload_blob(p.dev.data(), p.dev.size() * sizeof(JobDev), s, &djobs);
+        if (r) return r;
+        for (size_t k = 0; k < n_jobs; k++)
+            HIP_TRY(launch_job((const JobDev*)djobs + k, tiles_of(p.dev[k].block_count), s));
+        HIP_TRY(hipFreeAsync(djobs, s));
+    }
+    return finish(jobs, n_jobs, s, p);
+}
+
+}  // extern "C"
+
+// ---- plans ------------------------------------------------------------------
+
+struct icbc_bc1_plan {
+    int device = 0;
+    icbc_plan_options opt{};
+    JobDev* djobs = nullptr;
+    uint2* dtiles = nullptr;
+    uint32_t* dcounter = nullptr;
+    uint32_t ntiles = 0;
+    uint32_t grid = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// Relative cost of one tile at each level, used to issue expensive tiles
+// first (larger candidate searches and refinement at the top levels).
+float level_cost(int level) {
+    static const float cost[10] = {1, 2, 40, 40, 60, 60, 60, 60, 70, 110};
+    return (level >= 0 && level <= 9) ? cost[level] : 0.0f;
+}
+
+int plan_launch(icbc_bc1_plan* pl, hipStream_t s) {
+    if (pl->opt.schedule == ICBC_SCHEDULE_PERSISTENT)
+        HIP_TRY(launch_persistent(pl->djobs, pl->dtiles, pl->ntiles, pl->dcounter, pl->grid, s));
+    else
+        HIP_TRY(launch_tiles(pl->djobs, pl->dtiles, pl->ntiles, s));
+    return ICBC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_plan_options* opt,
+                         void* stream, icbc_bc1_plan** out) {
+    if (!out) return fail(ICBC_ERR_INVALID_ARGUMENT, "null plan pointer");
+    *out = nullptr;
+    int decoder;
+    int r = ensure_device_ready(&decoder);
+    if (r) return r;
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<JobDev> dev(n_jobs);
+    for (size_t k = 0; k < n_jobs; k++) {
+        r = validate(jobs[k], k);
+        if (r) return r;
+        if (jobs[k].flags & ICBC_JOB_HOST_MEMORY)
+            return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: plans take device memory only", k);
+        dev[k] = to_dev(jobs[k], decoder);
+    }
+    std::vector<uint2> tiles = make_tiles(dev);
+    if (tiles.size() > 0xffffffffu) return fail(ICBC_ERR_INVALID_ARGUMENT, "plan too large");
+    icbc_plan_options o{ICBC_SCHEDULE_PERSISTENT, 1, 0, 1};
+    if (opt) o = *opt;
+    if (o.order_by_cost) {
+        std::stable_sort(tiles.begin(), tiles.end(), [&](const uint2& a, const uint2& b) {
+            return level_cost(dev[a.x].level) > level_cost(dev[b.x].level);
+        });
+    }
+    icbc_bc1_plan* pl = new icbc_bc1_plan();
+    pl->opt = o;
+    pl->ntiles = (uint32_t)tiles.size();
+    (void)hipGetDevice(&pl->device);
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, pl->device);
+    if (e != hipSuccess) {
+        delete pl;
+        return hip_fail(e, "hipGetDeviceProperties");
+    }
+    const int per_cu = o.workgroups_per_cu > 0 ? o.workgroups_per_cu : 8;
+    pl->grid = (uint32_t)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * per_cu,
+                                            std::max<uint32_t>(pl->ntiles, 1u));
+    auto cleanup = [&](int code) {
+        icbc_gpu_plan_destroy(pl);
+        return code;
+    };
+    if ((e = hipMalloc(&pl->djobs, std::max<size_t>(1, dev.size()) * sizeof(JobDev))) != hipSuccess ||
+        (e = hipMalloc(&pl->dtiles, std::max<size_t>(1, tiles.size()) * sizeof(uint2))) != hipSuccess ||
+        (e = hipMalloc(&pl->dcounter, sizeof(uint32_t))) != hipSuccess)
+        return cleanup(hip_fail(e, "hipMalloc (plan)"));
+    if (!dev.empty() &&
+        (e = hipMemcpy(pl->djobs, dev.data(), dev.size() * sizeof(JobDev), hipMemcpyHostToDevice)) != hipSuccess)
+        return cleanup(hip_fail(e, "hipMemcpy (plan jobs)"));
+    if (!tiles.empty() &&
+        (e = hipMemcpy(pl->dtiles, tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice)) != hipSuccess)
+        return cleanup(hip_fail(e, "hipMemcpy (plan tiles)"));
+    if (o.use_graph && pl->ntiles) {
+        hipStream_t cs;
+        if ((e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)) != hipSuccess)
+            return cleanup(hip_fail(e, "hipStreamCreate"));
+        e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            r = plan_launch(pl, cs);
+            hipError_t e2 = hipStreamEndCapture(cs, &pl->graph);
+            if (r == ICBC_OK && e2 != hipSuccess) r = hip_fail(e2, "hipStreamEndCapture");
+            if (r == ICBC_OK && (e2 = hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0)) != hipSuccess)
+                r = hip_fail(e2, "hipGraphInstantiate");
+        } else {
+            r = hip_fail(e, "hipStreamBeginCapture");
+        }
+        (void)hipStreamDestroy(cs);
+        if (r) return cleanup(r);
+    }
+    (void)s;
+    *out = pl;
+    return ICBC_OK;
+}
+
+int icbc_gpu_plan_execute(icbc_bc1_plan* pl, void* stream) {
+    if (!pl) return fail(ICBC_ERR_INVALID_ARGUMENT, "null plan");
+    int decoder;
+    int r = ensure_device_ready(&decoder);
+    if (r) return r;
+    hipStream_t s = (hipStream_t)stream;
+    if (pl->ntiles == 0) return ICBC_OK;
+    if (pl->exec) {
+        HIP_TRY(hipGraphLaunch(pl->exec, s));
+        return ICBC_OK;
+    }
+    return plan_launch(pl, s);
+}
+
+int icbc_gpu_plan_destroy(icbc_bc1_plan* pl) {
+    if (!pl) return ICBC_OK;
+    if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    if (pl->djobs) (void)hipFree(pl->djobs);
+    if (pl->dtiles) (void)hipFree(pl->dtiles);
+    if (pl->dcounter) (void)hipFree(pl->dcounter);
+    delete pl;
+    return ICBC_OK;
+}
+
+uint64_t icbc_gpu_plan_tiles(const icbc_bc1_plan* pl) { return pl ? pl->ntiles : 0; }
+
+}  // extern "C"
+
+// ---- single block -----------------------------------------------------------
+
+namespace {
+
+// Per-thread resources for single-block calls (the reference's compress_bc1
+// is reentrant; each calling thread gets its own stream and buffers).
+struct BlockContext {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    void* dbuf = nullptr;    // colors 256 | weights 64 | code 8 | error 4
+    void* hbuf = nullptr;    // pinned mirror
+    ~BlockContext() {
+        // process teardown may already have destroyed the runtime; leak instead
+    }
+};
+thread_local BlockContext t_block;
+
+int block_context(BlockContext*& ctx) {
+    ctx = &t_block;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (ctx->device != dev) {
+        ctx->device = dev;
+        HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        HIP_TRY(hipMalloc(&ctx->dbuf, 512));
+        HIP_TRY(hipHostMalloc(&ctx->hbuf, 512, hipHostMallocDefault));
+    }
+    return ICBC_OK;
+}
+
+// One block through the same kernel as the batch path; status code out.
+int compress_one(int level, const float* input_colors, const float* input_weights,
+                 const float color_weights[3], int three_color_mode, int three_color_black,
+                 void* output, float* error) {
+    if (!input_colors || !input_weights || !color_weights || !output)
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "null argument");
+    int decoder;
+    int r = ensure_device_ready(&decoder);
+    if (r) return r;
+    BlockContext* ctx;
+    r = block_context(ctx);
+    if (r) return r;
+    char* h = (char*)ctx->hbuf;
+    char* d = (char*)ctx->dbuf;
+    memcpy(h, input_colors, 256);
+    memcpy(h + 256, input_weights, 64);
+    memcpy(h + 320, output, 8);
+    icbc_bc1_job j;
+    memset(&j, 0, sizeof j);
+    j.level = level;
+    j.three_color_mode = three_color_mode;
+    j.three_color_black = three_color_black;
+    j.color_weights[0] = color_weights[0];
+    j.color_weights[1] = color_weights[1];
+    j.color_weights[2] = color_weights[2];
+    j.block_count = 1;
+    j.colors = (const float*)d;
+    j.weights = (const float*)(d + 256);
+    j.codes = d + 320;
+    j.errors = (float*)(d + 336);
+    JobDev jd = to_dev(j, decoder);
+    static_assert(sizeof(JobDev) <= 128, "job descriptor must fit the block buffer");
+    memcpy(h + 384, &jd, sizeof jd);
+    HIP_TRY(hipMemcpyAsync(d, h, 512, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_job((const JobDev*)(d + 384), 1, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(h + 320, d + 320, 32, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    memcpy(output, h + 320, 8);
+    memcpy(error, h + 336, 4);
+    return ICBC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+float icbc_gpu_compress_bc1(int level, const float* input_colors, const float* input_weights,
+                            const float color_weights[3], int three_color_mode, int three_color_black,
+                            void* output) {
+    float e = NAN;
+    if (compress_one(level, input_colors, input_weights, color_weights, three_color_mode,
+                     three_color_black, output, &e) != ICBC_OK)
+        return NAN;
+    return e;
+}
+
+// ---- decode / evaluate --------------------------------------------------------
+
+int icbc_gpu_decode_bc1_batch(const void* codes, uint64_t block_count, uint8_t* rgba, int decoder,
+                              void* stream) {
+    if (decoder < 0 || decoder > 3) return fail(ICBC_ERR_INVALID_ARGUMENT, "decoder %d", decoder);
+    if (block_count && (!codes || !rgba)) return fail(ICBC_ERR_INVALID_ARGUMENT, "null buffer");
+    if (((uintptr_t)codes & 7) || ((uintptr_t)rgba & 15))
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "codes must be 8-byte and rgba 16-byte aligned");
+    int d;
+    int r = ensure_device_ready(&d);
+    if (r) return r;
+    HIP_TRY(launch_decode((const uint2*)codes, block_count, (uint32_t*)rgba, decoder, (hipStream_t)stream));
+    return ICBC_OK;
+}
+
+int icbc_gpu_evaluate_bc1_error_batch(const uint8_t* rgba, const void* codes, uint64_t block_count,
+                                      float* errors, int decoder, void* stream) {
+    if (decoder < 0 || decoder > 3) return fail(ICBC_ERR_INVALID_ARGUMENT, "decoder %d", decoder);
+    if (block_count && (!codes || !rgba || !errors)) return fail(ICBC_ERR_INVALID_ARGUMENT, "null buffer");
+    if (((uintptr_t)codes & 7) || ((uintptr_t)rgba & 15) || ((uintptr_t)errors & 3))
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "misaligned buffer");
+    int d;
+    int r = ensure_device_ready(&d);
+    if (r) return r;
+    HIP_TRY(launch_evaluate((const uint32_t*)rgba, (const uint2*)codes, block_count, errors, decoder,
+                            (hipStream_t)stream));
+    return ICBC_OK;
+}
+
+// ---- diagnostics / plumbing ----------------------------------------------------
+
+const char* icbc_gpu_last_error(void) { return t_last_error.c_str(); }
+
+int icbc_gpu_device_name(char* buf, size_t len) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    return ICBC_OK;
+}
+
+int icbc_gpu_version(void) { return 10500; }
+
+void* icbc_gpu_malloc(size_t bytes) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        hip_fail(e, "hipMalloc");
+        return nullptr;
+    }
+    return p;
+}
+
+int icbc_gpu_free(void* ptr) {
+    HIP_TRY(hipFree(ptr));
+    return ICBC_OK;
+}
+
+int icbc_gpu_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return ICBC_OK;
+}
+
+int icbc_gpu_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return ICBC_OK;
+}
+
+int icbc_gpu_memset(void* dst, int value, size_t bytes) {
+    HIP_TRY(hipMemset(dst, value, bytes));
+    return ICBC_OK;
+}
+
+void* icbc_gpu_stream_create(void) {
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        hip_fail(e, "hipStreamCreate");
+        return nullptr;
+    }
+    return (void*)s;
+}
+
+int icbc_gpu_stream_destroy(void* stream) {
+    HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+    return ICBC_OK;
+}
+
+int icbc_gpu_stream_synchronize(void* stream) {
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return ICBC_OK;
+}
+
+void* icbc_gpu_event_create(void) {
+    hipEvent_t ev = nullptr;
+    hipError_t e = hipEventCreate(&ev);
+    if (e != hipSuccess) {
+        hip_fail(e, "hipEventCreate");
+        return nullptr;
+    }
+    return (void*)ev;
+}
+
+int icbc_gpu_event_destroy(void* event) {
+    HIP_TRY(hipEventDestroy((hipEvent_t)event));
+    return ICBC_OK;
+}
+
+int icbc_gpu_event_record(void* event, void* stream) {
+    HIP_TRY(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return ICBC_OK;
+}
+
+int icbc_gpu_event_synchronize(void* event) {
+    HIP_TRY(hipEventSynchronize((hipEvent_t)event));
+    return ICBC_OK;
+}
+
+float icbc_gpu_event_elapsed_ms(void* start, void* stop) {
+    float ms = NAN;
+    hipError_t e = hipEventElapsedTime(&ms, (hipEvent_t)start, (hipEvent_t)stop);
+    if (e != hipSuccess) {
+        hip_fail(e, "hipEventElapsedTime");
+        return NAN;
+    }
+    return ms;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Drop-in C++ API (include/icbc.h)
+
+namespace icbc {
+
+static void die(const char* what) {
+    fprintf(stderr, "icbc (MI355X): %s failed: %s\n", what, icbc_gpu_last_error());
+    abort();
+}
+
+void init(Decoder decoder) {
+    if (icbc_gpu_init((int)decoder) != ICBC_OK) die("init");
+}
+
+float compress_bc1(Quality level, const float* input_colors, const float* input_weights,
+                   const float color_weights[3], bool three_color_mode, bool three_color_black,
+                   void* output) {
+    float e = 0.0f;
+    if (compress_one((int)level, input_colors, input_weights, color_weights, three_color_mode ? 1 : 0,
+                     three_color_black ? 1 : 0, output, &e) != ICBC_OK)
+        die("compress_bc1");
+    return e;
+}
+
+float compress_bc3(Quality, const float*, const float*, const float[3], bool, void*) {
+    return 0.0f;  // the reference's BC3 encoder is a stub (icbc.h:3850-3853)
+}
+
+void decode_bc1(const void* block, unsigned char rgba_block[16 * 4], Decoder decoder) {
+    void* d = icbc_gpu_malloc(128);
+    if (!d) die("decode_bc1");
+    if (icbc_gpu_memcpy_htod(d, block, 8) || icbc_gpu_decode_bc1_batch(d, 1, (uint8_t*)d + 64, (int)decoder, nullptr) ||
+        icbc_gpu_memcpy_dtoh(rgba_block, (uint8_t*)d + 64, 64))
+        die("decode_bc1");
+    icbc_gpu_free(d);
+}
+
+float evaluate_bc1_error(const unsigned char rgba_block[16 * 4], const void* block, Decoder decoder) {
+    void* d = icbc_gpu_malloc(128);
+    if (!d) die("evaluate_bc1_error");
+    float e = 0.0f;
+    if (icbc_gpu_memcpy_htod(d, rgba_block, 64) || icbc_gpu_memcpy_htod((uint8_t*)d + 64, block, 8) ||
+        icbc_gpu_evaluate_bc1_error_batch((const uint8_t*)d, (uint8_t*)d + 64, 1, (float*)((uint8_t*)d + 80),
+                                          (int)decoder, nullptr) ||
+        icbc_gpu_memcpy_dtoh(&e, (uint8_t*)d + 80, 4))
+        die("evaluate_bc1_error");
+    icbc_gpu_free(d);
+    return e;
+}
+
+}  // namespace icbc

@@ -1,0 +1,138 @@
+/* icbc_gpu.h -- C ABI of the MI355X (gfx950) BC1 encoder.
+ *
+ * Plain C types only (no HIP or torch types in the signatures), so the library
+ * can be bound from C, ctypes, cgo or JNI.  HIP streams are passed as
+ * `void*` (a hipStream_t, NULL = the default stream).
+ *
+ * The encoder reproduces castano/icbc v1.05 (icbc.h) bit for bit: the 8-byte
+ * block codes equal those of the reference's scalar build (ICBC_SIMD=0) and the
+ * returned per-block errors are the same floats.
+ */
+#ifndef ICBC_GPU_H
+#define ICBC_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define ICBC_OK 0
+#define ICBC_ERR_INVALID_ARGUMENT (-1)
+#define ICBC_ERR_NOT_INITIALIZED (-2)
+#define ICBC_ERR_OUT_OF_MEMORY (-3)
+#define ICBC_ERR_DEVICE (-4)     /* a HIP runtime call failed */
+#define ICBC_ERR_NO_DEVICE (-5)  /* no usable gfx950 device */
+#define ICBC_ERR_BUDGET (-6)     /* serving: request larger than the memory budget */
+#define ICBC_ERR_CANCELLED (-7)  /* serving: request cancelled before it ran */
+
+/* ---- decoders and quality levels (icbc.h:12-17, 21-36) ----------------- */
+#define ICBC_DECODER_D3D10 0
+#define ICBC_DECODER_NVIDIA 1
+#define ICBC_DECODER_AMD 2
+#define ICBC_DECODER_INTEL 3
+
+/* ---- one encode job: a run of blocks sharing level and options ---------- */
+#define ICBC_JOB_HOST_MEMORY 1u /* colors/weights/codes/errors/stats are host pointers */
+
+typedef struct icbc_bc1_job {
+    int32_t level;             /* icbc::Quality 0..9; others give error FLT_MAX, codes untouched */
+    int32_t three_color_mode;  /* bool */
+    int32_t three_color_black; /* bool */
+    uint32_t flags;            /* ICBC_JOB_* */
+    float color_weights[3];    /* per-channel metric weights */
+    uint32_t reserved;         /* must be 0 */
+    uint64_t block_count;
+    const float* colors;       /* 64 floats per block: 16 texels x RGBA, texel = 4*y + x;
+                                  16-byte aligned */
+    const float* weights;      /* 16 floats per block (16-byte aligned), NULL = all 1.0 */
+    void* codes;               /* 8 bytes per block: u16 col0, u16 col1, u32 indices */
+    float* errors;             /* one float per block (the reference's return value), or NULL */
+    int32_t* stats;            /* optional, 4 int32 per block: reduced colour count n,
+                                  colours given to the 3-cluster fit (-1 = not run),
+                                  refinement steps, 0 */
+} icbc_bc1_job;
+
+/* Replaces icbc::init (icbc.h:19, 3813): selects the hardware decoder whose
+ * palette the encoder targets and uploads the encoder's tables to the current
+ * device.  Not thread-safe against concurrent compress calls, like the
+ * reference. */
+int icbc_gpu_init(int decoder);
+
+/* Replaces a loop of icbc::compress_bc1 (icbc.h:48, 3846) over many blocks.
+ * All jobs are encoded by one kernel launch (one 64-block tile per
+ * workgroup).  Device-memory jobs are enqueued on `stream` and the call
+ * returns without waiting; host-memory jobs are staged, and the call waits
+ * for their results. */
+int icbc_gpu_compress_bc1_batch(const icbc_bc1_job* jobs, size_t n_jobs, void* stream);
+
+/* Same results as icbc_gpu_compress_bc1_batch, one kernel launch per job
+ * (the plain comparison path). */
+int icbc_gpu_compress_bc1_plain(const icbc_bc1_job* jobs, size_t n_jobs, void* stream);
+
+/* One block, host memory: the reference's compress_bc1 signature over C types.
+ * Returns the block error (FLT_MAX for an invalid level, output untouched);
+ * returns NaN and sets icbc_gpu_last_error() if the device call fails. */
+float icbc_gpu_compress_bc1(int level, const float* input_colors, const float* input_weights,
+                            const float color_weights[3], int three_color_mode,
+                            int three_color_black, void* output);
+
+/* ---- persistent batched plans (many textures, one launch) -------------- */
+typedef struct icbc_bc1_plan icbc_bc1_plan;
+
+#define ICBC_SCHEDULE_GRID 0       /* one workgroup per tile */
+#define ICBC_SCHEDULE_PERSISTENT 1 /* resident workgroups pull tiles from a queue */
+
+typedef struct icbc_plan_options {
+    int32_t schedule;          /* ICBC_SCHEDULE_* */
+    int32_t use_graph;         /* capture the launch in a hipGraph, replay on execute */
+    int32_t workgroups_per_cu; /* persistent grid size per compute unit, 0 = automatic */
+    int32_t order_by_cost;     /* issue expensive tiles first (estimated from level) */
+} icbc_plan_options;
+
+/* Builds a plan over device-memory jobs: job table, tile list and (optionally)
+ * a captured graph, all resident on the device.  Buffers named by the jobs
+ * must stay valid while the plan exists. */
+int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_plan_options* opt,
+                         void* stream, icbc_bc1_plan** plan);
+/* Runs the plan on `stream` (asynchronous). */
+int icbc_gpu_plan_execute(icbc_bc1_plan* plan, void* stream);
+int icbc_gpu_plan_destroy(icbc_bc1_plan* plan);
+/* Number of 64-block tiles in the plan. */
+uint64_t icbc_gpu_plan_tiles(const icbc_bc1_plan* plan);
+
+/* ---- decode / evaluate (icbc.h:38, 43, 2915-2927, 3030-3046) ----------- */
+/* Device memory: codes 8 B/block, rgba 64 B/block (RGBA8), errors 1 float/block. */
+int icbc_gpu_decode_bc1_batch(const void* codes, uint64_t block_count, uint8_t* rgba, int decoder,
+                              void* stream);
+int icbc_gpu_evaluate_bc1_error_batch(const uint8_t* rgba, const void* codes,
+                                      uint64_t block_count, float* errors, int decoder,
+                                      void* stream);
+
+/* ---- diagnostics -------------------------------------------------------- */
+const char* icbc_gpu_last_error(void); /* text of this thread's last failure */
+int icbc_gpu_device_name(char* buf, size_t len);
+int icbc_gpu_version(void); /* 10500 + build revision, mirrors icbc v1.05 */
+
+/* ---- device plumbing for callers without their own HIP runtime ---------- */
+void* icbc_gpu_malloc(size_t bytes);
+int icbc_gpu_free(void* ptr);
+int icbc_gpu_memcpy_htod(void* dst, const void* src, size_t bytes);
+int icbc_gpu_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int icbc_gpu_memset(void* dst, int value, size_t bytes);
+void* icbc_gpu_stream_create(void);
+int icbc_gpu_stream_destroy(void* stream);
+int icbc_gpu_stream_synchronize(void* stream);
+void* icbc_gpu_event_create(void);
+int icbc_gpu_event_destroy(void* event);
+int icbc_gpu_event_record(void* event, void* stream);
+int icbc_gpu_event_synchronize(void* event);
+float icbc_gpu_event_elapsed_ms(void* start, void* stop);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ICBC_GPU_H */

@@ -1,0 +1,118 @@
+"""Writes tests/golden/bc1_golden.npz from the reference encoder.
+
+The reference (castano/icbc v1.05) has no tests or fixtures of its own, so the
+vectors are produced here by the reference header itself, compiled at
+ICBC_SIMD=0 by oracle/Makefile into oracle/_ref/libicbc_ref_scalar.so, and
+called through its public API (icbc.h:19, 38, 43, 48).  Run in a checkout
+where /root/reference exists:
+
+    make -C oracle && python tests/golden/gen_golden.py
+
+Contents (all inputs are seeded synthetic blocks, block-major 16 x RGBA):
+  colors, weights           the input blocks (edge cases, image content, weighted)
+  prior                     the output buffer before each call (to check that
+                            invalid levels leave it untouched)
+  cfg                       (decoder, level, three_color_mode, three_color_black, cw_index)
+  cw                        channel-weight sets indexed by cw_index
+  codes, errors             reference outputs per config
+  tables_*, midpoints*      the reference's init tables (decoder 0)
+  dec_codes, dec_rgba       decode_bc1 outputs for every decoder
+  eval_rgba, eval_err       evaluate_bc1_error outputs for every decoder
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from icbc_amd import synth  # noqa: E402
+from oracle import Reference  # noqa: E402
+
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "bc1_golden.npz")
+
+
+def inputs():
+    parts_c, parts_w = [], []
+    c, _ = synth.make_image_blocks(101, 64, 48, synth.MIX_DEFAULT)      # 192 blocks
+    parts_c.append(c)
+    parts_w.append(np.ones((len(c), 16), np.float32))
+    c, _ = synth.make_image_blocks(102, 32, 32, synth.MIX_PLAIN)        # 64 blocks
+    parts_c.append(c)
+    parts_w.append(np.ones((len(c), 16), np.float32))
+    c, w = synth.make_edge_blocks(103, 192)                             # 192 blocks
+    parts_c.append(c)
+    parts_w.append(w)
+    c, _ = synth.make_image_blocks(104, 32, 32, synth.MIX_DEFAULT)      # 64 weighted blocks
+    alpha = synth.make_weighted_alpha(104, 32, 32)
+    a = alpha.reshape(8, 4, 8, 4).transpose(0, 2, 1, 3).reshape(-1, 16)
+    c[..., 3] = a
+    parts_c.append(c)
+    parts_w.append(a.astype(np.float32))
+    return np.concatenate(parts_c), np.concatenate(parts_w)
+
+
+def main():
+    ref = Reference("scalar")
+    colors, weights = inputs()
+    n = len(colors)
+    cws = np.array([[1, 1, 1], [0.3, 0.59, 0.11]], np.float32)
+    configs = []
+    for level in range(10):
+        configs.append((0, level, 1, 1, 0))
+        configs.append((0, level, 0, 0, 0))
+    for level in (2, 5, 8, 9):
+        configs.append((0, level, 1, 0, 0))
+    for level in (1, 5, 8, 9):
+        configs.append((0, level, 1, 1, 1))
+    configs.append((0, 10, 1, 1, 0))   # invalid level: FLT_MAX, output untouched
+    configs.append((0, -1, 0, 0, 0))
+    for dec in (1, 2, 3):
+        for level in (1, 5, 8, 9):
+            configs.append((dec, level, 1, 1, 0))
+
+    rng = np.random.default_rng(105)
+    prior = rng.integers(0, 256, size=(n, 8), dtype=np.uint8)
+    codes = np.zeros((len(configs), n, 8), np.uint8)
+    errors = np.zeros((len(configs), n), np.float32)
+    for k, (dec, level, tcm, tcb, cwi) in enumerate(configs):
+        ref.init(dec)
+        codes[k], errors[k], _ = ref.compress_many(level, colors, weights, cws[cwi], tcm, tcb,
+                                                   prior=prior)
+
+    ref.init(0)
+    t = ref.tables()
+    m5, m6 = ref.midpoints()
+
+    # decode / evaluate on a slice of the level-8 codes, for every decoder
+    k8 = configs.index((0, 8, 1, 1, 0))
+    dec_codes = codes[k8, :128]
+    eval_rgba = np.clip(np.round(colors[:128] * 255), 0, 255).astype(np.uint8).reshape(128, 64)
+    dec_rgba = np.zeros((4, 128, 64), np.uint8)
+    eval_err = np.zeros((4, 128), np.float32)
+    for dec in range(4):
+        for i in range(128):
+            dec_rgba[dec, i] = ref.decode(dec_codes[i], dec)
+            eval_err[dec, i] = ref.evaluate_error(eval_rgba[i], dec_codes[i], dec)
+
+    match5 = np.zeros((4, 256, 2), np.uint8)
+    match6 = np.zeros((4, 256, 2), np.uint8)
+    dweights = np.zeros((4, 5), np.float32)
+    for dec in range(4):
+        ref.init(dec)
+        td = ref.tables()
+        match5[dec], match6[dec], dweights[dec] = td["match5"], td["match6"], td["weights"]
+
+    np.savez_compressed(
+        OUT, colors=colors, weights=weights, prior=prior, cfg=np.array(configs, np.int32),
+        cw=cws, codes=codes, errors=errors, tables_four=t["four"], tables_three=t["three"],
+        tables_four_total=t["four_total"], tables_three_total=t["three_total"],
+        tables_match5=match5, tables_match6=match6, tables_weights=dweights,
+        midpoints5=m5, midpoints6=m6, dec_codes=dec_codes, dec_rgba=dec_rgba,
+        eval_rgba=eval_rgba, eval_err=eval_err)
+    print(f"wrote {OUT}: {n} blocks x {len(configs)} configs, {os.path.getsize(OUT)} bytes")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,81 @@
+"""CPU tests of the C ABI library: it loads, exports every entry point that
+include/icbc_gpu.h and include/icbc.h declare, and rejects bad arguments
+before touching a device.  No encode runs here (no GPU needed)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import icbc_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_c_functions():
+    text = open(os.path.join(ROOT, "include", "icbc_gpu.h")).read()
+    return sorted(set(re.findall(r"\b(icbc_gpu_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_c_function():
+    lib = ctypes.CDLL(icbc_amd.LIB_PATH)
+    names = _declared_c_functions()
+    assert len(names) >= 25
+    for name in names:
+        assert hasattr(lib, name), f"{name} declared in icbc_gpu.h but not exported"
+    assert set(names) == set(icbc_amd._SIGNATURES), "python binding out of sync with the header"
+
+
+def test_library_exports_reference_cpp_api():
+    out = subprocess.check_output(["nm", "-D", "--defined-only", icbc_amd.LIB_PATH], text=True)
+    # mangled names of the drop-in namespace icbc API (include/icbc.h)
+    for sym in ("_ZN4icbc4initENS_7DecoderE",
+                "_ZN4icbc12compress_bc1ENS_7QualityEPKfS2_S2_bbPv",
+                "_ZN4icbc12compress_bc3ENS_7QualityEPKfS2_S2_bPv",
+                "_ZN4icbc10decode_bc1EPKvPhNS_7DecoderE",
+                "_ZN4icbc18evaluate_bc1_errorEPKhPKvNS_7DecoderE"):
+        assert sym in out, sym
+
+
+def test_no_internal_c_symbols_leak():
+    out = subprocess.check_output(["nm", "-D", "--defined-only", icbc_amd.LIB_PATH], text=True)
+    plain = [l.split()[-1] for l in out.splitlines() if " T " in l]
+    for s in plain:
+        assert s.startswith("icbc_gpu_") or s.startswith("_Z"), s
+
+
+def test_library_links_hip_runtime_and_has_gfx950_code():
+    out = subprocess.check_output(["ldd", icbc_amd.LIB_PATH], text=True)
+    assert "libamdhip64" in out
+    data = open(icbc_amd.LIB_PATH, "rb").read()
+    assert b"gfx950" in data
+
+
+def test_invalid_arguments_are_rejected_without_device():
+    lib = icbc_amd.lib()
+    assert lib.icbc_gpu_init(7) == icbc_amd.ERR_INVALID_ARGUMENT
+    assert "decoder" in icbc_amd.last_error()
+    assert lib.icbc_gpu_decode_bc1_batch(None, 0, None, 9, None) == icbc_amd.ERR_INVALID_ARGUMENT
+    assert lib.icbc_gpu_evaluate_bc1_error_batch(None, None, 0, None, -1, None) == icbc_amd.ERR_INVALID_ARGUMENT
+    assert lib.icbc_gpu_version() == 10500
+
+
+def test_job_struct_layout_matches_header():
+    # icbc_bc1_job: 4 int32, 3 float, 1 uint32, uint64, 5 pointers
+    assert ctypes.sizeof(icbc_amd.Job) == 16 + 16 + 8 + 5 * 8
+    assert icbc_amd.Job.block_count.offset == 32
+    assert icbc_amd.Job.colors.offset == 40
+    assert ctypes.sizeof(icbc_amd.PlanOptions) == 16
+
+
+def test_c_header_compiles_as_c_and_cpp_header_as_cpp(tmp_path):
+    src = tmp_path / "t.c"
+    src.write_text('#include "icbc_gpu.h"\nint main(void){icbc_bc1_job j; (void)j; return 0;}\n')
+    subprocess.check_call(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                           "-c", str(src), "-o", str(tmp_path / "t.o")])
+    cpp = tmp_path / "t.cpp"
+    cpp.write_text('#include "icbc.h"\nint main(){float c[64]={0},w[16]={0},cw[3]={1,1,1};'
+                   'unsigned char o[8];(void)&icbc::compress_bc1;(void)c;(void)w;(void)cw;(void)o;return 0;}\n')
+    subprocess.check_call(["g++", "-std=c++11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                           "-c", str(cpp), "-o", str(tmp_path / "t2.o")])

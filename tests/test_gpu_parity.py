@@ -1,0 +1,237 @@
+"""GPU parity: the HIP encoder against the golden vectors (written by the
+reference encoder) and against the oracle on seeded synthetic workloads.
+
+Bar: 8-byte block codes bit-identical; per-block errors bit-identical as well
+(the stated tolerance is 1e-6 relative, checked first so a failure reports the
+size of a discrepancy; the bit-level check follows).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from icbc_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-6
+NTHREADS = min(16, os.cpu_count() or 1)
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def assert_same(codes, errs, want_codes, want_errs, what):
+    bad = np.nonzero((codes != want_codes).any(axis=1))[0]
+    assert len(bad) == 0, f"{what}: {len(bad)} of {len(codes)} codes differ, first {bad[:8]}"
+    a = errs.astype(np.float64)
+    b = want_errs.astype(np.float64)
+    both_nan = np.isnan(a) & np.isnan(b)
+    rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
+    rel[both_nan | (a == b)] = 0
+    assert np.all(rel <= REL_TOL), f"{what}: error mismatch, worst rel {rel.max()} at {int(np.argmax(rel))}"
+    eq = (errs.view(np.uint32) == want_errs.view(np.uint32)) | both_nan
+    assert np.all(eq), f"{what}: {int((~eq).sum())} errors not bit-identical"
+
+
+def test_golden_vectors(gpu, golden):
+    colors, weights, prior = golden["colors"], golden["weights"], golden["prior"]
+    try:
+        for k, (dec, level, tcm, tcb, cwi) in enumerate(golden["cfg"]):
+            gpu.init(int(dec))
+            codes, errs, _ = gpu.compress_many(int(level), colors, weights, golden["cw"][cwi], tcm, tcb,
+                                               prior=prior)
+            assert_same(codes, errs, golden["codes"][k], golden["errors"][k],
+                        f"golden config {k} {tuple(int(x) for x in golden['cfg'][k])}")
+    finally:
+        gpu.init(0)
+
+
+def _oracle_run(oracle, level, colors, weights, cw, tcm, tcb, prior=None):
+    codes, errs, stats, _ = oracle.compress_many(level, colors, weights, cw, tcm, tcb,
+                                                 nthreads=NTHREADS, prior=prior, with_stats=True)
+    return codes, errs, stats
+
+
+@pytest.mark.parametrize("level", list(range(-1, 11)))
+def test_edge_blocks_every_level(gpu, oracle, level):
+    colors, weights = synth.make_edge_blocks(300 + max(level, 0), 1024)
+    prior = np.random.default_rng(level + 5).integers(0, 256, size=(1024, 8), dtype=np.uint8)
+    for tcm, tcb in ((0, 0), (1, 0), (1, 1)):
+        for cw in ((1, 1, 1), (0.3, 0.59, 0.11)):
+            want = _oracle_run(oracle, level, colors, weights, cw, tcm, tcb, prior)
+            got = gpu.compress_many(level, colors, weights, cw, tcm, tcb, prior=prior, with_stats=True)
+            assert_same(got[0], got[1], want[0], want[1], f"edge L{level} tcm={tcm} tcb={tcb} cw={cw}")
+            assert np.array_equal(got[2][:, :3], want[2][:, :3]), "stats differ"
+
+
+def test_plain_image_level8(gpu, oracle):
+    # 512x512, 50% gradients / 30% value noise / 20% flat, L8, three_color_mode off
+    colors, _ = synth.make_image_blocks(1, 512, 512, synth.MIX_PLAIN)
+    _say(f"512x512 L8: {len(colors)} blocks")
+    want = _oracle_run(oracle, 8, colors, None, (1, 1, 1), 0, 0)
+    got = gpu.compress_many(8, colors, None, (1, 1, 1), False, False, with_stats=True)
+    assert_same(got[0], got[1], want[0], want[1], "512x512 L8")
+    assert np.array_equal(got[2][:, :3], want[2][:, :3])
+
+
+@pytest.mark.parametrize("level", [1, 5, 9])
+def test_mixed_image_levels(gpu, oracle, level):
+    colors, _ = synth.make_image_blocks(2, 256, 256, synth.MIX_DEFAULT)
+    want = _oracle_run(oracle, level, colors, None, (1, 1, 1), 1, 1)
+    got = gpu.compress_many(level, colors, None, (1, 1, 1), True, True, with_stats=True)
+    assert_same(got[0], got[1], want[0], want[1], f"256x256 mix L{level}")
+    assert np.array_equal(got[2][:, :3], want[2][:, :3])
+
+
+def test_weighted_alpha_level9(gpu, oracle):
+    colors, _ = synth.make_image_blocks(3, 256, 256, synth.MIX_DEFAULT)
+    alpha = synth.make_weighted_alpha(3, 256, 256)
+    a = alpha.reshape(64, 4, 64, 4).transpose(0, 2, 1, 3).reshape(-1, 16)
+    colors[..., 3] = a
+    cw = (0.3, 0.59, 0.11)
+    want = _oracle_run(oracle, 9, colors, a, cw, 1, 1)
+    got = gpu.compress_many(9, colors, a, cw, True, True, with_stats=True)
+    assert_same(got[0], got[1], want[0], want[1], "weighted L9")
+
+
+@pytest.mark.parametrize("decoder", [1, 2, 3])
+def test_other_decoders(gpu, oracle, decoder):
+    colors, _ = synth.make_image_blocks(4, 128, 128, synth.MIX_DEFAULT)
+    ec, ew = synth.make_edge_blocks(40 + decoder, 256)
+    colors = np.concatenate([colors, ec])
+    weights = np.concatenate([np.ones((1024, 16), np.float32), ew])
+    try:
+        gpu.init(decoder)
+        oracle.init(decoder)
+        for level in (0, 1, 2, 4, 8, 9):
+            want = _oracle_run(oracle, level, colors, weights, (1, 1, 1), 1, 1)
+            got = gpu.compress_many(level, colors, weights, (1, 1, 1), True, True)
+            assert_same(got[0], got[1], want[0], want[1], f"decoder {decoder} L{level}")
+    finally:
+        gpu.init(0)
+        oracle.init(0)
+
+
+def test_single_block_c_abi(gpu, oracle):
+    lib = gpu.lib()
+    colors, weights = synth.make_edge_blocks(77, 48)
+    for level in (0, 1, 3, 8, 9, 12):
+        for b in range(len(colors)):
+            c = np.ascontiguousarray(colors[b].reshape(64))
+            w = np.ascontiguousarray(weights[b])
+            cw = np.array([1, 1, 1], np.float32)
+            out = np.full(8, 0x3c, np.uint8)
+            e = lib.icbc_gpu_compress_bc1(level, c.ctypes.data, w.ctypes.data, cw.ctypes.data, 1, 1,
+                                          out.ctypes.data)
+            want_c, want_e = oracle.compress_block(level, c, w, cw, True, True, prior=np.full(8, 0x3c, np.uint8))
+            assert np.array_equal(out, want_c)
+            assert np.float32(e).view(np.uint32) == np.float32(want_e).view(np.uint32) or (np.isnan(e) and np.isnan(want_e))
+            # the python mirror of icbc::compress_bc1
+            out2 = np.full(8, 0x3c, np.uint8)
+            e2 = gpu.compress_bc1(level, c, w, cw, True, True, out2)
+            assert np.array_equal(out2, want_c) and (np.float32(e2) == np.float32(want_e) or np.isnan(e2))
+
+
+def test_batch_plain_and_plans_agree(gpu, oracle):
+    # several jobs with different levels, sizes and options, on device memory
+    specs = [(8, 700, (1, 1, 1), 1, 1), (1, 64, (1, 1, 1), 0, 0), (9, 333, (0.3, 0.59, 0.11), 1, 1),
+             (5, 1000, (1, 1, 1), 1, 1), (0, 1, (1, 1, 1), 0, 0), (3, 129, (1, 2, 1), 1, 0),
+             (12, 50, (1, 1, 1), 1, 1), (2, 0, (1, 1, 1), 1, 1)]
+    host = []
+    for k, (level, n, cw, tcm, tcb) in enumerate(specs):
+        c, _ = synth.make_image_blocks(500 + k, 4 * max(n, 1), 4, synth.MIX_DEFAULT)
+        c = c[:n]
+        w = np.random.default_rng(k).random((n, 16)).astype(np.float32) if k % 2 else None
+        host.append((c, w))
+    bufs = []
+
+    def device_jobs():
+        jobs = []
+        for (level, n, cw, tcm, tcb), (c, w) in zip(specs, host):
+            dc = gpu.DeviceBuffer.from_array(c)
+            dw = gpu.DeviceBuffer.from_array(w) if w is not None else None
+            dcode = gpu.DeviceBuffer(max(8 * n, 8))
+            dcode.fill(0x77)
+            derr = gpu.DeviceBuffer(max(4 * n, 4))
+            bufs.append((dc, dw, dcode, derr))
+            jobs.append(gpu.make_job(level, n, dc, dcode, dw, derr, None, cw, tcm, tcb))
+        return jobs
+
+    def collect(first):
+        out = []
+        for (level, n, *_), (dc, dw, dcode, derr) in zip(specs, bufs[first:first + len(specs)]):
+            out.append((dcode.download(np.uint8, (n, 8)), derr.download(np.float32, n)))
+        return out
+
+    results = {}
+    jobs = device_jobs()
+    gpu.compress_batch(jobs)
+    results["batch"] = collect(0)
+    jobs = device_jobs()
+    gpu.compress_batch(jobs, plain=True)
+    results["plain"] = collect(len(specs))
+    for name, kw in (("persistent_graph", dict(schedule=1, use_graph=True)),
+                     ("persistent", dict(schedule=1, use_graph=False)),
+                     ("grid_graph", dict(schedule=0, use_graph=True, order_by_cost=False))):
+        first = len(bufs)
+        jobs = device_jobs()
+        plan = gpu.Plan(jobs, **kw)
+        stream = gpu.Stream()
+        for _ in range(3):   # replays must give the same bytes every time
+            plan.execute(stream)
+            stream.synchronize()
+            results[name] = collect(first)
+        plan.close()
+        stream.close()
+    for (level, n, cw, tcm, tcb), (c, w), k in zip(specs, host, range(len(specs))):
+        prior = np.full((n, 8), 0x77, np.uint8)
+        want = _oracle_run(oracle, level, c, w, cw, tcm, tcb, prior)
+        for name, res in results.items():
+            assert_same(res[k][0], res[k][1], want[0], want[1], f"{name} job {k} L{level}")
+
+
+def test_decode_and_evaluate(gpu, golden, oracle):
+    codes = np.ascontiguousarray(golden["dec_codes"])
+    n = len(codes)
+    rgba = np.ascontiguousarray(golden["eval_rgba"])
+    dcodes = gpu.DeviceBuffer.from_array(codes)
+    drgba = gpu.DeviceBuffer(64 * n)
+    dsrc = gpu.DeviceBuffer.from_array(rgba)
+    derr = gpu.DeviceBuffer(4 * n)
+    for dec in range(4):
+        gpu.decode_bc1_batch(dcodes.ptr, n, drgba.ptr, dec)
+        assert np.array_equal(drgba.download(np.uint8, (n, 64)), golden["dec_rgba"][dec])
+        gpu.evaluate_bc1_error_batch(dsrc.ptr, dcodes.ptr, n, derr.ptr, dec)
+        assert np.array_equal(derr.download(np.float32, n).view(np.uint32), golden["eval_err"][dec].view(np.uint32))
+    # single-block mirrors
+    assert np.array_equal(gpu.decode_bc1(codes[3].tobytes(), 2), golden["dec_rgba"][2, 3])
+    assert gpu.evaluate_bc1_error(rgba[5], codes[5].tobytes(), 1) == golden["eval_err"][1, 5]
+
+
+def test_full_size_image_level8_properties(gpu, oracle):
+    # 4096x4096 (1,048,576 blocks) at L8: a seeded sample against the oracle,
+    # determinism across runs, and size-independent invariants
+    colors, cat = synth.make_image_blocks(7, 4096, 4096, synth.MIX_DEFAULT)
+    n = len(colors)
+    _say(f"4096x4096 L8: {n} blocks")
+    dcol = gpu.DeviceBuffer.from_array(colors)
+    runs = []
+    for _ in range(2):
+        dcode = gpu.DeviceBuffer(8 * n)
+        derr = gpu.DeviceBuffer(4 * n)
+        dst = gpu.DeviceBuffer(16 * n)
+        gpu.compress_batch([gpu.make_job(8, n, dcol, dcode, None, derr, dst, (1, 1, 1), True, True)])
+        runs.append((dcode.download(np.uint8, (n, 8)), derr.download(np.float32, n), dst.download(np.int32, (n, 4))))
+        for b in (dcode, derr, dst):
+            b.close()
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1].view(np.uint32), runs[1][1].view(np.uint32))
+    codes, errs, stats = runs[0]
+    sample = np.random.default_rng(8).choice(n, size=8192, replace=False)
+    want = _oracle_run(oracle, 8, colors[sample], None, (1, 1, 1), 1, 1)
+    assert_same(codes[sample], errs[sample], want[0], want[1], "4096x4096 L8 sample")
+    assert np.all(np.isfinite(errs)) and np.all(errs >= 0)
+    flat = cat == synth.CATEGORIES.index("flat")
+    assert np.all(stats[flat, 0] == 1)          # flat blocks reduce to one colour
+    assert np.all((stats[:, 0] >= 1) & (stats[:, 0] <= 16))

@@ -1,0 +1,281 @@
+"""CPU tests: pin the oracle (oracle/icbc_oracle.c).
+
+1. Against the golden vectors written by the reference encoder itself
+   (tests/golden/gen_golden.py): every level, flag combination, channel
+   weighting and decoder, plus decode/evaluate and the init tables.
+2. Against closed forms and brute force that do not depend on the reference's
+   code: partition tables, 565 expansion, palettes, single-colour optimality,
+   least squares on exact two-level blocks, error invariants.
+3. Stage by stage against the reference build in oracle/_ref when present.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from icbc_amd import synth
+
+
+def _set_decoder(o, dec):
+    if o.decoder != dec:
+        o.init(dec)
+
+
+# --------------------------------------------------------------------------
+# 1. golden vectors
+
+
+def test_tables_match_golden(oracle, golden):
+    for dec in range(4):
+        oracle.init(dec)
+        t = oracle.tables()
+        assert np.array_equal(t["match5"], golden["tables_match5"][dec])
+        assert np.array_equal(t["match6"], golden["tables_match6"][dec])
+        assert np.array_equal(t["weights"].view(np.uint32), golden["tables_weights"][dec].view(np.uint32))
+        assert np.array_equal(t["four"], golden["tables_four"])
+        assert np.array_equal(t["three"], golden["tables_three"])
+        assert np.array_equal(t["four_total"], golden["tables_four_total"])
+        assert np.array_equal(t["three_total"], golden["tables_three_total"])
+    m5, m6 = oracle.midpoints()
+    assert np.array_equal(m5.view(np.uint32), golden["midpoints5"].view(np.uint32))
+    assert np.array_equal(m6.view(np.uint32), golden["midpoints6"].view(np.uint32))
+    oracle.init(0)
+
+
+def test_whole_block_outputs_match_golden(oracle, golden):
+    colors, weights, prior = golden["colors"], golden["weights"], golden["prior"]
+    for k, (dec, level, tcm, tcb, cwi) in enumerate(golden["cfg"]):
+        _set_decoder(oracle, int(dec))
+        codes, errs, _, _ = oracle.compress_many(int(level), colors, weights, golden["cw"][cwi],
+                                                 int(tcm), int(tcb), prior=prior)
+        bad = np.nonzero((codes != golden["codes"][k]).any(1))[0]
+        assert len(bad) == 0, f"config {k} {tuple(golden['cfg'][k])}: {len(bad)} codes differ, first {bad[:5]}"
+        assert np.array_equal(errs.view(np.uint32), golden["errors"][k].view(np.uint32)), f"config {k} errors"
+    oracle.init(0)
+
+
+def test_decode_and_evaluate_match_golden(oracle, golden):
+    for dec in range(4):
+        for i in range(len(golden["dec_codes"])):
+            rgba = oracle.decode(golden["dec_codes"][i], dec)
+            assert np.array_equal(rgba, golden["dec_rgba"][dec, i])
+            e = oracle.evaluate_error(golden["eval_rgba"][i], golden["dec_codes"][i], dec)
+            assert np.float32(e).view(np.uint32) == golden["eval_err"][dec, i].view(np.uint32)
+
+
+def test_invalid_level_leaves_output(oracle, golden):
+    colors, weights, prior = golden["colors"][:32], golden["weights"][:32], golden["prior"][:32]
+    for level in (-1, 10, 100):
+        codes, errs, _, _ = oracle.compress_many(level, colors, weights, (1, 1, 1), 1, 1, prior=prior)
+        assert np.array_equal(codes, prior)
+        assert np.all(errs == np.finfo(np.float32).max)
+
+
+# --------------------------------------------------------------------------
+# 2. closed forms, brute force, invariants
+
+
+def test_partition_tables_closed_form(oracle):
+    t = oracle.tables()
+    for n in range(1, 17):
+        assert t["four_total"][n - 1] == math.comb(n + 3, 3) - 1
+        assert t["three_total"][n - 1] == math.comb(n + 2, 2) - 1
+    # every ordered partition (c0 <= c1 <= c2 <= n) other than (0,0,0) appears
+    # exactly once among the first total(n) entries, for every n
+    for n in range(1, 17):
+        four = {tuple(r[:3]) for r in t["four"][: t["four_total"][n - 1]]}
+        want = {(a, b, c) for c in range(n + 1) for b in range(c + 1) for a in range(b + 1)} - {(0, 0, 0)}
+        assert four == want
+        three = {tuple(r[:2]) for r in t["three"][: t["three_total"][n - 1]]}
+        want3 = {(a, b) for b in range(n + 1) for a in range(b + 1)} - {(0, 0)}
+        assert three == want3
+
+
+def test_expand565_and_d3d10_palette(oracle):
+    rng = np.random.default_rng(0)
+    pairs = rng.integers(0, 65536, size=(4000, 2))
+    pairs[:16] = [[0, 0], [65535, 65535], [65535, 0], [0, 65535], [1, 0], [0, 1], [0x8000, 0x7fff],
+                  [0x7fff, 0x8000], [0xf800, 0x07e0], [0x07e0, 0x001f], [0x001f, 0xf800], [2, 2],
+                  [123, 123], [40000, 39999], [39999, 40000], [31, 32]]
+
+    def expand(c):
+        r, g, b = (c >> 11) & 31, (c >> 5) & 63, c & 31
+        return np.array([(r << 3) | (r >> 2), (g << 2) | (g >> 4), (b << 3) | (b >> 2)])
+
+    for c0, c1 in pairs:
+        p = oracle.evaluate_palette(int(c0), int(c1)).reshape(4, 4).astype(int)
+        e0, e1 = expand(int(c0)), expand(int(c1))
+        assert np.array_equal(p[0, :3], e0) and np.array_equal(p[1, :3], e1)
+        if c0 > c1:
+            assert np.array_equal(p[2, :3], (2 * e0 + e1) // 3)
+            assert np.array_equal(p[3, :3], (2 * e1 + e0) // 3)
+            assert p[3, 3] == 255
+        else:
+            assert np.array_equal(p[2, :3], (e0 + e1) // 2)
+            assert np.array_equal(p[3], [0, 0, 0, 0])
+
+
+def test_single_colour_tables_are_optimal(oracle):
+    # brute force over all endpoint pairs, independent of the implementation
+    for dec in range(4):
+        oracle.init(dec)
+        t = oracle.tables()
+        for bits, table in ((5, t["match5"]), (6, t["match6"])):
+            lv = np.arange(1 << bits)
+            ex = (lv << 3) | (lv >> 2) if bits == 5 else (lv << 2) | (lv >> 4)
+            lo, hi = np.meshgrid(lv, lv, indexing="ij")      # lo outer, hi inner
+            elo, ehi = ex[lo], ex[hi]
+            amd = (43 * ehi + 21 * elo + 32) >> 6
+            intel = (171 * ehi + 85 * elo + 128) >> 8
+            if bits == 5:
+                nv = ((2 * hi + lo) * 22) // 8
+            else:
+                d = ehi - elo
+                nv = (256 * elo + np.trunc(d / 4).astype(int) + 128 + d * 80) // 256
+            for target in range(256):
+                errs = {0: np.maximum(np.maximum(abs(amd - target), abs(nv - target)), abs(intel - target)),
+                        1: abs(nv - target), 2: abs(amd - target), 3: abs(intel - target)}[dec]
+                flat = errs.reshape(-1)
+                k = int(np.argmin(flat))   # first minimum in (lo, hi) iteration order
+                assert table[target, 0] == hi.reshape(-1)[k] and table[target, 1] == lo.reshape(-1)[k]
+    oracle.init(0)
+
+
+def test_single_colour_block_encodes_colour(oracle):
+    for v in range(0, 256, 5):
+        blk = oracle.single_color(np.array([v, 255 - v, (v * 7) % 256], np.uint8))
+        idx = int.from_bytes(blk[4:8].tobytes(), "little")
+        c0 = int.from_bytes(blk[0:2].tobytes(), "little")
+        c1 = int.from_bytes(blk[2:4].tobytes(), "little")
+        assert idx in (0xAAAAAAAA, 0xFFFFFFFF)
+        assert c0 >= c1
+
+
+def test_lsq_recovers_two_level_block(oracle):
+    # exact two-level blocks: LSQ with the true indices solves the 2x2 normal
+    # equations exactly, so the endpoints come back (up to float rounding)
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        a, b = rng.random(3).astype(np.float32), rng.random(3).astype(np.float32)
+        sel = rng.random(16) < 0.5
+        sel[0], sel[1] = True, False
+        colors = np.ones((16, 4), np.float32)
+        colors[sel, :3] = a
+        colors[~sel, :3] = b
+        idx = 0
+        for i in range(16):
+            idx |= (0 if sel[i] else 1) << (2 * i)
+        ok, ea, eb = oracle.optimize_end_points4(idx, colors.reshape(-1))
+        assert ok
+        assert np.allclose(ea, a, atol=2e-6) and np.allclose(eb, b, atol=2e-6)
+
+
+def test_returned_error_is_error_of_returned_block(oracle):
+    colors, _ = synth.make_image_blocks(5, 32, 32, synth.MIX_DEFAULT)
+    ec, ew = synth.make_edge_blocks(6, 64)
+    colors = np.concatenate([colors, ec])
+    weights = np.concatenate([np.ones((64, 16), np.float32), ew])
+    for level in range(10):
+        codes, errs, stats, _ = oracle.compress_many(level, colors, weights, (1, 1, 1), 1, 1, with_stats=True)
+        for b in range(len(colors)):
+            e = oracle.evaluate_mse_block(colors[b].reshape(-1), weights[b], (1, 1, 1), codes[b])
+            # the error is recomputed from the code alone
+            assert np.float32(e).view(np.uint32) == errs[b].view(np.uint32) or (np.isnan(e) and np.isnan(errs[b]))
+            c0 = int(codes[b, 0]) | (int(codes[b, 1]) << 8)
+            c1 = int(codes[b, 2]) | (int(codes[b, 3]) << 8)
+            if level < 9 and stats[b, 1] < 0 and stats[b, 0] >= 2:
+                assert c0 >= c1   # 4-colour output ordering
+        if level >= 2:
+            assert np.all(stats[:, 0] <= 16)
+
+
+def test_zero_weight_block_is_zero(oracle):
+    colors = np.random.default_rng(2).random((4, 16, 4)).astype(np.float32)
+    weights = np.zeros((4, 16), np.float32)
+    for level in range(2, 10):
+        codes, errs, _, _ = oracle.compress_many(level, colors, weights, (1, 1, 1), 1, 1)
+        assert np.all(codes == 0) and np.all(errs == 0)
+
+
+# --------------------------------------------------------------------------
+# 3. stage by stage against the reference build (skipped when absent)
+
+
+def _random_blocks(seed, n):
+    c1, _ = synth.make_image_blocks(seed, 32, 32, synth.MIX_DEFAULT)
+    c2, w2 = synth.make_edge_blocks(seed + 1, n)
+    colors = np.concatenate([c1, c2])
+    weights = np.concatenate([np.ones((len(c1), 16), np.float32), w2])
+    return colors, weights
+
+
+def _bits(x):
+    return np.asarray(x, np.float32).view(np.uint32)
+
+
+def test_stages_against_reference(oracle, reference):
+    colors, weights = _random_blocks(11, 128)
+    metric = np.array([1, 1, 1], np.float32)
+    for b in range(len(colors)):
+        c64 = colors[b].reshape(-1)
+        w16 = weights[b]
+        for thr in (1 / 24, 1 / 64, 1 / 256):
+            ro = oracle.reduce_colors(c64, w16, np.float32(thr))
+            rr = reference.reduce_colors(c64, w16, np.float32(thr))
+            assert ro[0] == rr[0] and ro[3] == rr[3]
+            assert np.array_equal(_bits(ro[1]), _bits(rr[1])) and np.array_equal(_bits(ro[2]), _bits(rr[2]))
+        n, rgb, w, _ = ro
+        if n >= 1:
+            assert np.array_equal(_bits(oracle.principal_component(rgb, w)), _bits(reference.principal_component(rgb, w)))
+            so, sr = oracle.compute_sat(rgb, w), reference.compute_sat(rgb, w)
+            assert np.array_equal(_bits(so), _bits(sr))
+            for four in (True, False):
+                if four and n < 2:
+                    continue
+                ao, bo = oracle.cluster_fit(so, n, metric, four)
+                ar, br = reference.cluster_fit(sr, n, metric, four)
+                assert np.array_equal(_bits(ao), _bits(ar)) and np.array_equal(_bits(bo), _bits(br))
+            a, bb = oracle.box_fit(rgb)
+            ar, br = reference.box_fit(rgb)
+            assert np.array_equal(_bits(a), _bits(ar)) and np.array_equal(_bits(bb), _bits(br))
+        v0, v1 = colors[b, 0, :3], colors[b, 5, :3]
+        for four in (True, False):
+            for ab in (False, True):
+                oo, eo = oracle.output_block(c64, w16, (1, 1, 1), v0, v1, four, ab)
+                orr, er = reference.output_block(c64, w16, (1, 1, 1), v0, v1, four, ab)
+                assert np.array_equal(oo, orr) and _bits(eo) == _bits(er)
+        idx = int.from_bytes(oo[4:8].tobytes(), "little")
+        oko, ao, bo = oracle.optimize_end_points4(idx, c64)
+        okr, ar, br = reference.optimize_end_points4(idx, c64)
+        assert oko == okr
+        if oko:
+            assert np.array_equal(_bits(ao), _bits(ar)) and np.array_equal(_bits(bo), _bits(br))
+        pal = np.concatenate([colors[b, 3, :3], colors[b, 7, :3], colors[b, 11, :3], colors[b, 15, :3]])
+        for kind in (4, 3, 0):
+            for ab in (False, True):
+                assert oracle.compute_indices(kind, c64, (1, 1, 1), pal, ab) == \
+                    reference.compute_indices(kind, c64, (1, 1, 1), pal, ab)
+        for tcm in (False, True):
+            bo, eo = oracle.refine(c64, w16, (1, 1, 1), tcm, 1e30, oo)
+            br, er = reference.refine(c64, w16, (1, 1, 1), tcm, 1e30, oo)
+            assert np.array_equal(bo, br) and _bits(eo) == _bits(er)
+        v = colors[b, 2, :3] * 1.1 - 0.05
+        assert oracle.vector3_to_color16(v) == reference.vector3_to_color16(v)
+
+
+def test_whole_blocks_against_reference_all_decoders(oracle, reference):
+    colors, weights = _random_blocks(21, 256)
+    rng = np.random.default_rng(22)
+    prior = rng.integers(0, 256, size=(len(colors), 8), dtype=np.uint8)
+    for dec in range(4):
+        oracle.init(dec)
+        reference.init(dec)
+        for level in range(10):
+            for tcm, tcb in ((0, 0), (1, 0), (1, 1)):
+                for cw in ((1, 1, 1), (0.2126, 0.7152, 0.0722)):
+                    co, eo, _, _ = oracle.compress_many(level, colors, weights, cw, tcm, tcb, prior=prior)
+                    cr, er, _ = reference.compress_many(level, colors, weights, cw, tcm, tcb, prior=prior)
+                    assert np.array_equal(co, cr), (dec, level, tcm, tcb, cw)
+                    assert np.array_equal(_bits(eo), _bits(er)), (dec, level, tcm, tcb, cw)
+    oracle.init(0)
+    reference.init(0)
