@@ -1,0 +1,240 @@
+"""Benchmark: BC1 Mblocks/s at quality level 8 (bit-exact), batched vs plain GPU.
+
+Workload (default): the mixed-size texture batch -- 256 synthetic textures,
+sides log-uniform in [64, 4096] (width and height drawn independently), each
+with its full box-filtered mip chain down to 4x4 and edges padded by
+replication to multiples of 4; block content 40% gradients / 30% white noise
+/ 20% near-black / 10% flat; every texture encoded at quality level 8 with
+three_color_mode and three_color_black on, per-texel weights 1, channel
+weights (1, 1, 1) -- the reference harness's settings (icbc_test.cpp:383-408).
+
+A step is one encode of the whole batch with inputs resident in HBM:
+  batched  one persistent, graph-captured launch over all textures and mips
+  plain    one launch per texture mip level (the comparison path)
+`value` is the batched path's whole-batch throughput.
+
+Also reported: the dominant kernel's roofline (algorithmic FP32 FLOPs from the
+encoder's own per-block work counters over its average duration, measured
+with HIP events on the launch stream), and the reference encoder's own SIMD
+build run multithreaded on this host on a bounded sample as the CPU baseline.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "BC1 Mblocks/s at level 8 (bit-exact), batched vs plain GPU; p99 request latency"
+PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector, dense (AMD spec)
+PEAK_FP32_NOFMA_TFLOPS = 39.3  # same units with unfused, unpacked FP32 (bit-exact build)
+# algorithmic FP32 operations per unit of work (divisions count 1), from the
+# reference's arithmetic: one 4-cluster candidate, one 3-cluster candidate,
+# one refinement step (icbc.h:2685-2742, 2299-2344, 3614-3648)
+FLOPS_FIT4 = 121
+FLOPS_FIT3 = 104
+FLOPS_REFINE_STEP = 888
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_flops(stats):
+    n = stats[:, 0].astype(np.int64)
+    n3 = stats[:, 1].astype(np.int64)
+    steps = stats[:, 2].astype(np.int64)
+    fit4 = np.where(n >= 2, (n + 3) * (n + 2) * (n + 1) // 6 - 1, 0)
+    fit3 = np.where(n3 >= 1, (n3 + 2) * (n3 + 1) // 2 - 1, 0)
+    return int((FLOPS_FIT4 * fit4 + FLOPS_FIT3 * fit3 + FLOPS_REFINE_STEP * steps).sum())
+
+
+def cpu_features():
+    try:
+        flags = open("/proc/cpuinfo").read()
+    except OSError:
+        return set()
+    for line in flags.splitlines():
+        if line.startswith("flags"):
+            return set(line.split(":", 1)[1].split())
+    return set()
+
+
+def cpu_baseline(blocks, level, tcm, tcb, threads, budget_cpu_s=20.0):
+    """The reference encoder's SIMD build (oracle/_ref) run multithreaded on
+    a prefix of the workload sized to about `budget_cpu_s` of CPU time.  Falls
+    back to the oracle restatement when the reference build is absent."""
+    from oracle import Oracle, Reference, reference_available
+    feats = cpu_features()
+    variant = None
+    if {"avx512f", "avx512bw", "avx512dq", "avx512vl"} <= feats and reference_available("avx512"):
+        variant = "avx512"
+    elif {"avx2", "fma", "bmi2"} <= feats and reference_available("avx2"):
+        variant = "avx2"
+    elif reference_available("scalar"):
+        variant = "scalar"
+    if variant:
+        enc = Reference(variant)
+        enc.init(0)
+        kind, label = "reference", f"icbc.h ICBC_SIMD={enc.simd_level} ({variant})"
+
+        def run(b):
+            return enc.compress_many(level, b, None, (1, 1, 1), tcm, tcb, nthreads=threads)[2]
+    else:
+        enc = Oracle()
+        enc.init(0)
+        kind, label = "port", "oracle restatement (scalar C)"
+
+        def run(b):
+            return enc.compress_many(level, b, None, (1, 1, 1), tcm, tcb, nthreads=threads)[3]
+    probe = blocks[: min(len(blocks), 32768)]
+    t = run(probe)
+    per_block_cpu = t * threads / len(probe)
+    n = int(min(len(blocks), max(len(probe), budget_cpu_s / max(per_block_cpu, 1e-9))))
+    sample = blocks[:n]
+    secs = run(sample)
+    return {"value": round(n / secs / 1e6, 4), "unit": "Mblocks/s", "cores": threads, "kind": kind,
+            "sample": f"first {n} blocks of the workload, level {level}, {label}, "
+                      f"{threads} threads, chunks of 32 blocks; {secs:.2f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--textures", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--level", type=int, default=8)
+    ap.add_argument("--mixed-levels", action="store_true",
+                    help="per-texture level drawn from 1..9 instead of the fixed level")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the encoder shards with no exchange step, "
+              "so multi-GPU figures are not produced (run with --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    import icbc_amd as g
+    from icbc_amd import synth
+
+    g.init(g.Decoder.D3D10)
+    dev = g.device_name()
+    log(f"device: {dev}")
+
+    t0 = time.time()
+    levels = range(1, 10) if args.mixed_levels else [args.level]
+    texs = synth.texture_set(args.seed, count=args.textures, levels=levels)
+    counts = np.array([len(t[5]) for t in texs], np.int64)
+    total = int(counts.sum())
+    offsets = np.concatenate([[0], np.cumsum(counts)])
+    colors = np.empty((total, 16, 4), np.float32)
+    for (tex, mip, w, h, lv, blocks), o in zip(texs, offsets[:-1]):
+        colors[o:o + len(blocks)] = blocks
+    job_levels = [t[4] for t in texs]
+    log(f"workload: {args.textures} textures, {len(texs)} images (with mips), {total} blocks, "
+        f"generated in {time.time() - t0:.1f} s")
+
+    tcm = tcb = True
+    dcol = g.DeviceBuffer.from_array(colors)
+    dcode = g.DeviceBuffer(8 * total)
+    derr = g.DeviceBuffer(4 * total)
+    dstat = g.DeviceBuffer(16 * total)
+
+    def jobs(with_stats=False):
+        return [g.make_job(lv, int(n), dcol.ptr + 256 * int(o), dcode.ptr + 8 * int(o), None,
+                           derr.ptr + 4 * int(o), (dstat.ptr + 16 * int(o)) if with_stats else None,
+                           (1, 1, 1), tcm, tcb)
+                for lv, n, o in zip(job_levels, counts, offsets[:-1])]
+
+    stream = g.Stream()
+    plan = g.Plan(jobs(), schedule=g.SCHEDULE_PERSISTENT, use_graph=True, order_by_cost=True,
+                  stream=stream)
+    ev = [g.Event() for _ in range(args.steps + 1)]
+
+    def timed(run_once):
+        for _ in range(args.warmup):
+            run_once()
+        stream.synchronize()
+        ev[0].record(stream)
+        for k in range(args.steps):
+            run_once()
+            ev[k + 1].record(stream)
+        ev[-1].synchronize()
+        per = [ev[k].elapsed_ms(ev[k + 1]) for k in range(args.steps)]
+        return per
+
+    per_batched = timed(lambda: plan.execute(stream))
+    ms_batched = float(np.mean(per_batched))
+    codes_batched = dcode.download(np.uint8, (total, 8))
+    log(f"batched: {ms_batched:.3f} ms/step ({total / ms_batched / 1e3:.2f} Mblocks/s)")
+
+    result_plain = None
+    if not args.no_plain:
+        plain_jobs = jobs()
+        per_plain = timed(lambda: g.compress_batch(plain_jobs, stream=stream, plain=True))
+        ms_plain = float(np.mean(per_plain))
+        codes_plain = dcode.download(np.uint8, (total, 8))
+        same = bool(np.array_equal(codes_plain, codes_batched))
+        result_plain = {"ms_per_step": round(ms_plain, 4),
+                        "value": round(total / ms_plain / 1e3, 3),
+                        "launches_per_step": len(texs), "codes_identical_to_batched": same}
+        log(f"plain: {ms_plain:.3f} ms/step ({total / ms_plain / 1e3:.2f} Mblocks/s), identical={same}")
+
+    # work counters for the roofline (separate, untimed pass)
+    g.compress_batch(jobs(with_stats=True), stream=stream)
+    stream.synchronize()
+    stats = dstat.download(np.int32, (total, 4))
+    flops = algorithmic_flops(stats)
+    achieved = flops / (ms_batched * 1e-3) / 1e12
+    roofline = {"bound": "valu-fp32", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                "frac_of_unfused_peak": round(achieved / PEAK_FP32_NOFMA_TFLOPS, 4),
+                "traffic": None, "kernel": "bc1_persistent_kernel",
+                "algorithmic_gflop_per_launch": round(flops / 1e9, 3),
+                "mean_fit4_candidates_per_block": round(float(np.mean(
+                    np.where(stats[:, 0] >= 2, (stats[:, 0] + 3) * (stats[:, 0] + 2) * (stats[:, 0] + 1) // 6 - 1, 0))), 2)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        sample_lv = args.level if not args.mixed_levels else 8
+        cpu = cpu_baseline(colors, sample_lv, tcm, tcb, threads)
+        log(f"cpu baseline: {cpu}")
+
+    value = total / ms_batched / 1e3
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mblocks/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_batched, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": ("texture batch: 256 textures log-uniform 64-4096 with full mip chains, "
+                                + ("levels 1-9 per texture" if args.mixed_levels else f"quality level {args.level}")
+                                + ", three_color_mode+three_color_black, weights 1, cw (1,1,1)"),
+                   "textures": args.textures, "images": len(texs), "blocks": total,
+                   "level": "1-9" if args.mixed_levels else args.level, "seed": args.seed,
+                   "path": "persistent graph-captured launch", "device": dev},
+        "plain": result_plain,
+        "batched_over_plain": (round(result_plain["ms_per_step"] / ms_batched, 4) if result_plain else None),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    plan.close()
+    for b in (dcol, dcode, derr, dstat):
+        b.close()
+
+
+if __name__ == "__main__":
+    main()

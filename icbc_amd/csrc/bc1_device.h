@@ -70,8 +70,11 @@ __device__ __forceinline__ Code make_code(uint32_t c0, uint32_t c1, uint32_t idx
 // ---------------------------------------------------------------------------
 // 565 conversion (icbc.h:1588-1667)
 
-__device__ __forceinline__ float q5(float x) { return round01(x * 31.0f) * (1.0f / 31.0f); }
-__device__ __forceinline__ float q6(float x) { return round01(x * 63.0f) * (1.0f / 63.0f); }
+// round01 of a saturated value: x*31 + 0.5 lies in [0.5, 31.5] (x*63 + 0.5 in
+// [0.5, 63.5]), where float(int(y)) equals truncf(y); v_trunc_f32 does it in
+// one instruction instead of a convert pair.
+__device__ __forceinline__ float q5(float x) { return truncf(x * 31.0f + 0.5f) * (1.0f / 31.0f); }
+__device__ __forceinline__ float q6(float x) { return truncf(x * 63.0f + 0.5f) * (1.0f / 63.0f); }
 __device__ __forceinline__ f3 quantize_endpoint(f3 v) {
     return mk(q5(sat01(v.x)), q6(sat01(v.y)), q5(sat01(v.z)));
 }
@@ -160,6 +163,20 @@ __device__ __forceinline__ void palette_f(uint32_t c0, uint32_t c1, int decoder,
 }
 
 // ---------------------------------------------------------------------------
+// The 16 texels and weights of one block, read from memory on demand (L1/L2
+// resident after the first pass) rather than held in registers.
+
+struct Texels {
+    const float4* c;  // 16 texels, RGBA (alpha unused)
+    const float* w;   // 16 weights, or null for all 1.0
+    __device__ __forceinline__ f3 color(int i) const {
+        const float4 v = c[i];
+        return mk(v.x, v.y, v.z);
+    }
+    __device__ __forceinline__ float weight(int i) const { return w ? w[i] : 1.0f; }
+};
+
+// ---------------------------------------------------------------------------
 // Error of a block for a palette and index word (icbc.h:2955-2958, 3020-3028).
 
 __device__ __forceinline__ float texel_error(f3 p, f3 c, f3 w) {
@@ -173,115 +190,136 @@ __device__ __forceinline__ f3 pick(const f3 pal[4], uint32_t k) {
     return (k & 2) ? hi : lo;
 }
 
-__device__ __forceinline__ float palette_error(const f3 tex[16], const float wt[16], f3 cw,
-                                               const f3 pal[4], uint32_t indices) {
+__device__ __forceinline__ float palette_error(const Texels& t, f3 cw, const f3 pal[4], uint32_t indices) {
     float e = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 16; i++) e += wt[i] * texel_error(pick(pal, (indices >> (2 * i)) & 3), tex[i], cw);
+#pragma unroll 4
+    for (int i = 0; i < 16; i++)
+        e += t.weight(i) * texel_error(pick(pal, (indices >> (2 * i)) & 3), t.color(i), cw);
     return e;
 }
 
-__device__ __forceinline__ float code_error(const f3 tex[16], const float wt[16], f3 cw, int decoder,
-                                            Code c) {
+__device__ __forceinline__ float code_error(const Texels& t, f3 cw, int decoder, Code c) {
     f3 pal[4];
     palette_f(col0(c), col1(c), decoder, pal);
-    return palette_error(tex, wt, cw, pal, c.idx);
+    return palette_error(t, cw, pal, c.idx);
 }
 
 // ---------------------------------------------------------------------------
-// Index selection (icbc.h:3097-3235).  Returned words hold texel i's 2-bit
-// index at bits 2i..2i+1.
+// Index selection (icbc.h:3097-3235) fused with the error of the chosen
+// indices: texel i's index goes to bits 2i..2i+1 and its weighted error is
+// added in texel order, exactly as selecting all indices first and then
+// summing the errors would.
 
-__device__ __forceinline__ uint32_t indices4(const f3 tex[16], f3 w, const f3 pal[4]) {
-    f3 p0 = pal[0] * w, p1 = pal[1] * w, p2 = pal[2] * w, p3 = pal[3] * w;
+// compute_indices4 rule (icbc.h:3097-3131)
+__device__ __forceinline__ float select4(const Texels& t, f3 w, const f3 pal[4], uint32_t& indices) {
+    const f3 p0 = pal[0] * w, p1 = pal[1] * w, p2 = pal[2] * w, p3 = pal[3] * w;
     uint32_t out = 0;
-#pragma unroll
+    float e = 0.0f;
+#pragma unroll 4
     for (int i = 0; i < 16; i++) {
-        f3 c = tex[i] * w;
-        f3 e0 = c - p0, e1 = c - p1, e2 = c - p2, e3 = c - p3;
-        float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2), d3 = dot3(e3, e3);
-        uint32_t hi = ((d1 > d2) & (d0 > d2)) | ((d0 > d3) & (d1 > d3));
-        uint32_t lo = (d0 > d3) & (d2 > d3);
-        out |= (lo | (hi << 1)) << (2 * i);
+        const f3 x = t.color(i);
+        const f3 c = x * w;
+        const f3 e0 = c - p0, e1 = c - p1, e2 = c - p2, e3 = c - p3;
+        const float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2), d3 = dot3(e3, e3);
+        const uint32_t hi = ((d1 > d2) & (d0 > d2)) | ((d0 > d3) & (d1 > d3));
+        const uint32_t lo = (d0 > d3) & (d2 > d3);
+        const uint32_t k = lo | (hi << 1);
+        out |= k << (2 * i);
+        e += t.weight(i) * texel_error(pick(pal, k), x, w);
     }
-    return out;
+    indices = out;
+    return e;
 }
 
-__device__ __forceinline__ uint32_t indices3(const f3 tex[16], f3 w, bool allow_black, const f3 pal[4]) {
-    f3 p0 = pal[0] * w, p1 = pal[1] * w, p2 = pal[2] * w;
+// compute_indices3 rule (icbc.h:3133-3176)
+__device__ __forceinline__ float select3(const Texels& t, f3 w, bool allow_black, const f3 pal[4],
+                                         uint32_t& indices) {
+    const f3 p0 = pal[0] * w, p1 = pal[1] * w, p2 = pal[2] * w;
     uint32_t out = 0;
-    if (allow_black) {
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            f3 c = tex[i] * w;
-            f3 e0 = p0 - c, e1 = p1 - c, e2 = p2 - c;
-            float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2), d3 = dot3(c, c);
-            uint32_t i1 = d1 < d2;
-            uint32_t i2 = (d2 <= d0) & (d2 <= d1);
-            uint32_t i3 = (d3 <= d0) & (d3 <= d1) & (d3 <= d2);
-            out |= ((i1 | i3) | ((i2 | i3) << 1)) << (2 * i);
+    float e = 0.0f;
+#pragma unroll 4
+    for (int i = 0; i < 16; i++) {
+        const f3 x = t.color(i);
+        const f3 c = x * w;
+        uint32_t k;
+        if (allow_black) {
+            const f3 e0 = p0 - c, e1 = p1 - c, e2 = p2 - c;
+            const float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2), d3 = dot3(c, c);
+            const uint32_t i1 = d1 < d2;
+            const uint32_t i2 = (d2 <= d0) & (d2 <= d1);
+            const uint32_t i3 = (d3 <= d0) & (d3 <= d1) & (d3 <= d2);
+            k = (i1 | i3) | ((i2 | i3) << 1);
+        } else {
+            const f3 e0 = c - p0, e1 = c - p1, e2 = c - p2;
+            const float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2);
+            const uint32_t i1 = d1 < d2;
+            const uint32_t i2 = (d2 <= d0) & (d2 <= d1);
+            k = i1 | (i2 << 1);
         }
-    } else {
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            f3 c = tex[i] * w;
-            f3 e0 = c - p0, e1 = c - p1, e2 = c - p2;
-            float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2);
-            uint32_t i1 = d1 < d2;
-            uint32_t i2 = (d2 <= d0) & (d2 <= d1);
-            out |= (i1 | (i2 << 1)) << (2 * i);
-        }
+        out |= k << (2 * i);
+        e += t.weight(i) * texel_error(pick(pal, k), x, w);
     }
-    return out;
+    indices = out;
+    return e;
 }
 
-__device__ __forceinline__ uint32_t indices_refine(const f3 tex[16], f3 w, const f3 pal[4]) {
-    f3 p0 = pal[0] * w, p1 = pal[1] * w, p2 = pal[2] * w, p3 = pal[3] * w;
+// compute_indices rule used by refinement (icbc.h:3179-3235): indices chosen
+// against `sel` (the current best block's palette), error measured against
+// `pal` (the candidate block's palette).
+__device__ __forceinline__ float select_refine(const Texels& t, f3 w, const f3 sel[4], const f3 pal[4],
+                                               uint32_t& indices) {
+    const f3 p0 = sel[0] * w, p1 = sel[1] * w, p2 = sel[2] * w, p3 = sel[3] * w;
     uint32_t out = 0;
-#pragma unroll
+    float e = 0.0f;
+#pragma unroll 4
     for (int i = 0; i < 16; i++) {
-        f3 c = tex[i] * w;
-        f3 e0 = c - p0, e1 = c - p1, e2 = c - p2, e3 = c - p3;
-        float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2), d3 = dot3(e3, e3);
-        uint32_t i1 = (d1 <= d0) & (d1 < d2) & (d1 < d3);
-        uint32_t i2 = (d2 <= d0) & (d2 <= d1) & (d2 < d3);
-        uint32_t i3 = (d3 <= d0) & (d3 <= d1) & (d3 <= d2);
-        out |= ((i1 | i3) | ((i2 | i3) << 1)) << (2 * i);
+        const f3 x = t.color(i);
+        const f3 c = x * w;
+        const f3 e0 = c - p0, e1 = c - p1, e2 = c - p2, e3 = c - p3;
+        const float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2), d3 = dot3(e3, e3);
+        const uint32_t i1 = (d1 <= d0) & (d1 < d2) & (d1 < d3);
+        const uint32_t i2 = (d2 <= d0) & (d2 <= d1) & (d2 < d3);
+        const uint32_t i3 = (d3 <= d0) & (d3 <= d1) & (d3 <= d2);
+        const uint32_t k = (i1 | i3) | ((i2 | i3) << 1);
+        out |= k << (2 * i);
+        e += t.weight(i) * texel_error(pick(pal, k), x, w);
     }
-    return out;
+    indices = out;
+    return e;
 }
 
 // ---------------------------------------------------------------------------
 // Block output (icbc.h:3238-3274)
 
-__device__ __forceinline__ float emit4(const f3 tex[16], const float wt[16], f3 cw, int decoder, f3 a,
-                                       f3 b, Code& out) {
+__device__ __forceinline__ float emit4(const Texels& t, f3 cw, int decoder, f3 a, f3 b, Code& out) {
     uint32_t c0 = to565(a), c1 = to565(b);
     if (c0 < c1) {
-        uint32_t t = c0;
+        uint32_t tmp = c0;
         c0 = c1;
-        c1 = t;
+        c1 = tmp;
     }
     f3 pal[4];
     palette_f(c0, c1, decoder, pal);
-    uint32_t idx = indices4(tex, cw, pal);
+    uint32_t idx;
+    const float e = select4(t, cw, pal, idx);
     out = make_code(c0, c1, idx);
-    return palette_error(tex, wt, cw, pal, idx);
+    return e;
 }
 
-__device__ __forceinline__ float emit3(const f3 tex[16], const float wt[16], f3 cw, int decoder,
-                                       bool allow_black, f3 a, f3 b, Code& out) {
+__device__ __forceinline__ float emit3(const Texels& t, f3 cw, int decoder, bool allow_black, f3 a, f3 b,
+                                       Code& out) {
     uint32_t c0 = to565(a), c1 = to565(b);
     if (c0 > c1) {
-        uint32_t t = c0;
+        uint32_t tmp = c0;
         c0 = c1;
-        c1 = t;
+        c1 = tmp;
     }
     f3 pal[4];
     palette_f(c0, c1, decoder, pal);
-    uint32_t idx = indices3(tex, cw, allow_black, pal);
+    uint32_t idx;
+    const float e = select3(t, cw, allow_black, pal, idx);
     out = make_code(c0, c1, idx);
-    return palette_error(tex, wt, cw, pal, idx);
+    return e;
 }
 
 // ---------------------------------------------------------------------------
@@ -298,14 +336,14 @@ __device__ __forceinline__ bool near3(f3 a, f3 b, float eps) {
 // Merge each weighted texel into the first earlier colour within `threshold`
 // (running weighted average), else append it.  Texels with weight <= 0 (or
 // NaN) are dropped.
-__device__ __forceinline__ int reduce_colors(const f3 tex[16], const float wt[16], float threshold,
-                                             f3 cols[16], float ws[16], bool& any_black) {
+__device__ __forceinline__ int reduce_colors(const Texels& t, float threshold, f3 cols[16], float ws[16],
+                                             bool& any_black) {
     int n = 0;
     any_black = false;
     for (int i = 0; i < 16; i++) {
-        const float wi = wt[i];
+        const float wi = t.weight(i);
         if (!(wi > 0)) continue;
-        const f3 ci = tex[i];
+        const f3 ci = t.color(i);
         int j = 0;
         while (j < n && !near3(cols[j], ci, threshold)) j++;
         if (j < n) {
@@ -478,10 +516,10 @@ __device__ __forceinline__ float candidate3(float4 s0, float4 s1, float4 tot, f3
 // Least-squares endpoints for fixed indices (icbc.h:3278-3310) and box fit
 // (icbc.h:3313-3368).
 
-__device__ __forceinline__ bool lsq_endpoints(uint32_t indices, const f3 tex[16], f3& a, f3& b) {
+__device__ __forceinline__ bool lsq_endpoints(uint32_t indices, const Texels& t, f3& a, f3& b) {
     float a2 = 0.0f, b2 = 0.0f, ab = 0.0f;
     f3 ax = mk(0.0f, 0.0f, 0.0f), bx = mk(0.0f, 0.0f, 0.0f);
-#pragma unroll
+#pragma unroll 4
     for (int i = 0; i < 16; i++) {
         const uint32_t bits = indices >> (2 * i);
         float beta = (float)(bits & 1);
@@ -490,8 +528,9 @@ __device__ __forceinline__ bool lsq_endpoints(uint32_t indices, const f3 tex[16]
         a2 += alpha * alpha;
         b2 += beta * beta;
         ab += alpha * beta;
-        ax = ax + tex[i] * alpha;
-        bx = bx + tex[i] * beta;
+        const f3 x = t.color(i);
+        ax = ax + x * alpha;
+        bx = bx + x * beta;
     }
     const float denom = a2 * b2 - ab * ab;
     if (fabsf(denom - 0.0f) < (float)0.0001) return false;
@@ -501,11 +540,14 @@ __device__ __forceinline__ bool lsq_endpoints(uint32_t indices, const f3 tex[16]
     return true;
 }
 
-__device__ __forceinline__ void box_fit(const f3 cols[16], int n, f3& c0, f3& c1) {
+// `get(i)` returns colour i of the n colours to fit.
+template <typename Get>
+__device__ __forceinline__ void box_fit(Get get, int n, f3& c0, f3& c1) {
     f3 hi = mk(0.0f, 0.0f, 0.0f), lo = mk(1.0f, 1.0f, 1.0f);
     for (int i = 0; i < n; i++) {
-        hi = mk(max_r(hi.x, cols[i].x), max_r(hi.y, cols[i].y), max_r(hi.z, cols[i].z));
-        lo = mk(min_r(lo.x, cols[i].x), min_r(lo.y, cols[i].y), min_r(lo.z, cols[i].z));
+        const f3 c = get(i);
+        hi = mk(max_r(hi.x, c.x), max_r(hi.y, c.y), max_r(hi.z, c.z));
+        lo = mk(min_r(lo.x, c.x), min_r(lo.y, c.y), min_r(lo.z, c.z));
     }
     const float bias = (8.0f / 255.0f) / 16.0f;
     const f3 d = hi - lo;
@@ -515,7 +557,7 @@ __device__ __forceinline__ void box_fit(const f3 cols[16], int n, f3& c0, f3& c1
     const f3 center = (hi + lo) * 0.5f;
     float cxz = 0.0f, cyz = 0.0f;
     for (int i = 0; i < n; i++) {
-        const f3 t = cols[i] - center;
+        const f3 t = get(i) - center;
         cxz += t.x * t.z;
         cyz += t.y * t.z;
     }
@@ -585,9 +627,8 @@ __device__ __forceinline__ void refine_step(int k, int& dr, int& dg, int& db) {
     db = (int)((kPackB >> (2 * k)) & 3) - 1;
 }
 
-__device__ __forceinline__ float refine(const f3 tex[16], const float wt[16], f3 cw, int decoder,
-                                        bool three_color_mode, float input_error, Code& out,
-                                        int& steps) {
+__device__ __forceinline__ float refine(const Texels& t, f3 cw, int decoder, bool three_color_mode,
+                                        float input_error, Code& out, int& steps) {
     float best = input_error;
     int last = 0;
     int i = 0;
@@ -600,19 +641,19 @@ __device__ __forceinline__ float refine(const f3 tex[16], const float wt[16], f3
         if (!three_color_mode) {
             if (c0 == c1) c1 = nudge565(c1, 0, 1, 0);
             if (c0 < c1) {
-                uint32_t t = c0;
+                uint32_t tmp = c0;
                 c0 = c1;
-                c1 = t;
+                c1 = tmp;
             }
         }
-        f3 pal[4];
-        palette_f(col0(out), col1(out), decoder, pal);  // palette of the current best block
-        const uint32_t idx = indices_refine(tex, cw, pal);
-        const Code cand = make_code(c0, c1, idx);
-        const float e = code_error(tex, wt, cw, decoder, cand);
+        f3 sel[4], pal[4];
+        palette_f(col0(out), col1(out), decoder, sel);  // palette of the current best block
+        palette_f(c0, c1, decoder, pal);
+        uint32_t idx;
+        const float e = select_refine(t, cw, sel, pal, idx);
         if (e < best) {
             best = e;
-            out = cand;
+            out = make_code(c0, c1, idx);
             last = i;
         }
         if (i - last > 32) break;

@@ -51,84 +51,111 @@ hipError_t upload_tables(const HostTables& t, hipStream_t stream) {
 }
 
 // ---------------------------------------------------------------------------
-// wave-wide argmin over (error, candidate index): smaller error wins, equal
-// errors (including -0 == +0) resolve to the smaller index.
+// Wave-wide argmin over (error, candidate index) with DPP lane moves: smaller
+// error wins, equal errors (including -0 == +0) resolve to the smaller index.
+// Quad swaps, half-row and row mirrors, then the row broadcasts carry the
+// minimum to lane 63, which is read back as a wave-uniform value.
 
-__device__ __forceinline__ void wave_argmin(float& best, int& idx) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const float ob = __shfl_xor(best, off, 64);
-        const int oi = __shfl_xor(idx, off, 64);
-        if (ob < best || (ob == best && oi < idx)) {
-            best = ob;
-            idx = oi;
-        }
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ void argmin_step(float& e, int& i) {
+    const float oe = __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(e), __float_as_int(e), kCtrl, kRowMask, 0xf, false));
+    const int oi = __builtin_amdgcn_update_dpp(i, i, kCtrl, kRowMask, 0xf, false);
+    if (oe < e || (oe == e && oi < i)) {
+        e = oe;
+        i = oi;
     }
+}
+
+__device__ __forceinline__ int wave_argmin(float e, int i) {
+    argmin_step<0xB1, 0xf>(e, i);   // quad_perm [1,0,3,2]
+    argmin_step<0x4E, 0xf>(e, i);   // quad_perm [2,3,0,1]
+    argmin_step<0x141, 0xf>(e, i);  // row_half_mirror
+    argmin_step<0x140, 0xf>(e, i);  // row_mirror
+    argmin_step<0x142, 0xa>(e, i);  // row_bcast:15 into rows 1 and 3
+    argmin_step<0x143, 0xc>(e, i);  // row_bcast:31 into rows 2 and 3
+    return __builtin_amdgcn_readlane(i, 63);
 }
 
 constexpr int kNoWinner = 0x7fffffff;
 
-// Four-cluster search for every block of the tile whose owner passed n >= 2.
-// Returns, in the owning lane, the winning candidate index or -1.
-__device__ __forceinline__ int search4(const float4* lds, int my_n, f3 metric,
+// Per-workgroup LDS: candidate tables and the SAT of the block being searched.
+struct Shared {
+    uint32_t four[976];
+    uint32_t three[160];
+    float4 sat[kSatRows];
+};
+
+// Copies rows 0..nb of block b's SAT from the workgroup's workspace into LDS.
+__device__ __forceinline__ void stage_sat(Shared& sh, const float4* ws, int b, int nb) {
+    __syncthreads();  // previous block's readers are done
+    const int lane = threadIdx.x;
+    if (lane <= nb) sh.sat[lane] = ws[b * kSatRows + lane];
+    __syncthreads();
+}
+
+// Four-cluster search over every block of the tile whose owner passed
+// n >= 2; returns, in the owning lane, the winning candidate index or -1.
+__device__ __forceinline__ int search4(Shared& sh, const float4* ws, int my_n, f3 metric,
                                        const DecoderWeights& dw) {
     const int lane = threadIdx.x;
     int my_win = -1;
     for (int b = 0; b < kTile; b++) {
         const int nb = __builtin_amdgcn_readlane(my_n, b);
         if (nb < 2) continue;
-        const float4* s = lds + b * kSatRows;
-        const float4 tot = s[nb];
+        stage_sat(sh, ws, b, nb);
+        const float4 tot = sh.sat[nb];
         const int total = k_four_total[nb - 1];
         float best = FLT_MAX;
         int besti = kNoWinner;
         for (int i = lane; i < total; i += kTile) {
-            const uint32_t pk = k_four[i];
+            const uint32_t pk = sh.four[i];
             f3 a, e;
-            const float err = candidate4(s[pk & 0xff], s[(pk >> 8) & 0xff], s[pk >> 16], tot, metric,
-                                         dw, a, e);
+            const float err = candidate4(sh.sat[pk & 0xff], sh.sat[(pk >> 8) & 0xff], sh.sat[pk >> 16], tot,
+                                         metric, dw, a, e);
             if (err < best) {
                 best = err;
                 besti = i;
             }
         }
-        wave_argmin(best, besti);
-        if (lane == b && besti != kNoWinner) my_win = besti;
+        const int w = wave_argmin(best, besti);
+        if (lane == b && w != kNoWinner) my_win = w;
     }
     return my_win;
 }
 
-// Three-cluster search for every block whose owner passed n >= 1.
-__device__ __forceinline__ int search3(const float4* lds, int my_n, f3 metric) {
+// Three-cluster search over every block whose owner passed n >= 1.
+__device__ __forceinline__ int search3(Shared& sh, const float4* ws, int my_n, f3 metric) {
     const int lane = threadIdx.x;
     int my_win = -1;
     for (int b = 0; b < kTile; b++) {
         const int nb = __builtin_amdgcn_readlane(my_n, b);
         if (nb < 1) continue;
-        const float4* s = lds + b * kSatRows;
-        const float4 tot = s[nb];
+        stage_sat(sh, ws, b, nb);
+        const float4 tot = sh.sat[nb];
         const int total = k_three_total[nb - 1];
         float best = FLT_MAX;
         int besti = kNoWinner;
         for (int i = lane; i < total; i += kTile) {
-            const uint32_t pk = k_three[i];
+            const uint32_t pk = sh.three[i];
             f3 a, e;
-            const float err = candidate3(s[pk & 0xff], s[(pk >> 8) & 0xff], tot, metric, a, e);
+            const float err = candidate3(sh.sat[pk & 0xff], sh.sat[(pk >> 8) & 0xff], tot, metric, a, e);
             if (err < best) {
                 best = err;
                 besti = i;
             }
         }
-        wave_argmin(best, besti);
-        if (lane == b && besti != kNoWinner) my_win = besti;
+        const int w = wave_argmin(best, besti);
+        if (lane == b && w != kNoWinner) my_win = w;
     }
     return my_win;
 }
 
 // ---------------------------------------------------------------------------
 // One tile: the reference's compress_bc1 driver (icbc.h:3734-3808) per lane.
+// `ws` is this workgroup's workspace: 64 blocks x 17 SAT rows.
 
-__device__ void encode_tile(const JobDev& job, uint64_t first, float4* lds) {
+__device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, Shared& sh, float4* ws) {
     const int lane = threadIdx.x;
     const uint64_t bi = first + lane;
     const bool active = bi < job.block_count;
@@ -136,37 +163,9 @@ __device__ void encode_tile(const JobDev& job, uint64_t first, float4* lds) {
     const int dec = job.decoder;
     const f3 cw = mk(job.cw0, job.cw1, job.cw2);
     const DecoderWeights dw = k_dweights[dec];
-
-    f3 tex[16];
-    float wt[16];
-    if (active) {
-        const float4* src = job.colors + bi * 16;
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const float4 v = src[i];
-            tex[i] = mk(v.x, v.y, v.z);
-        }
-        if (job.weights) {
-            const float4* ws = job.weights + bi * 4;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const float4 v = ws[k];
-                wt[4 * k + 0] = v.x;
-                wt[4 * k + 1] = v.y;
-                wt[4 * k + 2] = v.z;
-                wt[4 * k + 3] = v.w;
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < 16; i++) wt[i] = 1.0f;
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            tex[i] = mk(0.0f, 0.0f, 0.0f);
-            wt[i] = 0.0f;
-        }
-    }
+    const uint64_t safe = active ? bi : 0;
+    const Texels tx{job.colors + safe * 16,
+                    job.weights ? reinterpret_cast<const float*>(job.weights + safe * 4) : nullptr};
 
     Code out{0u, 0u};
     bool written = false;
@@ -177,7 +176,7 @@ __device__ void encode_tile(const JobDev& job, uint64_t first, float4* lds) {
     bool any_black = false;
     bool done = !active;
 
-    if (p.cluster && active) n = reduce_colors(tex, wt, p.threshold, rc, rw, any_black);
+    if (p.cluster && active) n = reduce_colors(tx, p.threshold, rc, rw, any_black);
 
     if (!done && n == 0) {  // no weighted texel: trivial block (icbc.h:3752-3758)
         err = 0.0f;
@@ -186,22 +185,22 @@ __device__ void encode_tile(const JobDev& job, uint64_t first, float4* lds) {
     }
     if (!done && n == 1) {  // one colour: optimal single-colour tables (icbc.h:3761-3764)
         out = single_color(to_u8(rc[0].x), to_u8(rc[0].y), to_u8(rc[0].z), dec);
-        err = code_error(tex, wt, cw, dec, out);
+        err = code_error(tx, cw, dec, out);
         written = true;
         done = true;
     }
 
     if (p.box && !done) {  // icbc.h:3769-3786
         f3 c0, c1;
-        if (p.cluster) box_fit(rc, n, c0, c1);
-        else box_fit(tex, 16, c0, c1);
-        err = emit4(tex, wt, cw, dec, c0, c1, out);
+        if (p.cluster) box_fit([&](int i) { return rc[i]; }, n, c0, c1);
+        else box_fit([&](int i) { return tx.color(i); }, 16, c0, c1);
+        err = emit4(tx, cw, dec, c0, c1, out);
         written = true;
         if (p.lsq) {
             f3 a, b;
-            if (lsq_endpoints(out.idx, tex, a, b)) {
+            if (lsq_endpoints(out.idx, tx, a, b)) {
                 Code o2;
-                const float e2 = emit4(tex, wt, cw, dec, a, b, o2);
+                const float e2 = emit4(tx, cw, dec, a, b, o2);
                 if (e2 < err) {
                     err = e2;
                     out = o2;
@@ -212,12 +211,11 @@ __device__ void encode_tile(const JobDev& job, uint64_t first, float4* lds) {
 
     if (p.cluster) {  // icbc.h:3788-3801, 3537-3571
         const bool fit = !done;
-        float4* my_sat = lds + lane * kSatRows;
+        float4* my_sat = ws + lane * kSatRows;
         const f3 metric = cw * cw;
         if (fit) build_sat(rc, rw, n, my_sat);
         __syncthreads();
-        const int win4 = search4(lds, fit ? n : 0, metric, dw);
-        __syncthreads();
+        const int win4 = search4(sh, ws, fit ? n : 0, metric, dw);
 
         Code o4{0u, 0u};
         float e4 = 0.0f;
@@ -226,11 +224,11 @@ __device__ void encode_tile(const JobDev& job, uint64_t first, float4* lds) {
         if (fit) {
             f3 a = mk(0.0f, 0.0f, 0.0f), b = mk(0.0f, 0.0f, 0.0f);
             if (win4 >= 0) {
-                const uint32_t pk = k_four[win4];
-                candidate4(my_sat[pk & 0xff], my_sat[(pk >> 8) & 0xff], my_sat[pk >> 16], my_sat[n],
-                           metric, dw, a, b);
+                const uint32_t pk = sh.four[win4];
+                candidate4(my_sat[pk & 0xff], my_sat[(pk >> 8) & 0xff], my_sat[pk >> 16], my_sat[n], metric,
+                           dw, a, b);
             }
-            e4 = emit4(tex, wt, cw, dec, a, b, o4);
+            e4 = emit4(tx, cw, dec, a, b, o4);
             const bool use_black = any_black && job.tcb != 0;
             use3 = p.fit3_always || (use_black && p.fit3_black_only);
             if (use3) {
@@ -246,8 +244,7 @@ __device__ void encode_tile(const JobDev& job, uint64_t first, float4* lds) {
             if (use3) n3 = m;
         }
         __syncthreads();
-        const int win3 = search3(lds, use3 ? m : 0, metric);
-        __syncthreads();
+        const int win3 = search3(sh, ws, use3 ? m : 0, metric);
 
         if (fit) {
             Code oc = o4;
@@ -255,11 +252,11 @@ __device__ void encode_tile(const JobDev& job, uint64_t first, float4* lds) {
             if (use3) {
                 f3 a = mk(0.0f, 0.0f, 0.0f), b = mk(0.0f, 0.0f, 0.0f);
                 if (win3 >= 0) {
-                    const uint32_t pk = k_three[win3];
+                    const uint32_t pk = sh.three[win3];
                     candidate3(my_sat[pk & 0xff], my_sat[(pk >> 8) & 0xff], my_sat[m], metric, a, b);
                 }
                 Code o3;
-                const float e3 = emit3(tex, wt, cw, dec, job.tcb != 0, a, b, o3);
+                const float e3 = emit3(tx, cw, dec, job.tcb != 0, a, b, o3);
                 if (e3 < ec) {
                     ec = e3;
                     oc = o3;
@@ -271,6 +268,7 @@ __device__ void encode_tile(const JobDev& job, uint64_t first, float4* lds) {
                 written = true;
             }
         }
+        __syncthreads();  // workspace rows are reused by the next tile
     }
 
     if (p.refine && !done) {  // icbc.h:3803-3805
@@ -280,7 +278,7 @@ __device__ void encode_tile(const JobDev& job, uint64_t first, float4* lds) {
             out = Code{v.x, v.y};
             written = true;
         }
-        err = refine(tex, wt, cw, dec, job.tcm != 0, err, out, steps);
+        err = refine(tx, cw, dec, job.tcm != 0, err, out, steps);
     }
 
     if (active) {
@@ -293,37 +291,40 @@ __device__ void encode_tile(const JobDev& job, uint64_t first, float4* lds) {
 // ---------------------------------------------------------------------------
 // kernels
 
-__global__ __launch_bounds__(kTile) void bc1_tiles_kernel(const JobDev* __restrict__ jobs,
-                                                          const uint2* __restrict__ tiles,
-                                                          uint32_t ntiles) {
-    __shared__ float4 lds[kTile * kSatRows];
-    const uint32_t t = blockIdx.x;
-    if (t >= ntiles) return;
-    const uint2 tile = tiles[t];
-    encode_tile(jobs[tile.x], tile.y, lds);
+__device__ __forceinline__ void load_tables(Shared& sh) {
+    for (int i = threadIdx.x; i < 976; i += kTile) sh.four[i] = k_four[i];
+    for (int i = threadIdx.x; i < 160; i += kTile) sh.three[i] = k_three[i];
+    __syncthreads();
 }
 
-__global__ __launch_bounds__(kTile) void bc1_job_kernel(const JobDev* __restrict__ job,
-                                                        uint32_t ntiles) {
-    __shared__ float4 lds[kTile * kSatRows];
-    const uint32_t t = blockIdx.x;
-    if (t >= ntiles) return;
-    encode_tile(*job, (uint64_t)t * kTile, lds);
-}
-
-__global__ __launch_bounds__(kTile) void bc1_persistent_kernel(const JobDev* __restrict__ jobs,
-                                                               const uint2* __restrict__ tiles,
-                                                               uint32_t ntiles,
-                                                               uint32_t* __restrict__ counter) {
-    __shared__ float4 lds[kTile * kSatRows];
+// Resident workgroups encode tiles until none are left.  With `counter` the
+// tiles are taken from a queue in list order (the batched path); without, a
+// workgroup takes tiles blockIdx.x, blockIdx.x + gridDim.x, ...  With `tiles`
+// null the launch covers one job, tile k = blocks [64 k, 64 k + 64).
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) void bc1_encode_kernel(const JobDev* __restrict__ jobs,
+                                                           const uint2* __restrict__ tiles,
+                                                           uint32_t ntiles, uint32_t* __restrict__ counter,
+                                                           float4* __restrict__ workspace) {
+    __shared__ Shared sh;
+    load_tables(sh);
+    float4* ws = workspace + (size_t)blockIdx.x * kTile * kSatRows;
+    uint32_t t = blockIdx.x;
     for (;;) {
-        uint32_t t = 0;
-        if (threadIdx.x == 0) t = atomicAdd(counter, 1u);
-        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
-        if (t >= ntiles) break;  // every wave reaches this once the queue drains
-        const uint2 tile = tiles[t];
-        encode_tile(jobs[tile.x], tile.y, lds);
-        __syncthreads();
+        if (counter) {
+            uint32_t v = 0;
+            if (threadIdx.x == 0) v = atomicAdd(counter, 1u);
+            t = __builtin_amdgcn_readfirstlane(__shfl(v, 0, 64));
+        }
+        if (t >= ntiles) break;  // every wave reaches this once the tiles run out
+        uint32_t job = 0;
+        uint64_t first = (uint64_t)t * kTile;
+        if (tiles) {
+            const uint2 tile = tiles[t];
+            job = tile.x;
+            first = tile.y;
+        }
+        encode_tile(jobs[job], first, sh, ws);
+        if (!counter) t += gridDim.x;
     }
 }
 
@@ -390,26 +391,25 @@ __global__ void bc1_evaluate_kernel(const uint32_t* __restrict__ rgba, const uin
 // ---------------------------------------------------------------------------
 // launch glue
 
-hipError_t launch_tiles(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, hipStream_t stream) {
-    if (ntiles == 0) return hipSuccess;
-    hipLaunchKernelGGL(bc1_tiles_kernel, dim3(ntiles), dim3(kTile), 0, stream, jobs, tiles, ntiles);
-    return hipGetLastError();
+int encode_workgroups_per_cu() {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, bc1_encode_kernel, kTile, 0) != hipSuccess ||
+        blocks < 1)
+        blocks = 1;
+    return blocks;
 }
 
-hipError_t launch_job(const JobDev* job, uint32_t ntiles, hipStream_t stream) {
+hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, uint32_t* counter,
+                         float4* workspace, uint32_t grid, hipStream_t stream) {
     if (ntiles == 0) return hipSuccess;
-    hipLaunchKernelGGL(bc1_job_kernel, dim3(ntiles), dim3(kTile), 0, stream, job, ntiles);
-    return hipGetLastError();
-}
-
-hipError_t launch_persistent(const JobDev* jobs, const uint2* tiles, uint32_t ntiles,
-                             uint32_t* counter, uint32_t grid, hipStream_t stream) {
-    if (ntiles == 0) return hipSuccess;
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(bc1_persistent_kernel, dim3(grid), dim3(kTile), 0, stream, jobs, tiles,
-                       ntiles, counter);
+    if (grid > ntiles) grid = ntiles;
+    if (counter) {
+        hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(bc1_encode_kernel, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles, counter,
+                       workspace);
     return hipGetLastError();
 }
 

@@ -38,13 +38,18 @@ struct HostTables {
 
 hipError_t upload_tables(const HostTables& t, hipStream_t stream);
 
-// One workgroup per tile; tile k covers blocks [tiles[k].y, +64) of job tiles[k].x.
-hipError_t launch_tiles(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, hipStream_t stream);
-// One job: tile k covers blocks [64 k, 64 k + 64).
-hipError_t launch_job(const JobDev* job, uint32_t ntiles, hipStream_t stream);
-// Persistent workgroups pulling tiles from `counter` (reset by the same call).
-hipError_t launch_persistent(const JobDev* jobs, const uint2* tiles, uint32_t ntiles,
-                             uint32_t* counter, uint32_t grid, hipStream_t stream);
+// Workgroups of the encode kernel that fit on one compute unit at once.
+int encode_workgroups_per_cu();
+
+// Encodes `ntiles` tiles with `grid` resident workgroups (clamped to ntiles).
+//   tiles != null: tile k covers blocks [tiles[k].y, +64) of job tiles[k].x
+//   tiles == null: one job (jobs[0]), tile k covers blocks [64 k, 64 k + 64)
+//   counter != null: tiles are handed out in order from a queue (reset by this
+//   call); otherwise workgroup g takes tiles g, g + grid, ...
+// `workspace` holds kWorkspaceBytesPerWorkgroup bytes per workgroup of `grid`.
+constexpr size_t kWorkspaceBytesPerWorkgroup = (size_t)kTile * kSatRows * 16;
+hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, uint32_t* counter,
+                         float4* workspace, uint32_t grid, hipStream_t stream);
 
 hipError_t launch_decode(const uint2* codes, uint64_t n, uint32_t* rgba, int decoder,
                          hipStream_t stream);

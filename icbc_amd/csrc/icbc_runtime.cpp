@@ -176,6 +176,39 @@ int ensure_device_ready(int* decoder) {
     return ICBC_OK;
 }
 
+// Resident encode workgroups the current device holds at once.
+int resident_workgroups(uint32_t* grid) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    static std::mutex m;
+    static std::vector<std::pair<int, uint32_t>> cache;
+    std::lock_guard<std::mutex> lock(m);
+    for (auto& c : cache)
+        if (c.first == dev) {
+            *grid = c.second;
+            return ICBC_OK;
+        }
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    const uint32_t g = (uint32_t)prop.multiProcessorCount * (uint32_t)encode_workgroups_per_cu();
+    cache.push_back({dev, g});
+    *grid = g;
+    return ICBC_OK;
+}
+
+// Stream-ordered workspace for one encode launch: `grid` workgroup slices
+// and a tile-queue counter.
+struct Workspace {
+    void* base = nullptr;
+    float4* slices() const { return (float4*)((char*)base + 256); }
+    uint32_t* counter() const { return (uint32_t*)base; }
+};
+
+int workspace_alloc(uint32_t grid, hipStream_t s, Workspace& w) {
+    HIP_TRY(hipMallocAsync(&w.base, 256 + (size_t)grid * kWorkspaceBytesPerWorkgroup, s));
+    return ICBC_OK;
+}
+
 // ---------------------------------------------------------------------------
 // jobs
 
@@ -371,12 +404,21 @@ int icbc_gpu_compress_bc1_batch(const icbc_bc1_job* jobs, size_t n_jobs, void* s
     std::vector<uint2> tiles = make_tiles(p.dev);
     if (tiles.size() > 0xffffffffu) return fail(ICBC_ERR_INVALID_ARGUMENT, "batch too large");
     if (!tiles.empty()) {
+        uint32_t grid;
+        r = resident_workgroups(&grid);
+        if (r) return r;
+        grid = std::min<uint32_t>(grid, (uint32_t)tiles.size());
         void *djobs = nullptr, *dtiles = nullptr;
         r = upload_blob(p.dev.data(), p.dev.size() * sizeof(JobDev), s, &djobs);
         if (r) return r;
         r = upload_blob(tiles.data(), tiles.size() * sizeof(uint2), s, &dtiles);
         if (r) return r;
-        HIP_TRY(launch_tiles((const JobDev*)djobs, (const uint2*)dtiles, (uint32_t)tiles.size(), s));
+        Workspace w;
+        r = workspace_alloc(grid, s, w);
+        if (r) return r;
+        HIP_TRY(launch_encode((const JobDev*)djobs, (const uint2*)dtiles, (uint32_t)tiles.size(), w.counter(),
+                              w.slices(), grid, s));
+        HIP_TRY(hipFreeAsync(w.base, s));
         HIP_TRY(hipFreeAsync(djobs, s));
         HIP_TRY(hipFreeAsync(dtiles, s));
     }
@@ -392,11 +434,21 @@ int icbc_gpu_compress_bc1_plain(const icbc_bc1_job* jobs, size_t n_jobs, void* s
     r = prepare(jobs, n_jobs, s, decoder, p);
     if (r) return r;
     if (n_jobs) {
+        uint32_t resident;
+        r = resident_workgroups(&resident);
+        if (r) return r;
+        uint32_t grid = 1;
+        for (const JobDev& d : p.dev) grid = std::max(grid, std::min(resident, tiles_of(d.block_count)));
         void* djobs = nullptr;
         r = upload_blob(p.dev.data(), p.dev.size() * sizeof(JobDev), s, &djobs);
         if (r) return r;
+        Workspace w;
+        r = workspace_alloc(grid, s, w);
+        if (r) return r;
         for (size_t k = 0; k < n_jobs; k++)
-            HIP_TRY(launch_job((const JobDev*)djobs + k, tiles_of(p.dev[k].block_count), s));
+            HIP_TRY(launch_encode((const JobDev*)djobs + k, nullptr, tiles_of(p.dev[k].block_count), nullptr,
+                                  w.slices(), grid, s));
+        HIP_TRY(hipFreeAsync(w.base, s));
         HIP_TRY(hipFreeAsync(djobs, s));
     }
     return finish(jobs, n_jobs, s, p);
@@ -411,7 +463,7 @@ struct icbc_bc1_plan {
     icbc_plan_options opt{};
     JobDev* djobs = nullptr;
     uint2* dtiles = nullptr;
-    uint32_t* dcounter = nullptr;
+    void* dwork = nullptr;  // tile-queue counter (256 B) + per-workgroup slices
     uint32_t ntiles = 0;
     uint32_t grid = 0;
     hipGraph_t graph = nullptr;
@@ -428,10 +480,9 @@ float level_cost(int level) {
 }
 
 int plan_launch(icbc_bc1_plan* pl, hipStream_t s) {
-    if (pl->opt.schedule == ICBC_SCHEDULE_PERSISTENT)
-        HIP_TRY(launch_persistent(pl->djobs, pl->dtiles, pl->ntiles, pl->dcounter, pl->grid, s));
-    else
-        HIP_TRY(launch_tiles(pl->djobs, pl->dtiles, pl->ntiles, s));
+    uint32_t* counter = pl->opt.schedule == ICBC_SCHEDULE_PERSISTENT ? (uint32_t*)pl->dwork : nullptr;
+    float4* slices = (float4*)((char*)pl->dwork + 256);
+    HIP_TRY(launch_encode(pl->djobs, pl->dtiles, pl->ntiles, counter, slices, pl->grid, s));
     return ICBC_OK;
 }
 
@@ -474,16 +525,22 @@ int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_pla
         delete pl;
         return hip_fail(e, "hipGetDeviceProperties");
     }
-    const int per_cu = o.workgroups_per_cu > 0 ? o.workgroups_per_cu : 8;
-    pl->grid = (uint32_t)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * per_cu,
-                                            std::max<uint32_t>(pl->ntiles, 1u));
+    const int per_cu = o.workgroups_per_cu > 0 ? std::min(o.workgroups_per_cu, encode_workgroups_per_cu())
+                                               : encode_workgroups_per_cu();
+    if (o.schedule == ICBC_SCHEDULE_GRID) {
+        // one workgroup per tile, no queue
+        pl->grid = std::max<uint32_t>(pl->ntiles, 1u);
+    } else {
+        pl->grid = (uint32_t)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * per_cu,
+                                                std::max<uint32_t>(pl->ntiles, 1u));
+    }
     auto cleanup = [&](int code) {
         icbc_gpu_plan_destroy(pl);
         return code;
     };
     if ((e = hipMalloc(&pl->djobs, std::max<size_t>(1, dev.size()) * sizeof(JobDev))) != hipSuccess ||
         (e = hipMalloc(&pl->dtiles, std::max<size_t>(1, tiles.size()) * sizeof(uint2))) != hipSuccess ||
-        (e = hipMalloc(&pl->dcounter, sizeof(uint32_t))) != hipSuccess)
+        (e = hipMalloc(&pl->dwork, 256 + (size_t)pl->grid * kWorkspaceBytesPerWorkgroup)) != hipSuccess)
         return cleanup(hip_fail(e, "hipMalloc (plan)"));
     if (!dev.empty() &&
         (e = hipMemcpy(pl->djobs, dev.data(), dev.size() * sizeof(JobDev), hipMemcpyHostToDevice)) != hipSuccess)
@@ -533,7 +590,7 @@ int icbc_gpu_plan_destroy(icbc_bc1_plan* pl) {
     if (pl->graph) (void)hipGraphDestroy(pl->graph);
     if (pl->djobs) (void)hipFree(pl->djobs);
     if (pl->dtiles) (void)hipFree(pl->dtiles);
-    if (pl->dcounter) (void)hipFree(pl->dcounter);
+    if (pl->dwork) (void)hipFree(pl->dwork);
     delete pl;
     return ICBC_OK;
 }
@@ -566,7 +623,7 @@ int block_context(BlockContext*& ctx) {
     if (ctx->device != dev) {
         ctx->device = dev;
         HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-        HIP_TRY(hipMalloc(&ctx->dbuf, 512));
+        HIP_TRY(hipMalloc(&ctx->dbuf, 512 + 256 + kWorkspaceBytesPerWorkgroup));
         HIP_TRY(hipHostMalloc(&ctx->hbuf, 512, hipHostMallocDefault));
     }
     return ICBC_OK;
@@ -606,7 +663,7 @@ int compress_one(int level, const float* input_colors, const float* input_weight
     static_assert(sizeof(JobDev) <= 128, "job descriptor must fit the block buffer");
     memcpy(h + 384, &jd, sizeof jd);
     HIP_TRY(hipMemcpyAsync(d, h, 512, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(launch_job((const JobDev*)(d + 384), 1, ctx->stream));
+    HIP_TRY(launch_encode((const JobDev*)(d + 384), nullptr, 1, nullptr, (float4*)(d + 768), 1, ctx->stream));
     HIP_TRY(hipMemcpyAsync(h + 320, d + 320, 32, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     memcpy(output, h + 320, 8);

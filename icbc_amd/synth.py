@@ -98,9 +98,10 @@ def make_image_blocks(seed, width, height, mix):
         noisy = rng.random(len(sel)) < 0.5
         base[noisy] = rng.random((int(noisy.sum()), 16, 3), dtype=np.float32)
         k = rng.integers(1, 7, size=len(sel))
-        for i, b in enumerate(sel):
-            idx = rng.choice(16, size=k[i], replace=False)
-            base[i, idx] = rng.random((k[i], 3), dtype=np.float32) * np.float32(0.02)
+        # k distinct texels per block: the k smallest of 16 random keys
+        ranks = np.argsort(np.argsort(rng.random((len(sel), 16)), axis=1), axis=1)
+        dark = ranks < k[:, None]
+        base[dark] = rng.random((int(dark.sum()), 3), dtype=np.float32) * np.float32(0.02)
         rgb[sel] = base
 
     colors = np.empty((nb, 16, 4), dtype=np.float32)
@@ -182,3 +183,70 @@ def make_edge_blocks(seed, n):
         colors[b, :, 3] = 1.0
         weights[b] = w
     return colors.astype(np.float32), weights.astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# textures with mip chains (the batched workload)
+
+
+def image_to_blocks(img):
+    """(h, w, 4) float32 image -> block-major (nblocks, 16, 4), padding the
+    right and bottom edges by replication to multiples of 4."""
+    h, w, _ = img.shape
+    ph, pw = (h + 3) // 4 * 4, (w + 3) // 4 * 4
+    if ph != h or pw != w:
+        img = np.pad(img, ((0, ph - h), (0, pw - w), (0, 0)), mode="edge")
+    return np.ascontiguousarray(
+        img.reshape(ph // 4, 4, pw // 4, 4, 4).transpose(0, 2, 1, 3, 4).reshape(-1, 16, 4))
+
+
+def make_image(seed, width, height, mix):
+    """A width x height RGBA image (any size) with the block content ``mix``."""
+    pw, ph = (width + 3) // 4 * 4, (height + 3) // 4 * 4
+    blocks, _ = make_image_blocks(seed, pw, ph, mix)
+    return blocks_to_image(blocks, pw, ph)[:height, :width]
+
+
+def box_downsample(img):
+    """2x2 box filter; an odd trailing row/column is dropped (sizes floor to
+    at least 1)."""
+    h, w, c = img.shape
+    if h == 1 and w == 1:
+        return img
+    if h == 1:
+        return (img[:, 0:w - 1:2] + img[:, 1:w:2]) * np.float32(0.5)
+    if w == 1:
+        return (img[0:h - 1:2] + img[1:h:2]) * np.float32(0.5)
+    a = img[0:h - 1:2, 0:w - 1:2]
+    b = img[0:h - 1:2, 1:w:2]
+    c_ = img[1:h:2, 0:w - 1:2]
+    d = img[1:h:2, 1:w:2]
+    return ((a + b) + (c_ + d)) * np.float32(0.25)
+
+
+def mip_chain(img):
+    """Full box-filtered chain from ``img`` down to the first level that fits
+    in one 4x4 block."""
+    levels = [img]
+    while levels[-1].shape[0] > 4 or levels[-1].shape[1] > 4:
+        levels.append(box_downsample(levels[-1]))
+    return levels
+
+
+def texture_set(seed, count=256, min_side=64, max_side=4096, mix=None, levels=range(1, 10)):
+    """The batched workload: ``count`` textures with sides drawn log-uniform in
+    [min_side, max_side] (width and height independent), each with its full
+    mip chain, and a quality level drawn uniformly from ``levels`` per texture.
+    Returns a list of (texture, mip, width, height, level, blocks)."""
+    mix = mix or MIX_DEFAULT
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(min_side), np.log(max_side)
+    sides = np.exp(rng.uniform(lo, hi, size=(count, 2))).round().astype(int)
+    qual = rng.choice(list(levels), size=count)
+    out = []
+    for t in range(count):
+        w, h = int(sides[t, 0]), int(sides[t, 1])
+        img = make_image(seed * 1000 + t, w, h, mix)
+        for m, lv in enumerate(mip_chain(img)):
+            out.append((t, m, lv.shape[1], lv.shape[0], int(qual[t]), image_to_blocks(lv)))
+    return out
