@@ -201,7 +201,7 @@ def main():
     roofline = {"bound": "valu-fp32", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                 "frac_of_unfused_peak": round(achieved / PEAK_FP32_NOFMA_TFLOPS, 4),
-                "traffic": None, "kernel": "bc1_persistent_kernel",
+                "traffic": None, "kernel": "bc1_encode_kernel",
                 "algorithmic_gflop_per_launch": round(flops / 1e9, 3),
                 "mean_fit4_candidates_per_block": round(float(np.mean(
                     np.where(stats[:, 0] >= 2, (stats[:, 0] + 3) * (stats[:, 0] + 2) * (stats[:, 0] + 1) // 6 - 1, 0))), 2)}
