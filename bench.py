@@ -107,6 +107,106 @@ def cpu_baseline(blocks, level, tcm, tcb, threads, budget_cpu_s=20.0):
                       f"{threads} threads, chunks of 32 blocks; {secs:.2f} s wall"}
 
 
+def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64 << 30,
+                  max_inflight=2):
+    """Poisson request stream against the request server: textures with sides
+    log-uniform in [16, 2048], level uniform in 1..9, three-colour mode on for
+    half of them, block mix 40/30/20/10, stored as RGBA8 (read by the encoder
+    as c / 255.0f).  Inputs sit in page-locked host memory and each request's
+    texels cross PCIe inside its latency.  First the whole
+    pool is submitted at once to find the saturation rate; then each QPS
+    fraction runs for `seconds`.  Every served result is compared with the
+    same texture encoded alone."""
+    import queue
+    import threading
+
+    rng = np.random.default_rng(seed)
+    pool = []
+    for k in range(pool_size):
+        w = int(round(np.exp(rng.uniform(np.log(16), np.log(2048)))))
+        h = int(round(np.exp(rng.uniform(np.log(16), np.log(2048)))))
+        blocks = synth.image_to_blocks(synth.make_image(seed * 10000 + k, w, h, synth.MIX_DEFAULT))
+        blocks8 = np.clip(np.round(blocks * 255), 0, 255).astype(np.uint8)   # RGBA8 texels
+        holder = g.pinned_copy(blocks8)
+        pool.append({"blocks": holder.array, "holder": holder, "n": len(blocks),
+                     "level": int(rng.integers(1, 10)), "tcm": bool(rng.integers(0, 2))})
+    alone = [g.compress_many(p["level"], p["blocks"], None, (1, 1, 1), p["tcm"], p["tcm"])[0] for p in pool]
+    srv = g.Server(budget, max_inflight=max_inflight)
+
+    def submit(k, codes):
+        p = pool[k]
+        return srv.submit(p["level"], p["blocks"], None, (1, 1, 1), p["tcm"], p["tcm"], codes=codes)
+
+    t0 = time.perf_counter()
+    tickets = [submit(k, np.zeros((pool[k]["n"], 8), np.uint8)) for k in range(pool_size)]
+    for t in tickets:
+        srv.wait(t)
+    sat_s = time.perf_counter() - t0
+    total_blocks = sum(p["n"] for p in pool)
+    sat_qps = pool_size / sat_s
+    log(f"serving saturation: {sat_qps:.1f} requests/s, {total_blocks / sat_s / 1e6:.1f} Mblocks/s "
+        f"(pool {pool_size} textures, mean {total_blocks / pool_size:.0f} blocks)")
+
+    runs = []
+    for frac in qps_fracs:
+        rate = frac * sat_qps
+        q = queue.Queue()
+        infos = []
+        bad = [0]
+
+        def waiter():
+            while True:
+                item = q.get()
+                if item is None:
+                    return
+                ticket, k, codes = item
+                _, _, info = srv.wait(ticket)
+                if not np.array_equal(codes, alone[k]):
+                    bad[0] += 1
+                infos.append(info)
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        start = time.perf_counter()
+        t_next = start
+        arrivals = 0
+        while True:
+            t_next += rng.exponential(1.0 / rate)
+            if t_next - start > seconds:
+                break
+            while True:
+                d = t_next - time.perf_counter()
+                if d <= 0:
+                    break
+                time.sleep(d if d > 0.002 else 0)
+            k = int(rng.integers(0, pool_size))
+            codes = np.empty((pool[k]["n"], 8), np.uint8)
+            q.put((submit(k, codes), k, codes))
+            arrivals += 1
+        q.put(None)
+        th.join()
+        lat = np.array([i.done_s - i.submit_s for i in infos]) * 1e3
+        blocks = sum(i.block_count for i in infos)
+        span = max(i.done_s for i in infos) - min(i.submit_s for i in infos)
+        runs.append({"qps_target": round(rate, 2), "fraction_of_saturation": frac, "requests": len(infos),
+                     "mblocks_per_s": round(blocks / span / 1e6, 3),
+                     "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+                     "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+                     "latency_ms_max": round(float(lat.max()), 3),
+                     "mean_batch_requests": round(float(np.mean([i.batch_size for i in infos])), 2),
+                     "results_identical_to_alone": bad[0] == 0})
+        log(f"serving @ {rate:.1f} qps: {runs[-1]}")
+    srv.close()
+    for p in pool:
+        p["holder"].close()
+    return {"saturation_requests_per_s": round(sat_qps, 2),
+            "saturation_mblocks_per_s": round(total_blocks / sat_s / 1e6, 3),
+            "pool_textures": pool_size, "mean_blocks_per_request": round(total_blocks / pool_size, 1),
+            "memory_budget_bytes": budget, "max_inflight_launches": max_inflight,
+            "seconds_per_rate": seconds, "runs": runs,
+            "input": "RGBA8 texels (64 B/block) in page-locked host memory, copied per request"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +219,11 @@ def main():
                     help="per-texture level drawn from 1..9 instead of the fixed level")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serve", action="store_true",
+                    help="run only the serving benchmark: Poisson arrivals at 50/80/95%% of saturation, 60 s each")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="length of the short serving run included in the default benchmark")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -132,6 +237,18 @@ def main():
     g.init(g.Decoder.D3D10)
     dev = g.device_name()
     log(f"device: {dev}")
+
+    if args.serve:
+        res = serving_bench(g, synth, 60.0, (0.5, 0.8, 0.95))
+        top = res["runs"][-1]
+        print(json.dumps({"metric": "BC1 serving: Mblocks/s and p50/p99 request latency under Poisson load",
+                          "value": top["mblocks_per_s"], "unit": "Mblocks/s", "n_gpus": 1,
+                          "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+                          "p99_ms": top["latency_ms_p99"], "p50_ms": top["latency_ms_p50"],
+                          "config": {"workload": "Poisson requests, sides log-uniform 16-2048, levels 1-9, "
+                                                 "three_color_mode 50/50, 64 GB budget", "device": dev},
+                          "serving": res}), flush=True)
+        return
 
     t0 = time.time()
     levels = range(1, 10) if args.mixed_levels else [args.level]
@@ -213,6 +330,10 @@ def main():
         cpu = cpu_baseline(colors, sample_lv, tcm, tcb, threads)
         log(f"cpu baseline: {cpu}")
 
+    serving = None
+    if not args.no_serve:
+        serving = serving_bench(g, synth, args.serve_seconds, (0.8,))
+
     value = total / ms_batched / 1e3
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mblocks/s", "n_gpus": 1,
@@ -229,6 +350,7 @@ def main():
         "batched_over_plain": (round(result_plain["ms_per_step"] / ms_batched, 4) if result_plain else None),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     plan.close()

@@ -55,8 +55,10 @@ ERR_DEVICE = -4
 ERR_NO_DEVICE = -5
 ERR_BUDGET = -6
 ERR_CANCELLED = -7
+ERR_TIMEOUT = -8
 
 JOB_HOST_MEMORY = 1
+JOB_COLORS_RGBA8 = 2
 SCHEDULE_GRID = 0
 SCHEDULE_PERSISTENT = 1
 
@@ -71,9 +73,24 @@ class Job(C.Structure):
     """icbc_bc1_job (include/icbc_gpu.h)."""
     _fields_ = [("level", C.c_int32), ("three_color_mode", C.c_int32),
                 ("three_color_black", C.c_int32), ("flags", C.c_uint32),
-                ("color_weights", C.c_float * 3), ("reserved", C.c_uint32),
+                ("color_weights", C.c_float * 3), ("tile_blocks", C.c_uint32),
                 ("block_count", C.c_uint64), ("colors", C.c_void_p), ("weights", C.c_void_p),
                 ("codes", C.c_void_p), ("errors", C.c_void_p), ("stats", C.c_void_p)]
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("memory_budget", C.c_uint64), ("max_inflight", C.c_uint32), ("max_batch", C.c_uint32)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("batch_size", C.c_uint32), ("block_count", C.c_uint64),
+                ("submit_s", C.c_double), ("admit_s", C.c_double), ("launch_s", C.c_double),
+                ("done_s", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("launches", C.c_uint64), ("completed", C.c_uint64), ("blocks", C.c_uint64),
+                ("pending", C.c_uint64), ("inflight", C.c_uint64), ("memory_in_use", C.c_uint64)]
 
 
 class PlanOptions(C.Structure):
@@ -101,6 +118,8 @@ _SIGNATURES = {
     "icbc_gpu_version": (C.c_int, []),
     "icbc_gpu_malloc": (C.c_void_p, [C.c_size_t]),
     "icbc_gpu_free": (C.c_int, [C.c_void_p]),
+    "icbc_gpu_host_alloc": (C.c_void_p, [C.c_size_t]),
+    "icbc_gpu_host_free": (C.c_int, [C.c_void_p]),
     "icbc_gpu_memcpy_htod": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "icbc_gpu_memcpy_dtoh": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "icbc_gpu_memset": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t]),
@@ -112,6 +131,12 @@ _SIGNATURES = {
     "icbc_gpu_event_record": (C.c_int, [C.c_void_p, C.c_void_p]),
     "icbc_gpu_event_synchronize": (C.c_int, [C.c_void_p]),
     "icbc_gpu_event_elapsed_ms": (C.c_float, [C.c_void_p, C.c_void_p]),
+    "icbc_server_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "icbc_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
+    "icbc_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.c_void_p]),
+    "icbc_server_cancel": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "icbc_server_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "icbc_server_destroy": (C.c_int, [C.c_void_p]),
 }
 
 _lib = None
@@ -227,6 +252,33 @@ class DeviceBuffer:
             pass
 
 
+class PinnedArray:
+    """A numpy array in page-locked host memory (icbc_gpu_host_alloc)."""
+
+    def __init__(self, shape, dtype):
+        dtype = np.dtype(dtype)
+        nbytes = int(np.prod(shape)) * dtype.itemsize
+        p = lib().icbc_gpu_host_alloc(max(1, nbytes))
+        if not p:
+            raise IcbcError(ERR_OUT_OF_MEMORY, last_error())
+        self.ptr = p
+        buf = (C.c_uint8 * max(1, nbytes)).from_address(p)
+        self.array = np.frombuffer(buf, dtype=np.uint8, count=nbytes).view(dtype).reshape(shape)
+
+    def close(self):
+        if self.ptr:
+            self.array = None
+            lib().icbc_gpu_host_free(self.ptr)
+            self.ptr = None
+
+
+def pinned_copy(a):
+    """Copy of `a` in page-locked host memory (keep the returned holder alive)."""
+    h = PinnedArray(a.shape, a.dtype)
+    h.array[...] = a
+    return h
+
+
 class Stream:
     def __init__(self):
         self.handle = lib().icbc_gpu_stream_create()
@@ -275,7 +327,7 @@ def device_name():
 
 def make_job(level, block_count, colors, codes, weights=None, errors=None, stats=None,
              color_weights=(1.0, 1.0, 1.0), three_color_mode=False, three_color_black=False,
-             host=False):
+             host=False, tile_blocks=0, rgba8=False):
     """Builds an icbc_bc1_job; pointer arguments are ints (device) or numpy arrays (host)."""
     def addr(x):
         if x is None:
@@ -289,10 +341,10 @@ def make_job(level, block_count, colors, codes, weights=None, errors=None, stats
     j.level = int(level)
     j.three_color_mode = int(bool(three_color_mode))
     j.three_color_black = int(bool(three_color_black))
-    j.flags = JOB_HOST_MEMORY if host else 0
+    j.flags = (JOB_HOST_MEMORY if host else 0) | (JOB_COLORS_RGBA8 if rgba8 else 0)
     for k in range(3):
         j.color_weights[k] = float(color_weights[k])
-    j.reserved = 0
+    j.tile_blocks = int(tile_blocks)
     j.block_count = int(block_count)
     j.colors = addr(colors)
     j.weights = addr(weights)
@@ -317,16 +369,18 @@ def compress_batch(jobs, stream=None, plain=False):
 def compress_many(level, colors, weights=None, color_weights=(1.0, 1.0, 1.0),
                   three_color_mode=False, three_color_black=False, prior=None, with_stats=False,
                   plain=False):
-    """Encodes block-major host arrays: colors (n, 16, 4) float32, weights (n, 16)
-    or None.  Returns (codes uint8[n, 8], errors float32[n], stats int32[n, 4] | None)."""
-    colors = np.ascontiguousarray(colors, dtype=np.float32)
+    """Encodes block-major host arrays: colors (n, 16, 4) float32 -- or uint8,
+    read as c / 255.0f -- and weights (n, 16) or None.  Returns
+    (codes uint8[n, 8], errors float32[n], stats int32[n, 4] | None)."""
+    rgba8 = np.asarray(colors).dtype == np.uint8
+    colors = np.ascontiguousarray(colors, dtype=np.uint8 if rgba8 else np.float32)
     n = colors.shape[0]
     w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
     codes = np.zeros((n, 8), np.uint8) if prior is None else np.array(prior, np.uint8, copy=True)
     errors = np.zeros(n, np.float32)
     stats = np.zeros((n, 4), np.int32) if with_stats else None
     job = make_job(level, n, colors, codes, w, errors, stats, color_weights, three_color_mode,
-                   three_color_black, host=True)
+                   three_color_black, host=True, rgba8=rgba8)
     compress_batch([job], plain=plain)
     return codes, errors, stats
 
@@ -388,3 +442,62 @@ def evaluate_bc1_error(rgba_block, block, decoder=Decoder.D3D10):
     e = float(d.download(np.float32, 1, offset=80)[0])
     d.close()
     return e
+
+
+# ---------------------------------------------------------------------------
+# request server (continuous batching under a device-memory budget)
+
+
+class Server:
+    """icbc_server_*: textures submitted as requests are admitted by memory
+    budget, batched with whatever else is waiting, and run while earlier
+    launches are still on the device."""
+
+    def __init__(self, memory_budget, max_inflight=2, max_batch=0):
+        opt = ServerOptions(int(memory_budget), int(max_inflight), int(max_batch))
+        h = C.c_void_p()
+        check(lib().icbc_server_create(C.byref(opt), C.byref(h)))
+        self.handle = h
+        self._keep = {}
+
+    def submit(self, level, colors, weights=None, color_weights=(1.0, 1.0, 1.0),
+               three_color_mode=False, three_color_black=False, codes=None, errors=None,
+               tile_blocks=0):
+        """Queues one texture (block-major host arrays, float32 or uint8 RGBA).
+        Returns a ticket."""
+        rgba8 = colors.dtype == np.uint8
+        colors = np.ascontiguousarray(colors, dtype=np.uint8 if rgba8 else np.float32)
+        n = colors.shape[0]
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+        codes = np.zeros((n, 8), np.uint8) if codes is None else codes
+        errors = np.zeros(n, np.float32) if errors is None else errors
+        job = make_job(level, n, colors, codes, w, errors, None, color_weights, three_color_mode,
+                       three_color_black, host=True, tile_blocks=tile_blocks, rgba8=rgba8)
+        t = C.c_uint64()
+        check(lib().icbc_server_submit(self.handle, C.byref(job), C.byref(t)))
+        self._keep[t.value] = (colors, w, codes, errors)
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        """Waits for a request; returns (codes, errors, info) or raises."""
+        info = RequestInfo()
+        r = lib().icbc_server_wait(self.handle, ticket, float(timeout), C.byref(info))
+        if r == ERR_TIMEOUT:
+            raise TimeoutError(f"request {ticket} still running")
+        colors, w, codes, errors = self._keep.pop(ticket)
+        if r != OK:
+            raise IcbcError(r, last_error() if r != info.status else f"request failed ({r})")
+        return codes, errors, info
+
+    def cancel(self, ticket):
+        check(lib().icbc_server_cancel(self.handle, ticket))
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().icbc_server_stats(self.handle, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().icbc_server_destroy(self.handle)
+            self.handle = None

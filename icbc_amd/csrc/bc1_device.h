@@ -333,43 +333,71 @@ __device__ __forceinline__ bool near3(f3 a, f3 b, float eps) {
     return fabsf(a.x - b.x) < eps && fabsf(a.y - b.y) < eps && fabsf(a.z - b.z) < eps;
 }
 
+// The colour-reduction, principal-axis and sort stages below keep their
+// arrays in registers: every loop is fully unrolled so each array index is a
+// compile-time constant, and data-dependent choices (which slot a texel merges
+// into, where an appended colour goes, whether a sort step swaps) become
+// predicated register moves.  This gives the same results as the reference's
+// index-driven loops without per-lane memory traffic.
+
 // Merge each weighted texel into the first earlier colour within `threshold`
-// (running weighted average), else append it.  Texels with weight <= 0 (or
-// NaN) are dropped.
-__device__ __forceinline__ int reduce_colors(const Texels& t, float threshold, f3 cols[16], float ws[16],
+// (running weighted average), else append it; texels with weight <= 0 (or
+// NaN) are dropped (icbc.h:1683-1728).  Returns the colour count.
+__device__ __forceinline__ int reduce_colors(const Texels& t, float threshold, f3 (&c)[16], float (&w)[16],
                                              bool& any_black) {
     int n = 0;
     any_black = false;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        c[j] = mk(0.0f, 0.0f, 0.0f);
+        w[j] = 0.0f;
+    }
+#pragma unroll
     for (int i = 0; i < 16; i++) {
         const float wi = t.weight(i);
-        if (!(wi > 0)) continue;
         const f3 ci = t.color(i);
-        int j = 0;
-        while (j < n && !near3(cols[j], ci, threshold)) j++;
-        if (j < n) {
-            const float wj = ws[j];
-            const f3 s = cols[j] * wj + ci * wi;
-            const float d = wj + wi;
-            cols[j] = mk(s.x / d, s.y / d, s.z / d);
-            ws[j] = wj + wi;
-        } else {
-            cols[n] = ci;
-            ws[n] = wi;
-            n++;
+        const bool valid = wi > 0;
+        bool found = false;
+#pragma unroll
+        for (int j = 0; j < i; j++) {  // slot j holds a colour only when j < n (n <= i here)
+            const bool m = valid && !found && j < n && near3(c[j], ci, threshold);
+            if (m) {
+                const float wj = w[j];
+                const f3 s = c[j] * wj + ci * wi;
+                const float d = wj + wi;
+                c[j] = mk(s.x / d, s.y / d, s.z / d);
+                w[j] = wj + wi;
+            }
+            found = found || m;
         }
-        if (is_black(ci)) any_black = true;
+        const bool add = valid && !found;
+#pragma unroll
+        for (int j = 0; j <= i; j++) {
+            if (add && j == n) {
+                c[j] = ci;
+                w[j] = wi;
+            }
+        }
+        n += add ? 1 : 0;
+        if (valid && is_black(ci)) any_black = true;
     }
     return n;
 }
 
-__device__ __forceinline__ int drop_blacks(const f3 cols[16], const float ws[16], int n, f3 out[16],
-                                           float wout[16]) {
+// Removes black colours, keeping the order of the rest (icbc.h:1730-1748).
+__device__ __forceinline__ int drop_blacks(f3 (&c)[16], float (&w)[16], int n) {
     int m = 0;
-    for (int i = 0; i < n; i++) {
-        if (is_black(cols[i])) continue;
-        out[m] = cols[i];
-        wout[m] = ws[i];
-        m++;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const bool keep = i < n && !is_black(c[i]);
+#pragma unroll
+        for (int k = 0; k < i; k++) {  // m <= i: move colour i down to slot m
+            if (keep && k == m) {
+                c[k] = c[i];
+                w[k] = w[i];
+            }
+        }
+        m += keep ? 1 : 0;
     }
     return m;
 }
@@ -378,25 +406,31 @@ __device__ __forceinline__ int drop_blacks(const f3 cols[16], const float ws[16]
 // Principal axis: weighted centroid, covariance, 6 power-method steps
 // (icbc.h:1752-1844).
 
-__device__ __forceinline__ f3 principal_axis(const f3 pts[16], const float ws[16], int n) {
+__device__ __forceinline__ f3 principal_axis(const f3 (&pts)[16], const float (&ws)[16], int n) {
     float total = 0.0f;
     f3 c = mk(0.0f, 0.0f, 0.0f);
-    for (int i = 0; i < n; i++) {
-        total += ws[i];
-        c = c + pts[i] * ws[i];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        if (i < n) {
+            total += ws[i];
+            c = c + pts[i] * ws[i];
+        }
     }
     c = c * (1.0f / total);
 
     float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f, m4 = 0.0f, m5 = 0.0f;
-    for (int i = 0; i < n; i++) {
-        f3 a = pts[i] - c;
-        f3 b = a * ws[i];
-        m0 += a.x * b.x;
-        m1 += a.x * b.y;
-        m2 += a.x * b.z;
-        m3 += a.y * b.y;
-        m4 += a.y * b.z;
-        m5 += a.z * b.z;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        if (i < n) {
+            f3 a = pts[i] - c;
+            f3 b = a * ws[i];
+            m0 += a.x * b.x;
+            m1 += a.x * b.y;
+            m2 += a.x * b.z;
+            m3 += a.y * b.y;
+            m4 += a.y * b.z;
+            m5 += a.z * b.z;
+        }
     }
     if (m0 == 0 && m3 == 0 && m5 == 0) return mk(0.0f, 0.0f, 0.0f);
 
@@ -415,45 +449,57 @@ __device__ __forceinline__ f3 principal_axis(const f3 pts[16], const float ws[16
 }
 
 // Summed-area table of the colours sorted along the principal axis
-// (icbc.h:1857-1937).  Row k+1 of `sat` (float4 r,g,b,w) holds the prefix
-// over the first k+1 sorted colours; row 0 holds zeros so that a cluster
-// boundary of 0 reads a zero sum.  Only rows 0..n are written.
-__device__ __forceinline__ void build_sat(const f3 cols[16], const float ws[16], int n, float4* sat) {
-    f3 axis = principal_axis(cols, ws, n);
-    int order[16];
+// (icbc.h:1857-1937).  The colours are sorted in place (a stable insertion
+// sort with strict <, the reference's loop, which behaves the same with NaN
+// keys).  Row k+1 of `sat` (float4 r,g,b,w) holds the prefix over the first
+// k+1 sorted colours; row 0 holds zeros so that a cluster boundary of 0 reads
+// a zero sum.  Rows 0..n are written.
+// With `copy` non-null the rows are written there as well.
+__device__ __forceinline__ void build_sat(f3 (&c)[16], float (&w)[16], int n, float4* sat, float4* copy) {
+    const f3 axis = principal_axis(c, w, n);
     float key[16];
-    for (int i = 0; i < n; i++) {
-        order[i] = i;
-        key[i] = dot3(cols[i], axis);
-    }
-    // stable insertion sort, strict < (identical to the reference's loop even with NaN keys)
-    for (int i = 1; i < n; i++) {
-        for (int j = i; j > 0 && key[j] < key[j - 1]; j--) {
-            float tk = key[j];
-            key[j] = key[j - 1];
-            key[j - 1] = tk;
-            int to = order[j];
-            order[j] = order[j - 1];
-            order[j - 1] = to;
+#pragma unroll
+    for (int i = 0; i < 16; i++) key[i] = dot3(c[i], axis);
+#pragma unroll
+    for (int i = 1; i < 16; i++) {
+        bool moving = i < n;
+#pragma unroll
+        for (int j = i; j > 0; j--) {
+            moving = moving && (key[j] < key[j - 1]);
+            if (moving) {
+                const float tk = key[j];
+                key[j] = key[j - 1];
+                key[j - 1] = tk;
+                const f3 tc = c[j];
+                c[j] = c[j - 1];
+                c[j - 1] = tc;
+                const float tw = w[j];
+                w[j] = w[j - 1];
+                w[j - 1] = tw;
+            }
         }
     }
     sat[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float r = 0.0f, g = 0.0f, b = 0.0f, w = 0.0f;
-    for (int i = 0; i < n; i++) {
-        const float wi = ws[order[i]];
-        const f3 c = cols[order[i]];
-        if (i == 0) {
-            r = c.x * wi;
-            g = c.y * wi;
-            b = c.z * wi;
-            w = wi;
-        } else {
-            r = r + c.x * wi;
-            g = g + c.y * wi;
-            b = b + c.z * wi;
-            w = w + wi;
+    if (copy) copy[0] = sat[0];
+    float r = 0.0f, g = 0.0f, b = 0.0f, s = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        if (i < n) {
+            const float wi = w[i];
+            if (i == 0) {
+                r = c[i].x * wi;
+                g = c[i].y * wi;
+                b = c[i].z * wi;
+                s = wi;
+            } else {
+                r = r + c[i].x * wi;
+                g = g + c[i].y * wi;
+                b = b + c[i].z * wi;
+                s = s + wi;
+            }
+            sat[i + 1] = make_float4(r, g, b, s);
+            if (copy) copy[i + 1] = sat[i + 1];
         }
-        sat[i + 1] = make_float4(r, g, b, w);
     }
 }
 
@@ -544,10 +590,13 @@ __device__ __forceinline__ bool lsq_endpoints(uint32_t indices, const Texels& t,
 template <typename Get>
 __device__ __forceinline__ void box_fit(Get get, int n, f3& c0, f3& c1) {
     f3 hi = mk(0.0f, 0.0f, 0.0f), lo = mk(1.0f, 1.0f, 1.0f);
-    for (int i = 0; i < n; i++) {
-        const f3 c = get(i);
-        hi = mk(max_r(hi.x, c.x), max_r(hi.y, c.y), max_r(hi.z, c.z));
-        lo = mk(min_r(lo.x, c.x), min_r(lo.y, c.y), min_r(lo.z, c.z));
+#pragma unroll
+    for (int i = 0; i < 16; i++) {  // unrolled with a guard: colour arrays stay in registers
+        if (i < n) {
+            const f3 c = get(i);
+            hi = mk(max_r(hi.x, c.x), max_r(hi.y, c.y), max_r(hi.z, c.z));
+            lo = mk(min_r(lo.x, c.x), min_r(lo.y, c.y), min_r(lo.z, c.z));
+        }
     }
     const float bias = (8.0f / 255.0f) / 16.0f;
     const f3 d = hi - lo;
@@ -556,10 +605,13 @@ __device__ __forceinline__ void box_fit(Get get, int n, f3& c0, f3& c1) {
     lo = sat3(lo + inset);
     const f3 center = (hi + lo) * 0.5f;
     float cxz = 0.0f, cyz = 0.0f;
-    for (int i = 0; i < n; i++) {
-        const f3 t = get(i) - center;
-        cxz += t.x * t.z;
-        cyz += t.y * t.z;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        if (i < n) {
+            const f3 t = get(i) - center;
+            cxz += t.x * t.z;
+            cyz += t.y * t.z;
+        }
     }
     float x0 = hi.x, y0 = hi.y, x1 = lo.x, y1 = lo.y;
     if (cxz < 0) {

@@ -79,75 +79,81 @@ __device__ __forceinline__ int wave_argmin(float e, int i) {
 
 constexpr int kNoWinner = 0x7fffffff;
 
-// Per-workgroup LDS: candidate tables and the SAT of the block being searched.
+// Per-workgroup LDS: candidate tables and two SAT buffers (the block being
+// searched and the next one).
 struct Shared {
     uint32_t four[976];
     uint32_t three[160];
-    float4 sat[kSatRows];
+    float4 sat[2][kSatRows];
 };
 
-// Copies rows 0..nb of block b's SAT from the workgroup's workspace into LDS.
-__device__ __forceinline__ void stage_sat(Shared& sh, const float4* ws, int b, int nb) {
-    __syncthreads();  // previous block's readers are done
+// First block b >= from (wave-uniform) whose owner lane holds my_n >= min_n.
+__device__ __forceinline__ int next_block(int my_n, int tile_blocks, int from, int min_n) {
+    int b = from;
+    while (b < tile_blocks && __builtin_amdgcn_readlane(my_n, b) < min_n) b++;
+    return b;
+}
+
+// Wave-cooperative cluster search over every block of the tile whose owner
+// passed my_n >= kMinN: for each such block the 64 lanes split its candidates
+// (table `tab`, `total_of(n)` entries) and the DPP argmin picks the winner,
+// returned in the owning lane (-1 if none).  SAT rows stream from the
+// workgroup's workspace through a double-buffered LDS slot: the next block's
+// rows are loaded into registers while the current block is searched.
+template <int kClusters>
+__device__ __forceinline__ int search(Shared& sh, const float4* ws, int tile_blocks, int my_n, f3 metric,
+                                      const DecoderWeights& dw) {
+    constexpr int kMinN = kClusters == 4 ? 2 : 1;
     const int lane = threadIdx.x;
-    if (lane <= nb) sh.sat[lane] = ws[b * kSatRows + lane];
+    int my_win = -1;
+    int b = next_block(my_n, tile_blocks, 0, kMinN);
+    int nb = b < tile_blocks ? __builtin_amdgcn_readlane(my_n, b) : 0;
+    float4 row = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (b < tile_blocks && lane <= nb) row = ws[b * kSatRows + lane];
+    int buf = 0;
+    while (b < tile_blocks) {
+        if (lane <= nb) sh.sat[buf][lane] = row;
+        __syncthreads();
+        // prefetch the next block's rows while this one is searched
+        const int b2 = next_block(my_n, tile_blocks, b + 1, kMinN);
+        const int nb2 = b2 < tile_blocks ? __builtin_amdgcn_readlane(my_n, b2) : 0;
+        if (b2 < tile_blocks && lane <= nb2) row = ws[b2 * kSatRows + lane];
+
+        const float4* sat = sh.sat[buf];
+        const float4 tot = sat[nb];
+        float best = FLT_MAX;
+        int besti = kNoWinner;
+        if (kClusters == 4) {
+            const int total = k_four_total[nb - 1];
+            for (int i = lane; i < total; i += kTile) {
+                const uint32_t pk = sh.four[i];
+                f3 a, e;
+                const float err = candidate4(sat[pk & 0xff], sat[(pk >> 8) & 0xff], sat[pk >> 16], tot, metric,
+                                             dw, a, e);
+                if (err < best) {
+                    best = err;
+                    besti = i;
+                }
+            }
+        } else {
+            const int total = k_three_total[nb - 1];
+            for (int i = lane; i < total; i += kTile) {
+                const uint32_t pk = sh.three[i];
+                f3 a, e;
+                const float err = candidate3(sat[pk & 0xff], sat[(pk >> 8) & 0xff], tot, metric, a, e);
+                if (err < best) {
+                    best = err;
+                    besti = i;
+                }
+            }
+        }
+        const int w = wave_argmin(best, besti);
+        if (lane == b && w != kNoWinner) my_win = w;
+        b = b2;
+        nb = nb2;
+        buf ^= 1;
+    }
     __syncthreads();
-}
-
-// Four-cluster search over every block of the tile whose owner passed
-// n >= 2; returns, in the owning lane, the winning candidate index or -1.
-__device__ __forceinline__ int search4(Shared& sh, const float4* ws, int my_n, f3 metric,
-                                       const DecoderWeights& dw) {
-    const int lane = threadIdx.x;
-    int my_win = -1;
-    for (int b = 0; b < kTile; b++) {
-        const int nb = __builtin_amdgcn_readlane(my_n, b);
-        if (nb < 2) continue;
-        stage_sat(sh, ws, b, nb);
-        const float4 tot = sh.sat[nb];
-        const int total = k_four_total[nb - 1];
-        float best = FLT_MAX;
-        int besti = kNoWinner;
-        for (int i = lane; i < total; i += kTile) {
-            const uint32_t pk = sh.four[i];
-            f3 a, e;
-            const float err = candidate4(sh.sat[pk & 0xff], sh.sat[(pk >> 8) & 0xff], sh.sat[pk >> 16], tot,
-                                         metric, dw, a, e);
-            if (err < best) {
-                best = err;
-                besti = i;
-            }
-        }
-        const int w = wave_argmin(best, besti);
-        if (lane == b && w != kNoWinner) my_win = w;
-    }
-    return my_win;
-}
-
-// Three-cluster search over every block whose owner passed n >= 1.
-__device__ __forceinline__ int search3(Shared& sh, const float4* ws, int my_n, f3 metric) {
-    const int lane = threadIdx.x;
-    int my_win = -1;
-    for (int b = 0; b < kTile; b++) {
-        const int nb = __builtin_amdgcn_readlane(my_n, b);
-        if (nb < 1) continue;
-        stage_sat(sh, ws, b, nb);
-        const float4 tot = sh.sat[nb];
-        const int total = k_three_total[nb - 1];
-        float best = FLT_MAX;
-        int besti = kNoWinner;
-        for (int i = lane; i < total; i += kTile) {
-            const uint32_t pk = sh.three[i];
-            f3 a, e;
-            const float err = candidate3(sh.sat[pk & 0xff], sh.sat[(pk >> 8) & 0xff], tot, metric, a, e);
-            if (err < best) {
-                best = err;
-                besti = i;
-            }
-        }
-        const int w = wave_argmin(best, besti);
-        if (lane == b && w != kNoWinner) my_win = w;
-    }
     return my_win;
 }
 
@@ -157,8 +163,9 @@ __device__ __forceinline__ int search3(Shared& sh, const float4* ws, int my_n, f
 
 __device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, Shared& sh, float4* ws) {
     const int lane = threadIdx.x;
+    const int tile_blocks = job.tile_blocks;
     const uint64_t bi = first + lane;
-    const bool active = bi < job.block_count;
+    const bool active = lane < tile_blocks && bi < job.block_count;
     const Plan p = make_plan(job.level, job.tcm != 0, job.tcb != 0);
     const int dec = job.decoder;
     const f3 cw = mk(job.cw0, job.cw1, job.cw2);
@@ -210,50 +217,52 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, S
     }
 
     if (p.cluster) {  // icbc.h:3788-3801, 3537-3571
+        // workspace slices: SATs for the 4-cluster and 3-cluster fits, and the
+        // reduced colours parked in their original order for skip_blacks
+        float4* sat4 = ws + lane * kSatRows;
+        float4* sat3 = ws + (kTile + lane) * kSatRows;
+        float4* park = ws + 2 * kTile * kSatRows + lane * 16;
         const bool fit = !done;
-        float4* my_sat = ws + lane * kSatRows;
         const f3 metric = cw * cw;
-        if (fit) build_sat(rc, rw, n, my_sat);
+        const bool use_black = fit && any_black && job.tcb != 0;
+        bool use3 = fit && (p.fit3_always || (use_black && p.fit3_black_only));
+        int m = n;
+        const bool skip = use3 && use_black;
+        if (skip) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) park[i] = make_float4(rc[i].x, rc[i].y, rc[i].z, rw[i]);
+        }
+        if (fit) build_sat(rc, rw, n, sat4, (use3 && !skip) ? sat3 : nullptr);
+        if (skip) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const float4 v = park[i];
+                rc[i] = mk(v.x, v.y, v.z);
+                rw[i] = v.w;
+            }
+            m = drop_blacks(rc, rw, n);
+            if (m == 0) use3 = false;  // nothing but black: keep the 4-colour result
+            else build_sat(rc, rw, m, sat3, nullptr);
+        }
+        if (use3) n3 = m;
         __syncthreads();
-        const int win4 = search4(sh, ws, fit ? n : 0, metric, dw);
+        const int win4 = search<4>(sh, ws, tile_blocks, fit ? n : 0, metric, dw);
+        const int win3 = search<3>(sh, ws + kTile * kSatRows, tile_blocks, use3 ? m : 0, metric, dw);
 
-        Code o4{0u, 0u};
-        float e4 = 0.0f;
-        bool use3 = false;
-        int m = 0;
         if (fit) {
             f3 a = mk(0.0f, 0.0f, 0.0f), b = mk(0.0f, 0.0f, 0.0f);
             if (win4 >= 0) {
                 const uint32_t pk = sh.four[win4];
-                candidate4(my_sat[pk & 0xff], my_sat[(pk >> 8) & 0xff], my_sat[pk >> 16], my_sat[n], metric,
-                           dw, a, b);
+                candidate4(sat4[pk & 0xff], sat4[(pk >> 8) & 0xff], sat4[pk >> 16], sat4[n], metric, dw, a, b);
             }
-            e4 = emit4(tx, cw, dec, a, b, o4);
-            const bool use_black = any_black && job.tcb != 0;
-            use3 = p.fit3_always || (use_black && p.fit3_black_only);
+            Code oc;
+            float ec = emit4(tx, cw, dec, a, b, oc);
             if (use3) {
-                m = n;
-                if (use_black) {
-                    f3 c2[16];
-                    float w2[16];
-                    m = drop_blacks(rc, rw, n, c2, w2);
-                    if (m == 0) use3 = false;
-                    else build_sat(c2, w2, m, my_sat);
-                }
-            }
-            if (use3) n3 = m;
-        }
-        __syncthreads();
-        const int win3 = search3(sh, ws, use3 ? m : 0, metric);
-
-        if (fit) {
-            Code oc = o4;
-            float ec = e4;
-            if (use3) {
-                f3 a = mk(0.0f, 0.0f, 0.0f), b = mk(0.0f, 0.0f, 0.0f);
+                a = mk(0.0f, 0.0f, 0.0f);
+                b = mk(0.0f, 0.0f, 0.0f);
                 if (win3 >= 0) {
                     const uint32_t pk = sh.three[win3];
-                    candidate3(my_sat[pk & 0xff], my_sat[(pk >> 8) & 0xff], my_sat[m], metric, a, b);
+                    candidate3(sat3[pk & 0xff], sat3[(pk >> 8) & 0xff], sat3[m], metric, a, b);
                 }
                 Code o3;
                 const float e3 = emit3(tx, cw, dec, job.tcb != 0, a, b, o3);
@@ -307,7 +316,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) v
                                                            float4* __restrict__ workspace) {
     __shared__ Shared sh;
     load_tables(sh);
-    float4* ws = workspace + (size_t)blockIdx.x * kTile * kSatRows;
+    float4* ws = workspace + (size_t)blockIdx.x * (kWorkspaceBytesPerWorkgroup / sizeof(float4));
     uint32_t t = blockIdx.x;
     for (;;) {
         if (counter) {
@@ -317,7 +326,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         }
         if (t >= ntiles) break;  // every wave reaches this once the tiles run out
         uint32_t job = 0;
-        uint64_t first = (uint64_t)t * kTile;
+        uint64_t first = (uint64_t)t * (uint32_t)jobs[0].tile_blocks;
         if (tiles) {
             const uint2 tile = tiles[t];
             job = tile.x;
@@ -329,6 +338,18 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) v
 }
 
 __global__ void reset_counter_kernel(uint32_t* counter) { *counter = 0u; }
+
+// RGBA8 texels -> float RGBA, each channel c / 255.0f as the reference's
+// harness converts 8-bit images (icbc_test.cpp:401-405); alpha is carried
+// along although the encoder never reads it.
+__global__ void rgba8_to_float_kernel(const uint32_t* __restrict__ src, float4* __restrict__ dst,
+                                      uint64_t texels) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= texels) return;
+    const uint32_t v = src[i];
+    dst[i] = make_float4(k_u8_to_unit[v & 0xff], k_u8_to_unit[(v >> 8) & 0xff], k_u8_to_unit[(v >> 16) & 0xff],
+                         k_u8_to_unit[v >> 24]);
+}
 
 // decode_bc1 (icbc.h:2915-2927): one thread per block, RGBA8 out.
 __global__ void bc1_decode_kernel(const uint2* __restrict__ codes, uint64_t n,
@@ -410,6 +431,15 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
     }
     hipLaunchKernelGGL(bc1_encode_kernel, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles, counter,
                        workspace);
+    return hipGetLastError();
+}
+
+hipError_t launch_rgba8_to_float(const uint32_t* src, float4* dst, uint64_t blocks, hipStream_t stream) {
+    if (blocks == 0) return hipSuccess;
+    const uint64_t texels = blocks * 16;
+    const uint32_t threads = 256;
+    hipLaunchKernelGGL(rgba8_to_float_kernel, dim3((uint32_t)((texels + threads - 1) / threads)), dim3(threads), 0,
+                       stream, src, dst, texels);
     return hipGetLastError();
 }
 

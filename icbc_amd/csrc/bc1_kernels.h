@@ -19,7 +19,7 @@ struct JobDev {
     uint64_t block_count;
     int32_t level, tcm, tcb, decoder;
     float cw0, cw1, cw2;
-    int32_t pad;
+    int32_t tile_blocks;  // blocks per tile (8, 16, 32 or 64)
 };
 
 // Tables built by the host at init (all four decoders at once).
@@ -42,14 +42,20 @@ hipError_t upload_tables(const HostTables& t, hipStream_t stream);
 int encode_workgroups_per_cu();
 
 // Encodes `ntiles` tiles with `grid` resident workgroups (clamped to ntiles).
-//   tiles != null: tile k covers blocks [tiles[k].y, +64) of job tiles[k].x
-//   tiles == null: one job (jobs[0]), tile k covers blocks [64 k, 64 k + 64)
+//   tiles != null: tile k covers blocks [tiles[k].y, +T) of job tiles[k].x
+//   tiles == null: one job (jobs[0]), tile k covers blocks [T k, T k + T)
+// where T is the job's tile_blocks.
 //   counter != null: tiles are handed out in order from a queue (reset by this
 //   call); otherwise workgroup g takes tiles g, g + grid, ...
 // `workspace` holds kWorkspaceBytesPerWorkgroup bytes per workgroup of `grid`.
-constexpr size_t kWorkspaceBytesPerWorkgroup = (size_t)kTile * kSatRows * 16;
+// Per workgroup: two SAT slices (4- and 3-cluster fits) of 64 blocks x 17
+// rows, and the tile's reduced colours (64 x 16 float4).
+constexpr size_t kWorkspaceBytesPerWorkgroup = ((size_t)2 * kTile * kSatRows + (size_t)kTile * 16) * 16;
 hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, uint32_t* counter,
                          float4* workspace, uint32_t grid, hipStream_t stream);
+
+// 16 x RGBA8 per block -> 16 x float4 per block (c / 255.0f).
+hipError_t launch_rgba8_to_float(const uint32_t* src, float4* dst, uint64_t blocks, hipStream_t stream);
 
 hipError_t launch_decode(const uint2* codes, uint64_t n, uint32_t* rgba, int decoder,
                          hipStream_t stream);

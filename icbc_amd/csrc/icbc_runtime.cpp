@@ -25,6 +25,7 @@
 #include "../../include/icbc.h"
 #include "../../include/icbc_gpu.h"
 #include "bc1_kernels.h"
+#include "icbc_internal.h"
 
 using namespace icbc_gpu;
 
@@ -35,13 +36,18 @@ namespace {
 
 thread_local std::string t_last_error;
 
-int fail(int code, const char* fmt, ...) {
+int vfail(int code, const char* fmt, va_list ap) {
     char buf[512];
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    t_last_error = buf;
+    return code;
+}
+
+int fail(int code, const char* fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
+    vfail(code, fmt, ap);
     va_end(ap);
-    t_last_error = buf;
     return code;
 }
 
@@ -213,8 +219,11 @@ int workspace_alloc(uint32_t grid, hipStream_t s, Workspace& w) {
 // jobs
 
 int validate(const icbc_bc1_job& j, size_t k) {
-    if (j.reserved != 0 || (j.flags & ~ICBC_JOB_HOST_MEMORY) != 0)
-        return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: unknown flags or nonzero reserved field", k);
+    if ((j.flags & ~(ICBC_JOB_HOST_MEMORY | ICBC_JOB_COLORS_RGBA8)) != 0)
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: unknown flags", k);
+    if (j.tile_blocks != 0 && j.tile_blocks != 8 && j.tile_blocks != 16 && j.tile_blocks != 32 &&
+        j.tile_blocks != 64)
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: tile_blocks must be 0, 8, 16, 32 or 64", k);
     if (j.block_count == 0) return ICBC_OK;
     if (!j.colors || !j.codes) return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: null colors or codes", k);
     if (!(j.flags & ICBC_JOB_HOST_MEMORY)) {
@@ -223,9 +232,27 @@ int validate(const icbc_bc1_job& j, size_t k) {
         if (((uintptr_t)j.codes & 7) || ((uintptr_t)j.errors & 3) || ((uintptr_t)j.stats & 15))
             return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: misaligned codes/errors/stats", k);
     }
-    if (j.block_count > (uint64_t)0xffffffffu * kTile)
-        return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: too many blocks", k);
+    if (j.block_count > (uint64_t)0xffffffffu)
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: more than 2^32-1 blocks", k);
     return ICBC_OK;
+}
+
+// Blocks per tile when the caller leaves it to the library.  With enough work
+// to give every resident workgroup 64-block tiles, use 64 (shrinking only for
+// jobs smaller than a tile); otherwise split the blocks over about one tile per
+// resident workgroup, since a tile's blocks are searched one after another by
+// a single wavefront and small tiles finish sooner.
+uint32_t auto_tile_blocks_impl(uint64_t job_blocks, uint64_t total_blocks, uint32_t resident) {
+    uint32_t t;
+    if (total_blocks >= 64ull * resident) {
+        t = 64;
+        while (t > 8 && t / 2 >= job_blocks) t /= 2;
+    } else {
+        const uint64_t want = (total_blocks + resident - 1) / std::max<uint32_t>(resident, 1);
+        t = 8;
+        while (t < 64 && t < want) t *= 2;
+    }
+    return t;
 }
 
 JobDev to_dev(const icbc_bc1_job& j, int decoder) {
@@ -243,11 +270,13 @@ JobDev to_dev(const icbc_bc1_job& j, int decoder) {
     d.cw0 = j.color_weights[0];
     d.cw1 = j.color_weights[1];
     d.cw2 = j.color_weights[2];
-    d.pad = 0;
+    d.tile_blocks = j.tile_blocks ? (int32_t)j.tile_blocks : kTile;
     return d;
 }
 
-uint32_t tiles_of(uint64_t blocks) { return (uint32_t)((blocks + kTile - 1) / kTile); }
+uint32_t tiles_of(uint64_t blocks, uint32_t tile_blocks) {
+    return (uint32_t)((blocks + tile_blocks - 1) / tile_blocks);
+}
 
 // Host blob kept alive until the stream has consumed it.
 void free_host_blob(void* p) { free(p); }
@@ -282,8 +311,9 @@ int stage_in(const icbc_bc1_job& j, hipStream_t s, Staged& st, icbc_bc1_job& dj)
     dj.flags &= ~ICBC_JOB_HOST_MEMORY;
     const uint64_t n = j.block_count;
     if (n == 0) return ICBC_OK;
-    HIP_TRY(hipMallocAsync(&st.colors, n * 256, s));
-    HIP_TRY(hipMemcpyAsync(st.colors, j.colors, n * 256, hipMemcpyHostToDevice, s));
+    const size_t cbytes = n * ((j.flags & ICBC_JOB_COLORS_RGBA8) ? 64 : 256);
+    HIP_TRY(hipMallocAsync(&st.colors, cbytes, s));
+    HIP_TRY(hipMemcpyAsync(st.colors, j.colors, cbytes, hipMemcpyHostToDevice, s));
     dj.colors = (const float*)st.colors;
     if (j.weights) {
         HIP_TRY(hipMallocAsync(&st.weights, n * 64, s));
@@ -322,8 +352,30 @@ struct Prepared {
     std::vector<icbc_bc1_job> jobs;  // device-pointer view
     std::vector<Staged> staged;      // per job (empty for device jobs)
     std::vector<JobDev> dev;
+    std::vector<void*> temps;        // float copies of RGBA8 inputs, freed after the launch
     bool any_host = false;
 };
+
+// RGBA8 jobs: convert the texels to float RGBA in stream-ordered scratch and
+// point the device job at it.
+int convert_rgba8(Prepared& p, hipStream_t s) {
+    for (size_t k = 0; k < p.jobs.size(); k++) {
+        const icbc_bc1_job& j = p.jobs[k];
+        if (!(j.flags & ICBC_JOB_COLORS_RGBA8) || j.block_count == 0) continue;
+        void* f = nullptr;
+        HIP_TRY(hipMallocAsync(&f, j.block_count * 256, s));
+        p.temps.push_back(f);
+        HIP_TRY(launch_rgba8_to_float((const uint32_t*)j.colors, (float4*)f, j.block_count, s));
+        p.dev[k].colors = (const float4*)f;
+    }
+    return ICBC_OK;
+}
+
+int free_temps(Prepared& p, hipStream_t s) {
+    for (void* t : p.temps) HIP_TRY(hipFreeAsync(t, s));
+    p.temps.clear();
+    return ICBC_OK;
+}
 
 int prepare(const icbc_bc1_job* jobs, size_t n_jobs, hipStream_t s, int decoder, Prepared& p) {
     if (n_jobs && !jobs) return fail(ICBC_ERR_INVALID_ARGUMENT, "null job array");
@@ -344,10 +396,12 @@ int prepare(const icbc_bc1_job* jobs, size_t n_jobs, hipStream_t s, int decoder,
         }
         p.dev[k] = to_dev(p.jobs[k], decoder);
     }
-    return ICBC_OK;
+    return convert_rgba8(p, s);
 }
 
 int finish(const icbc_bc1_job* jobs, size_t n_jobs, hipStream_t s, Prepared& p) {
+    int r = free_temps(p, s);
+    if (r) return r;
     if (!p.any_host) return ICBC_OK;
     for (size_t k = 0; k < n_jobs; k++)
         if (jobs[k].flags & ICBC_JOB_HOST_MEMORY) {
@@ -358,18 +412,78 @@ int finish(const icbc_bc1_job* jobs, size_t n_jobs, hipStream_t s, Prepared& p) 
     return ICBC_OK;
 }
 
+// Fills in automatic tile sizes for a set of jobs launched together.
+int choose_tiles(std::vector<JobDev>& dev, const icbc_bc1_job* jobs, bool per_job_launch) {
+    uint32_t resident;
+    int r = resident_workgroups(&resident);
+    if (r) return r;
+    uint64_t total = 0;
+    for (const JobDev& d : dev) total += d.block_count;
+    for (size_t k = 0; k < dev.size(); k++)
+        if (jobs[k].tile_blocks == 0)
+            dev[k].tile_blocks = (int32_t)auto_tile_blocks_impl(dev[k].block_count,
+                                                           per_job_launch ? dev[k].block_count : total, resident);
+    return ICBC_OK;
+}
+
+// Relative cost of one block at each level, used to issue expensive tiles
+// first (larger candidate searches and refinement at the top levels).
+float level_cost(int level) {
+    static const float cost[10] = {1, 2, 40, 40, 60, 60, 60, 60, 70, 110};
+    return (level >= 0 && level <= 9) ? cost[level] : 0.0f;
+}
+
+void order_tiles(std::vector<uint2>& tiles, const std::vector<JobDev>& dev) {
+    std::stable_sort(tiles.begin(), tiles.end(), [&](const uint2& a, const uint2& b) {
+        return level_cost(dev[a.x].level) * dev[a.x].tile_blocks > level_cost(dev[b.x].level) * dev[b.x].tile_blocks;
+    });
+}
+
 std::vector<uint2> make_tiles(const std::vector<JobDev>& dev) {
     std::vector<uint2> tiles;
     size_t total = 0;
-    for (const JobDev& d : dev) total += tiles_of(d.block_count);
+    for (const JobDev& d : dev) total += tiles_of(d.block_count, d.tile_blocks);
     tiles.reserve(total);
     for (size_t k = 0; k < dev.size(); k++)
-        for (uint32_t t = 0; t < tiles_of(dev[k].block_count); t++)
-            tiles.push_back(make_uint2((uint32_t)k, t * (uint32_t)kTile));
+        for (uint32_t t = 0; t < tiles_of(dev[k].block_count, dev[k].tile_blocks); t++)
+            tiles.push_back(make_uint2((uint32_t)k, t * (uint32_t)dev[k].tile_blocks));
     return tiles;
 }
 
 }  // namespace
+
+// ===========================================================================
+// internal interface (icbc_internal.h)
+
+int icbc_internal_fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vfail(code, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int icbc_internal_ready(int* decoder) { return ensure_device_ready(decoder); }
+
+int icbc_internal_resident(uint32_t* grid) { return resident_workgroups(grid); }
+
+int icbc_internal_validate(const icbc_bc1_job& job) { return validate(job, 0); }
+
+namespace icbc_gpu {
+
+JobDev job_to_device(const icbc_bc1_job& j, int decoder) { return to_dev(j, decoder); }
+
+uint32_t auto_tile_blocks(uint64_t job_blocks, uint64_t total_blocks, uint32_t resident) {
+    return auto_tile_blocks_impl(job_blocks, total_blocks, resident);
+}
+
+std::vector<uint2> tiles_for(const std::vector<JobDev>& dev, bool order_by_cost) {
+    std::vector<uint2> t = make_tiles(dev);
+    if (order_by_cost) order_tiles(t, dev);
+    return t;
+}
+
+}  // namespace icbc_gpu
 
 // ===========================================================================
 // C ABI
@@ -401,8 +515,11 @@ int icbc_gpu_compress_bc1_batch(const icbc_bc1_job* jobs, size_t n_jobs, void* s
     Prepared p;
     r = prepare(jobs, n_jobs, s, decoder, p);
     if (r) return r;
+    r = choose_tiles(p.dev, jobs, false);
+    if (r) return r;
     std::vector<uint2> tiles = make_tiles(p.dev);
     if (tiles.size() > 0xffffffffu) return fail(ICBC_ERR_INVALID_ARGUMENT, "batch too large");
+    order_tiles(tiles, p.dev);
     if (!tiles.empty()) {
         uint32_t grid;
         r = resident_workgroups(&grid);
@@ -437,8 +554,10 @@ int icbc_gpu_compress_bc1_plain(const icbc_bc1_job* jobs, size_t n_jobs, void* s
         uint32_t resident;
         r = resident_workgroups(&resident);
         if (r) return r;
+        r = choose_tiles(p.dev, jobs, true);
+        if (r) return r;
         uint32_t grid = 1;
-        for (const JobDev& d : p.dev) grid = std::max(grid, std::min(resident, tiles_of(d.block_count)));
+        for (const JobDev& d : p.dev) grid = std::max(grid, std::min(resident, tiles_of(d.block_count, d.tile_blocks)));
         void* djobs = nullptr;
         r = upload_blob(p.dev.data(), p.dev.size() * sizeof(JobDev), s, &djobs);
         if (r) return r;
@@ -446,8 +565,8 @@ int icbc_gpu_compress_bc1_plain(const icbc_bc1_job* jobs, size_t n_jobs, void* s
         r = workspace_alloc(grid, s, w);
         if (r) return r;
         for (size_t k = 0; k < n_jobs; k++)
-            HIP_TRY(launch_encode((const JobDev*)djobs + k, nullptr, tiles_of(p.dev[k].block_count), nullptr,
-                                  w.slices(), grid, s));
+            HIP_TRY(launch_encode((const JobDev*)djobs + k, nullptr, tiles_of(p.dev[k].block_count, p.dev[k].tile_blocks),
+                                  nullptr, w.slices(), grid, s));
         HIP_TRY(hipFreeAsync(w.base, s));
         HIP_TRY(hipFreeAsync(djobs, s));
     }
@@ -464,6 +583,12 @@ struct icbc_bc1_plan {
     JobDev* djobs = nullptr;
     uint2* dtiles = nullptr;
     void* dwork = nullptr;  // tile-queue counter (256 B) + per-workgroup slices
+    struct Convert {
+        const uint32_t* src;
+        float4* dst;
+        uint64_t blocks;
+    };
+    std::vector<Convert> converts;  // RGBA8 jobs, re-read on every execute
     uint32_t ntiles = 0;
     uint32_t grid = 0;
     hipGraph_t graph = nullptr;
@@ -472,14 +597,8 @@ struct icbc_bc1_plan {
 
 namespace {
 
-// Relative cost of one tile at each level, used to issue expensive tiles
-// first (larger candidate searches and refinement at the top levels).
-float level_cost(int level) {
-    static const float cost[10] = {1, 2, 40, 40, 60, 60, 60, 60, 70, 110};
-    return (level >= 0 && level <= 9) ? cost[level] : 0.0f;
-}
-
 int plan_launch(icbc_bc1_plan* pl, hipStream_t s) {
+    for (const auto& c : pl->converts) HIP_TRY(launch_rgba8_to_float(c.src, c.dst, c.blocks, s));
     uint32_t* counter = pl->opt.schedule == ICBC_SCHEDULE_PERSISTENT ? (uint32_t*)pl->dwork : nullptr;
     float4* slices = (float4*)((char*)pl->dwork + 256);
     HIP_TRY(launch_encode(pl->djobs, pl->dtiles, pl->ntiles, counter, slices, pl->grid, s));
@@ -506,15 +625,13 @@ int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_pla
             return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: plans take device memory only", k);
         dev[k] = to_dev(jobs[k], decoder);
     }
+    r = choose_tiles(dev, jobs, false);
+    if (r) return r;
     std::vector<uint2> tiles = make_tiles(dev);
     if (tiles.size() > 0xffffffffu) return fail(ICBC_ERR_INVALID_ARGUMENT, "plan too large");
     icbc_plan_options o{ICBC_SCHEDULE_PERSISTENT, 1, 0, 1};
     if (opt) o = *opt;
-    if (o.order_by_cost) {
-        std::stable_sort(tiles.begin(), tiles.end(), [&](const uint2& a, const uint2& b) {
-            return level_cost(dev[a.x].level) > level_cost(dev[b.x].level);
-        });
-    }
+    if (o.order_by_cost) order_tiles(tiles, dev);
     icbc_bc1_plan* pl = new icbc_bc1_plan();
     pl->opt = o;
     pl->ntiles = (uint32_t)tiles.size();
@@ -542,6 +659,13 @@ int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_pla
         (e = hipMalloc(&pl->dtiles, std::max<size_t>(1, tiles.size()) * sizeof(uint2))) != hipSuccess ||
         (e = hipMalloc(&pl->dwork, 256 + (size_t)pl->grid * kWorkspaceBytesPerWorkgroup)) != hipSuccess)
         return cleanup(hip_fail(e, "hipMalloc (plan)"));
+    for (size_t k = 0; k < n_jobs; k++) {
+        if (!(jobs[k].flags & ICBC_JOB_COLORS_RGBA8) || jobs[k].block_count == 0) continue;
+        void* f = nullptr;
+        if ((e = hipMalloc(&f, jobs[k].block_count * 256)) != hipSuccess) return cleanup(hip_fail(e, "hipMalloc (plan)"));
+        pl->converts.push_back({(const uint32_t*)jobs[k].colors, (float4*)f, jobs[k].block_count});
+        dev[k].colors = (const float4*)f;
+    }
     if (!dev.empty() &&
         (e = hipMemcpy(pl->djobs, dev.data(), dev.size() * sizeof(JobDev), hipMemcpyHostToDevice)) != hipSuccess)
         return cleanup(hip_fail(e, "hipMemcpy (plan jobs)"));
@@ -591,6 +715,7 @@ int icbc_gpu_plan_destroy(icbc_bc1_plan* pl) {
     if (pl->djobs) (void)hipFree(pl->djobs);
     if (pl->dtiles) (void)hipFree(pl->dtiles);
     if (pl->dwork) (void)hipFree(pl->dwork);
+    for (const auto& c : pl->converts) (void)hipFree(c.dst);
     delete pl;
     return ICBC_OK;
 }
@@ -741,6 +866,21 @@ void* icbc_gpu_malloc(size_t bytes) {
 
 int icbc_gpu_free(void* ptr) {
     HIP_TRY(hipFree(ptr));
+    return ICBC_OK;
+}
+
+void* icbc_gpu_host_alloc(size_t bytes) {
+    void* p = nullptr;
+    hipError_t e = hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        hip_fail(e, "hipHostMalloc");
+        return nullptr;
+    }
+    return p;
+}
+
+int icbc_gpu_host_free(void* ptr) {
+    HIP_TRY(hipHostFree(ptr));
     return ICBC_OK;
 }
 

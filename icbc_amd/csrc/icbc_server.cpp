@@ -1,0 +1,575 @@
+// Request server: continuous batching of BC1 encode requests on one device.
+//
+// A request is one texture (a host-memory icbc_bc1_job).  The server
+//   - admits requests in arrival order while their device footprint fits the
+//     memory budget (a request larger than the whole budget is rejected);
+//   - keeps request buffers resident: device memory comes from a pool of
+//     size-classed blocks that are reused, never returned to the driver
+//     while the server lives;
+//   - batches every admitted, not yet launched request into one encode launch
+//     (the same persistent kernel and tile queue as the batch API), with up to
+//     `max_inflight` launches overlapping on separate streams, so requests
+//     that arrive while others run join the next launch as soon as memory
+//     and a launch slot free up;
+//   - copies results back, timestamps completion and releases the budget.
+// Two host threads: a scheduler (admit, upload, launch) and a reaper (wait on
+// the oldest launch, complete its requests).  Results do not depend on the
+// batch a request lands in: blocks are independent and each job carries its
+// own options.
+
+#include <hip/hip_runtime.h>
+
+#include <float.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/icbc_gpu.h"
+#include "bc1_kernels.h"
+#include "icbc_internal.h"
+
+using namespace icbc_gpu;
+
+namespace {
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Device footprint of a request: colours, weights, codes, errors, stats.
+struct Layout {
+    size_t colors = 0, weights = 0, codes = 0, errors = 0, stats = 0, total = 0, color_bytes = 0;
+    size_t floats = 0;  // float texels converted from RGBA8 input (RGBA8 requests only)
+};
+
+Layout layout_of(const icbc_bc1_job& j) {
+    Layout l;
+    const uint64_t n = j.block_count;
+    size_t off = 0;
+    l.colors = off;
+    l.color_bytes = n * ((j.flags & ICBC_JOB_COLORS_RGBA8) ? 64 : 256);
+    off = align_up(off + l.color_bytes, 256);
+    l.floats = off;
+    if (j.flags & ICBC_JOB_COLORS_RGBA8) off = align_up(off + n * 256, 256);
+    l.weights = off;
+    if (j.weights) off = align_up(off + n * 64, 256);
+    l.codes = off;
+    off = align_up(off + n * 8, 256);
+    l.errors = off;
+    if (j.errors) off = align_up(off + n * 4, 256);
+    l.stats = off;
+    if (j.stats) off = align_up(off + n * 16, 256);
+    l.total = std::max<size_t>(off, 256);
+    return l;
+}
+
+// Size-classed device blocks (powers of two from 64 KiB), reused.  Blocks
+// are carved from large slabs so that steady-state serving never calls the
+// allocator.
+class Pool {
+public:
+    ~Pool() {
+        for (void* p : slabs_) (void)hipFree(p);
+    }
+    static size_t class_of(size_t bytes) {
+        size_t c = 64 << 10;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+    hipError_t take(size_t cls, void** p) {
+        auto& v = free_[cls];
+        if (!v.empty()) {
+            *p = v.back();
+            v.pop_back();
+            return hipSuccess;
+        }
+        const size_t slab = std::max<size_t>(cls, 64u << 20);
+        void* base = nullptr;
+        hipError_t e = hipMalloc(&base, slab);
+        if (e != hipSuccess) return e;
+        slabs_.push_back(base);
+        for (size_t off = cls; off + cls <= slab; off += cls) v.push_back((char*)base + off);
+        *p = base;
+        return hipSuccess;
+    }
+    void give(size_t cls, void* p) { free_[cls].push_back(p); }
+
+private:
+    std::map<size_t, std::vector<void*>> free_;
+    std::vector<void*> slabs_;
+};
+
+// Page-locked host staging that grows on demand.
+struct Staging {
+    void* p = nullptr;
+    size_t cap = 0;
+    ~Staging() {
+        if (p) (void)hipHostFree(p);
+    }
+    bool reserve(size_t bytes) {
+        if (bytes <= cap) return true;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(bytes + bytes / 2, 1 << 20);
+        if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) return false;
+        cap = want;
+        return true;
+    }
+};
+
+// True when `p` is page-locked host memory the DMA engines can read directly.
+bool is_pinned(const void* p) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
+enum State { kPending, kAdmitted, kLaunched, kDone };
+
+struct Request {
+    uint64_t id = 0;
+    icbc_bc1_job job{};  // host pointers as given by the caller
+    Layout lay;
+    size_t cls = 0;      // pool class (also the budget charge)
+    void* dev = nullptr;
+    State state = kPending;
+    int status = ICBC_OK;
+    uint32_t batch_size = 0;
+    double t_submit = 0, t_admit = 0, t_launch = 0, t_done = 0;
+};
+
+// One launch slot: a stream plus resident launch resources.
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    void* work = nullptr;  // counter + workspace slices
+    void* table = nullptr;  // job table + tile list
+    size_t table_bytes = 0;
+    void* host_table = nullptr;  // pinned staging for the table
+    Staging in;   // inputs of requests whose buffers are pageable
+    Staging out;  // codes / errors / stats on their way back
+    std::vector<size_t> out_off;  // per request: offset of its outputs in `out`
+    std::vector<Request*> reqs;
+    bool busy = false;
+};
+
+}  // namespace
+
+struct icbc_server {
+    icbc_server_options opt{};
+    int decoder = 0;
+    uint32_t grid = 0;
+    std::mutex m;
+    std::condition_variable cv_sched, cv_reap, cv_done;
+    std::deque<Request*> pending;
+    std::vector<Request*> admitted;
+    std::deque<Slot*> inflight;
+    std::vector<Slot> slots;
+    std::unordered_map<uint64_t, Request*> all;
+    Pool pool;
+    uint64_t used = 0;
+    uint64_t next_id = 1;
+    bool stop = false;
+    int fatal = ICBC_OK;
+    std::thread sched, reap;
+    // counters
+    uint64_t launches = 0, completed = 0, blocks_done = 0;
+
+    void run_scheduler();
+    void run_reaper();
+    int launch(Slot& s, std::vector<Request*>& reqs);
+};
+
+namespace {
+
+int set_fatal(icbc_server* sv, int code) {
+    if (sv->fatal == ICBC_OK) sv->fatal = code;
+    return code;
+}
+
+}  // namespace
+
+int icbc_server::launch(Slot& s, std::vector<Request*>& reqs) {
+    // device job table for the batch
+    std::vector<JobDev> dev;
+    std::vector<uint64_t> blocks;
+    uint64_t total = 0;
+    for (Request* r : reqs) total += r->job.block_count;
+    for (Request* r : reqs) {
+        icbc_bc1_job dj = r->job;
+        char* base = (char*)r->dev;
+        dj.flags = 0;
+        if (r->job.flags & ICBC_JOB_COLORS_RGBA8) {
+            if (r->job.block_count &&
+                launch_rgba8_to_float((const uint32_t*)(base + r->lay.colors), (float4*)(base + r->lay.floats),
+                                      r->job.block_count, s.stream) != hipSuccess)
+                return ICBC_ERR_DEVICE;
+            dj.colors = (const float*)(base + r->lay.floats);
+        } else {
+            dj.colors = (const float*)(base + r->lay.colors);
+        }
+        dj.weights = r->job.weights ? (const float*)(base + r->lay.weights) : nullptr;
+        dj.codes = base + r->lay.codes;
+        dj.errors = r->job.errors ? (float*)(base + r->lay.errors) : nullptr;
+        dj.stats = r->job.stats ? (int32_t*)(base + r->lay.stats) : nullptr;
+        JobDev d = job_to_device(dj, decoder);
+        if (dj.tile_blocks == 0) d.tile_blocks = (int32_t)auto_tile_blocks(dj.block_count, total, grid);
+        dev.push_back(d);
+    }
+    std::vector<uint2> tiles = tiles_for(dev, true);
+    const size_t jbytes = align_up(dev.size() * sizeof(JobDev), 256);
+    const size_t need = jbytes + std::max<size_t>(1, tiles.size()) * sizeof(uint2);
+    if (need > s.table_bytes) {
+        if (s.table) (void)hipFree(s.table);
+        if (s.host_table) (void)hipHostFree(s.host_table);
+        s.table = nullptr;
+        s.host_table = nullptr;
+        s.table_bytes = 0;
+        const size_t cap = std::max<size_t>(need * 2, 64 << 10);
+        if (hipMalloc(&s.table, cap) != hipSuccess || hipHostMalloc(&s.host_table, cap, hipHostMallocDefault) != hipSuccess)
+            return ICBC_ERR_OUT_OF_MEMORY;
+        s.table_bytes = cap;
+    }
+    memcpy(s.host_table, dev.data(), dev.size() * sizeof(JobDev));
+    memcpy((char*)s.host_table + jbytes, tiles.data(), tiles.size() * sizeof(uint2));
+    if (hipMemcpyAsync(s.table, s.host_table, need, hipMemcpyHostToDevice, s.stream) != hipSuccess)
+        return ICBC_ERR_DEVICE;
+    if (launch_encode((const JobDev*)s.table, (const uint2*)((char*)s.table + jbytes), (uint32_t)tiles.size(),
+                      (uint32_t*)s.work, (float4*)((char*)s.work + 256), grid, s.stream) != hipSuccess)
+        return ICBC_ERR_DEVICE;
+    // results go to pinned staging (a copy into pageable memory would block
+    // this thread until the kernel ends); the reaper hands them over
+    size_t out_bytes = 0;
+    s.out_off.assign(reqs.size(), 0);
+    for (size_t k = 0; k < reqs.size(); k++) {
+        s.out_off[k] = out_bytes;
+        const uint64_t n = reqs[k]->job.block_count;
+        out_bytes += align_up(n * 8, 256) + (reqs[k]->job.errors ? align_up(n * 4, 256) : 0) +
+                     (reqs[k]->job.stats ? align_up(n * 16, 256) : 0);
+    }
+    if (!s.out.reserve(std::max<size_t>(out_bytes, 256))) return ICBC_ERR_OUT_OF_MEMORY;
+    for (size_t k = 0; k < reqs.size(); k++) {
+        Request* r = reqs[k];
+        const uint64_t n = r->job.block_count;
+        char* base = (char*)r->dev;
+        char* dst = (char*)s.out.p + s.out_off[k];
+        if (n == 0) continue;
+        if (hipMemcpyAsync(dst, base + r->lay.codes, n * 8, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
+            return ICBC_ERR_DEVICE;
+        dst += align_up(n * 8, 256);
+        if (r->job.errors) {
+            if (hipMemcpyAsync(dst, base + r->lay.errors, n * 4, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
+                return ICBC_ERR_DEVICE;
+            dst += align_up(n * 4, 256);
+        }
+        if (r->job.stats &&
+            hipMemcpyAsync(dst, base + r->lay.stats, n * 16, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
+            return ICBC_ERR_DEVICE;
+    }
+    if (hipEventRecord(s.done, s.stream) != hipSuccess) return ICBC_ERR_DEVICE;
+    return ICBC_OK;
+}
+
+void icbc_server::run_scheduler() {
+    std::unique_lock<std::mutex> lk(m);
+    for (;;) {
+        // admit in arrival order while the budget allows
+        while (!pending.empty()) {
+            Request* r = pending.front();
+            if (used + r->cls > opt.memory_budget) break;
+            // the pool is only touched under the lock (the reaper returns blocks)
+            if (pool.take(r->cls, &r->dev) != hipSuccess) {
+                if (!admitted.empty() || !inflight.empty()) break;  // retry once memory frees up
+                pending.pop_front();
+                r->status = ICBC_ERR_OUT_OF_MEMORY;
+                r->state = kDone;
+                r->t_done = now_s();
+                cv_done.notify_all();
+                continue;
+            }
+            pending.pop_front();
+            used += r->cls;
+            r->state = kAdmitted;
+            r->t_admit = now_s();
+            admitted.push_back(r);
+        }
+        Slot* free_slot = nullptr;
+        for (Slot& s : slots)
+            if (!s.busy) {
+                free_slot = &s;
+                break;
+            }
+        if (!admitted.empty() && free_slot && fatal == ICBC_OK) {
+            std::vector<Request*> batch;
+            const size_t take = opt.max_batch ? std::min<size_t>(opt.max_batch, admitted.size()) : admitted.size();
+            batch.assign(admitted.begin(), admitted.begin() + take);
+            admitted.erase(admitted.begin(), admitted.begin() + take);
+            free_slot->busy = true;
+            free_slot->reqs = batch;
+            lk.unlock();
+            int r = ICBC_OK;
+            // upload inputs (and prior codes: blocks the encoder leaves
+            // untouched keep them); pageable buffers go through pinned staging
+            size_t in_bytes = 0;
+            std::vector<char> pinned(batch.size() * 3, 0);
+            for (size_t k = 0; k < batch.size(); k++) {
+                const icbc_bc1_job& j = batch[k]->job;
+                const uint64_t n = j.block_count;
+                if (n == 0) continue;
+                pinned[3 * k + 0] = is_pinned(j.colors);
+                pinned[3 * k + 1] = j.weights ? is_pinned(j.weights) : 1;
+                pinned[3 * k + 2] = is_pinned(j.codes);
+                if (!pinned[3 * k + 0]) in_bytes += align_up(batch[k]->lay.color_bytes, 256);
+                if (!pinned[3 * k + 1]) in_bytes += align_up(n * 64, 256);
+                if (!pinned[3 * k + 2]) in_bytes += align_up(n * 8, 256);
+            }
+            if (!free_slot->in.reserve(std::max<size_t>(in_bytes, 256))) r = ICBC_ERR_OUT_OF_MEMORY;
+            size_t in_off = 0;
+            auto up = [&](void* dev_dst, const void* src, size_t bytes, bool src_pinned) {
+                const void* from = src;
+                if (!src_pinned) {
+                    char* st = (char*)free_slot->in.p + in_off;
+                    memcpy(st, src, bytes);
+                    in_off += align_up(bytes, 256);
+                    from = st;
+                }
+                return hipMemcpyAsync(dev_dst, from, bytes, hipMemcpyHostToDevice, free_slot->stream) == hipSuccess;
+            };
+            for (size_t k = 0; k < batch.size() && r == ICBC_OK; k++) {
+                Request* q = batch[k];
+                const uint64_t n = q->job.block_count;
+                char* base = (char*)q->dev;
+                if (n == 0) continue;
+                if (!up(base + q->lay.colors, q->job.colors, q->lay.color_bytes, pinned[3 * k + 0]) ||
+                    (q->job.weights && !up(base + q->lay.weights, q->job.weights, n * 64, pinned[3 * k + 1])) ||
+                    !up(base + q->lay.codes, q->job.codes, n * 8, pinned[3 * k + 2]))
+                    r = ICBC_ERR_DEVICE;
+            }
+            const double t = now_s();
+            if (r == ICBC_OK) r = launch(*free_slot, batch);
+            lk.lock();
+            for (Request* q : batch) {
+                q->state = kLaunched;
+                q->t_launch = t;
+                q->batch_size = (uint32_t)batch.size();
+            }
+            if (r != ICBC_OK) {
+                set_fatal(this, r);
+                for (Request* q : batch) q->status = r;
+            }
+            launches++;
+            inflight.push_back(free_slot);
+            cv_reap.notify_one();
+            continue;
+        }
+        if (stop && pending.empty() && admitted.empty()) break;
+        cv_sched.wait(lk);
+    }
+}
+
+void icbc_server::run_reaper() {
+    std::unique_lock<std::mutex> lk(m);
+    for (;;) {
+        while (inflight.empty()) {
+            if (stop && pending.empty() && admitted.empty()) return;
+            cv_reap.wait(lk);
+        }
+        Slot* s = inflight.front();
+        lk.unlock();
+        const hipError_t e = hipEventSynchronize(s->done);
+        if (e == hipSuccess) {
+            for (size_t k = 0; k < s->reqs.size(); k++) {
+                Request* r = s->reqs[k];
+                const uint64_t n = r->job.block_count;
+                if (n == 0 || r->status != ICBC_OK) continue;
+                const char* src = (const char*)s->out.p + s->out_off[k];
+                memcpy(r->job.codes, src, n * 8);
+                src += align_up(n * 8, 256);
+                if (r->job.errors) {
+                    memcpy(r->job.errors, src, n * 4);
+                    src += align_up(n * 4, 256);
+                }
+                if (r->job.stats) memcpy(r->job.stats, src, n * 16);
+            }
+        }
+        const double t = now_s();
+        lk.lock();
+        inflight.pop_front();
+        for (Request* r : s->reqs) {
+            if (e != hipSuccess && r->status == ICBC_OK) r->status = ICBC_ERR_DEVICE;
+            r->state = kDone;
+            r->t_done = t;
+            pool.give(r->cls, r->dev);
+            r->dev = nullptr;
+            used -= r->cls;
+            completed++;
+            blocks_done += r->job.block_count;
+        }
+        if (e != hipSuccess) set_fatal(this, ICBC_ERR_DEVICE);
+        s->reqs.clear();
+        s->busy = false;
+        cv_done.notify_all();
+        cv_sched.notify_one();
+    }
+}
+
+extern "C" {
+
+int icbc_server_create(const icbc_server_options* opt, icbc_server** out) {
+    if (!out || !opt) return icbc_internal_fail(ICBC_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (opt->memory_budget < (64u << 10))
+        return icbc_internal_fail(ICBC_ERR_INVALID_ARGUMENT, "memory budget below 64 KiB");
+    int decoder;
+    int r = icbc_internal_ready(&decoder);
+    if (r) return r;
+    uint32_t grid;
+    r = icbc_internal_resident(&grid);
+    if (r) return r;
+    icbc_server* sv = new icbc_server();
+    sv->opt = *opt;
+    if (sv->opt.max_inflight == 0) sv->opt.max_inflight = 2;
+    if (sv->opt.max_inflight > 8) sv->opt.max_inflight = 8;
+    sv->decoder = decoder;
+    sv->grid = grid;
+    sv->slots.resize(sv->opt.max_inflight);
+    for (Slot& s : sv->slots) {
+        if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
+            hipMalloc(&s.work, 256 + (size_t)grid * kWorkspaceBytesPerWorkgroup) != hipSuccess) {
+            icbc_server_destroy(sv);
+            return icbc_internal_fail(ICBC_ERR_OUT_OF_MEMORY, "server launch slot allocation failed");
+        }
+    }
+    sv->sched = std::thread([sv] { sv->run_scheduler(); });
+    sv->reap = std::thread([sv] { sv->run_reaper(); });
+    *out = sv;
+    return ICBC_OK;
+}
+
+int icbc_server_submit(icbc_server* sv, const icbc_bc1_job* job, uint64_t* ticket) {
+    if (!sv || !job || !ticket) return icbc_internal_fail(ICBC_ERR_INVALID_ARGUMENT, "null argument");
+    if (!(job->flags & ICBC_JOB_HOST_MEMORY))
+        return icbc_internal_fail(ICBC_ERR_INVALID_ARGUMENT, "server requests take host memory (ICBC_JOB_HOST_MEMORY)");
+    int r = icbc_internal_validate(*job);
+    if (r) return r;
+    Request* q = new Request();
+    q->job = *job;
+    q->lay = layout_of(*job);
+    q->cls = Pool::class_of(q->lay.total);
+    q->t_submit = now_s();
+    std::lock_guard<std::mutex> lk(sv->m);
+    if (sv->fatal) {
+        delete q;
+        return icbc_internal_fail(sv->fatal, "server stopped after a device failure");
+    }
+    if (sv->stop) {
+        delete q;
+        return icbc_internal_fail(ICBC_ERR_CANCELLED, "server is shutting down");
+    }
+    if (q->cls > sv->opt.memory_budget) {
+        delete q;
+        return icbc_internal_fail(ICBC_ERR_BUDGET, "request needs %zu bytes, budget is %llu", q->cls,
+                                  (unsigned long long)sv->opt.memory_budget);
+    }
+    q->id = sv->next_id++;
+    sv->all[q->id] = q;
+    sv->pending.push_back(q);
+    *ticket = q->id;
+    sv->cv_sched.notify_one();
+    return ICBC_OK;
+}
+
+int icbc_server_wait(icbc_server* sv, uint64_t ticket, double timeout_s, icbc_request_info* info) {
+    if (!sv) return icbc_internal_fail(ICBC_ERR_INVALID_ARGUMENT, "null server");
+    std::unique_lock<std::mutex> lk(sv->m);
+    auto it = sv->all.find(ticket);
+    if (it == sv->all.end()) return icbc_internal_fail(ICBC_ERR_INVALID_ARGUMENT, "unknown ticket %llu",
+                                                        (unsigned long long)ticket);
+    Request* q = it->second;
+    const auto pred = [&] { return q->state == kDone; };
+    if (timeout_s < 0) sv->cv_done.wait(lk, pred);
+    else if (!sv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred))
+        return ICBC_ERR_TIMEOUT;
+    if (info) {
+        info->status = q->status;
+        info->batch_size = q->batch_size;
+        info->block_count = q->job.block_count;
+        info->submit_s = q->t_submit;
+        info->admit_s = q->t_admit;
+        info->launch_s = q->t_launch;
+        info->done_s = q->t_done;
+    }
+    const int status = q->status;
+    sv->all.erase(it);
+    delete q;
+    return status;
+}
+
+int icbc_server_cancel(icbc_server* sv, uint64_t ticket) {
+    if (!sv) return icbc_internal_fail(ICBC_ERR_INVALID_ARGUMENT, "null server");
+    std::lock_guard<std::mutex> lk(sv->m);
+    auto it = sv->all.find(ticket);
+    if (it == sv->all.end()) return icbc_internal_fail(ICBC_ERR_INVALID_ARGUMENT, "unknown ticket");
+    Request* q = it->second;
+    if (q->state != kPending) return icbc_internal_fail(ICBC_ERR_INVALID_ARGUMENT, "request already admitted");
+    sv->pending.erase(std::find(sv->pending.begin(), sv->pending.end(), q));
+    q->state = kDone;
+    q->status = ICBC_ERR_CANCELLED;
+    q->t_done = now_s();
+    sv->cv_done.notify_all();
+    return ICBC_OK;
+}
+
+int icbc_server_stats(icbc_server* sv, icbc_server_stats_t* st) {
+    if (!sv || !st) return icbc_internal_fail(ICBC_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lk(sv->m);
+    st->launches = sv->launches;
+    st->completed = sv->completed;
+    st->blocks = sv->blocks_done;
+    st->pending = sv->pending.size();
+    st->inflight = sv->inflight.size();
+    st->memory_in_use = sv->used;
+    return ICBC_OK;
+}
+
+int icbc_server_destroy(icbc_server* sv) {
+    if (!sv) return ICBC_OK;
+    {
+        std::lock_guard<std::mutex> lk(sv->m);
+        sv->stop = true;
+        sv->cv_sched.notify_all();
+        sv->cv_reap.notify_all();
+    }
+    if (sv->sched.joinable()) sv->sched.join();
+    if (sv->reap.joinable()) sv->reap.join();
+    for (Slot& s : sv->slots) {
+        if (s.stream) (void)hipStreamSynchronize(s.stream);
+        if (s.done) (void)hipEventDestroy(s.done);
+        if (s.stream) (void)hipStreamDestroy(s.stream);
+        if (s.work) (void)hipFree(s.work);
+        if (s.table) (void)hipFree(s.table);
+        if (s.host_table) (void)hipHostFree(s.host_table);
+    }
+    for (auto& kv : sv->all) delete kv.second;
+    delete sv;
+    return ICBC_OK;
+}
+
+}  // extern "C"

@@ -27,6 +27,7 @@ extern "C" {
 #define ICBC_ERR_NO_DEVICE (-5)  /* no usable gfx950 device */
 #define ICBC_ERR_BUDGET (-6)     /* serving: request larger than the memory budget */
 #define ICBC_ERR_CANCELLED (-7)  /* serving: request cancelled before it ran */
+#define ICBC_ERR_TIMEOUT (-8)    /* serving: wait timed out, request still running */
 
 /* ---- decoders and quality levels (icbc.h:12-17, 21-36) ----------------- */
 #define ICBC_DECODER_D3D10 0
@@ -36,6 +37,9 @@ extern "C" {
 
 /* ---- one encode job: a run of blocks sharing level and options ---------- */
 #define ICBC_JOB_HOST_MEMORY 1u /* colors/weights/codes/errors/stats are host pointers */
+#define ICBC_JOB_COLORS_RGBA8 2u /* colors are 16 x RGBA8 (64 bytes) per block, each channel
+                                    read as c / 255.0f like the reference's harness
+                                    (icbc_test.cpp:401-405) */
 
 typedef struct icbc_bc1_job {
     int32_t level;             /* icbc::Quality 0..9; others give error FLT_MAX, codes untouched */
@@ -43,10 +47,11 @@ typedef struct icbc_bc1_job {
     int32_t three_color_black; /* bool */
     uint32_t flags;            /* ICBC_JOB_* */
     float color_weights[3];    /* per-channel metric weights */
-    uint32_t reserved;         /* must be 0 */
+    uint32_t tile_blocks;      /* blocks per wavefront tile: 8, 16, 32 or 64; 0 = chosen by
+                                  the library (smaller tiles finish a small job sooner) */
     uint64_t block_count;
     const float* colors;       /* 64 floats per block: 16 texels x RGBA, texel = 4*y + x;
-                                  16-byte aligned */
+                                  16-byte aligned (64 bytes with ICBC_JOB_COLORS_RGBA8) */
     const float* weights;      /* 16 floats per block (16-byte aligned), NULL = all 1.0 */
     void* codes;               /* 8 bytes per block: u16 col0, u16 col1, u32 indices */
     float* errors;             /* one float per block (the reference's return value), or NULL */
@@ -62,8 +67,8 @@ typedef struct icbc_bc1_job {
 int icbc_gpu_init(int decoder);
 
 /* Replaces a loop of icbc::compress_bc1 (icbc.h:48, 3846) over many blocks.
- * All jobs are encoded by one kernel launch (one 64-block tile per
- * workgroup).  Device-memory jobs are enqueued on `stream` and the call
+ * All jobs are encoded by one kernel launch: resident workgroups, one
+ * wavefront each, take tiles of 8-64 blocks from a queue.  Device-memory jobs are enqueued on `stream` and the call
  * returns without waiting; host-memory jobs are staged, and the call waits
  * for their results. */
 int icbc_gpu_compress_bc1_batch(const icbc_bc1_job* jobs, size_t n_jobs, void* stream);
@@ -100,8 +105,44 @@ int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_pla
 /* Runs the plan on `stream` (asynchronous). */
 int icbc_gpu_plan_execute(icbc_bc1_plan* plan, void* stream);
 int icbc_gpu_plan_destroy(icbc_bc1_plan* plan);
-/* Number of 64-block tiles in the plan. */
+/* Number of tiles in the plan. */
 uint64_t icbc_gpu_plan_tiles(const icbc_bc1_plan* plan);
+
+/* ---- request server: continuous batching under a memory budget -------- */
+/* No counterpart in the reference (which encodes one block per call on the
+ * caller's thread); for services that encode textures as requests arrive. */
+typedef struct icbc_server icbc_server;
+
+typedef struct icbc_server_options {
+    uint64_t memory_budget; /* device bytes for resident request buffers */
+    uint32_t max_inflight;  /* launches overlapping on the device (0 = 2, at most 8) */
+    uint32_t max_batch;     /* requests merged into one launch (0 = no limit) */
+} icbc_server_options;
+
+typedef struct icbc_request_info {
+    int32_t status;       /* ICBC_OK or an error code */
+    uint32_t batch_size;  /* requests in the launch this one ran in */
+    uint64_t block_count;
+    double submit_s, admit_s, launch_s, done_s; /* steady-clock seconds */
+} icbc_request_info;
+
+typedef struct icbc_server_stats_t {
+    uint64_t launches, completed, blocks, pending, inflight, memory_in_use;
+} icbc_server_stats_t;
+
+int icbc_server_create(const icbc_server_options* opt, icbc_server** server);
+/* Queues one texture (a job with ICBC_JOB_HOST_MEMORY; its buffers must stay
+ * valid until the request completes).  ICBC_ERR_BUDGET if the request can
+ * never fit the memory budget. */
+int icbc_server_submit(icbc_server* server, const icbc_bc1_job* job, uint64_t* ticket);
+/* Waits for a request (timeout < 0: forever); returns its status (or
+ * ICBC_ERR_TIMEOUT) and releases the ticket unless it timed out. */
+int icbc_server_wait(icbc_server* server, uint64_t ticket, double timeout_s, icbc_request_info* info);
+/* Cancels a request that has not been admitted yet. */
+int icbc_server_cancel(icbc_server* server, uint64_t ticket);
+int icbc_server_stats(icbc_server* server, icbc_server_stats_t* stats);
+/* Drains outstanding requests and frees the server. */
+int icbc_server_destroy(icbc_server* server);
 
 /* ---- decode / evaluate (icbc.h:38, 43, 2915-2927, 3030-3046) ----------- */
 /* Device memory: codes 8 B/block, rgba 64 B/block (RGBA8), errors 1 float/block. */
@@ -119,6 +160,8 @@ int icbc_gpu_version(void); /* 10500 + build revision, mirrors icbc v1.05 */
 /* ---- device plumbing for callers without their own HIP runtime ---------- */
 void* icbc_gpu_malloc(size_t bytes);
 int icbc_gpu_free(void* ptr);
+void* icbc_gpu_host_alloc(size_t bytes); /* page-locked host memory (fast DMA) */
+int icbc_gpu_host_free(void* ptr);
 int icbc_gpu_memcpy_htod(void* dst, const void* src, size_t bytes);
 int icbc_gpu_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int icbc_gpu_memset(void* dst, int value, size_t bytes);

@@ -192,6 +192,42 @@ def test_batch_plain_and_plans_agree(gpu, oracle):
             assert_same(res[k][0], res[k][1], want[0], want[1], f"{name} job {k} L{level}")
 
 
+@pytest.mark.parametrize("tile_blocks", [8, 16, 32, 64, 0])
+def test_tile_sizes_agree(gpu, oracle, tile_blocks):
+    colors, _ = synth.make_image_blocks(9, 96, 80, synth.MIX_DEFAULT)   # 480 blocks: ragged tiles
+    ec, ew = synth.make_edge_blocks(91, 100)
+    colors = np.concatenate([colors, ec])
+    weights = np.concatenate([np.ones((480, 16), np.float32), ew])
+    n = len(colors)
+    for level in (1, 5, 8, 9):
+        want = _oracle_run(oracle, level, colors, weights, (1, 1, 1), 1, 1)
+        dc = gpu.DeviceBuffer.from_array(colors)
+        dw = gpu.DeviceBuffer.from_array(weights)
+        dcode = gpu.DeviceBuffer(8 * n)
+        dcode.fill(0)
+        derr = gpu.DeviceBuffer(4 * n)
+        job = gpu.make_job(level, n, dc, dcode, dw, derr, None, (1, 1, 1), True, True, tile_blocks=tile_blocks)
+        for plain in (False, True):
+            gpu.compress_batch([job], plain=plain)
+            assert_same(dcode.download(np.uint8, (n, 8)), derr.download(np.float32, n), want[0], want[1],
+                        f"tile_blocks={tile_blocks} L{level} plain={plain}")
+
+
+def test_rgba8_input_matches_float(gpu, oracle):
+    # 8-bit texels are read as c / 255.0f, exactly as the reference's harness
+    # converts them before calling compress_bc1 (icbc_test.cpp:401-405)
+    colors, _ = synth.make_image_blocks(12, 128, 96, synth.MIX_DEFAULT)
+    c8 = np.clip(np.round(colors * 255), 0, 255).astype(np.uint8)
+    c8[..., 3] = 255
+    cf = c8.astype(np.float32) / np.float32(255)
+    for level in (0, 1, 4, 8, 9):
+        want = _oracle_run(oracle, level, cf, None, (1, 1, 1), 1, 1)
+        got8 = gpu.compress_many(level, c8, None, (1, 1, 1), True, True)
+        gotf = gpu.compress_many(level, cf, None, (1, 1, 1), True, True)
+        assert_same(got8[0], got8[1], want[0], want[1], f"rgba8 L{level}")
+        assert_same(gotf[0], gotf[1], want[0], want[1], f"float L{level}")
+
+
 def test_decode_and_evaluate(gpu, golden, oracle):
     codes = np.ascontiguousarray(golden["dec_codes"])
     n = len(codes)

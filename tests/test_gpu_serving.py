@@ -1,0 +1,196 @@
+"""GPU tests of the request server (continuous batching under a memory
+budget): every request's result is the same alone and in the mix (and equal
+to the oracle), admission respects the budget, oversize and cancelled requests
+get defined statuses, and launches overlap."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from icbc_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(seed, count, max_side=256):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for k in range(count):
+        w = int(np.exp(rng.uniform(np.log(16), np.log(max_side))))
+        h = int(np.exp(rng.uniform(np.log(16), np.log(max_side))))
+        img = synth.make_image(seed * 100 + k, w, h, synth.MIX_DEFAULT)
+        blocks = synth.image_to_blocks(img)
+        level = int(rng.integers(0, 10))
+        tcm = bool(rng.integers(0, 2))
+        weights = rng.random((len(blocks), 16)).astype(np.float32) if k % 5 == 0 else None
+        reqs.append((level, blocks, weights, tcm))
+    return reqs
+
+
+def _oracle(oracle, level, blocks, weights, tcm):
+    codes, errs, _, _ = oracle.compress_many(level, blocks, weights, (1, 1, 1), tcm, tcm, nthreads=8)
+    return codes, errs
+
+
+def test_results_same_alone_and_in_mix(gpu, oracle):
+    reqs = _requests(1, 40)
+    srv = gpu.Server(memory_budget=64 << 20, max_inflight=3)
+    try:
+        tickets = [srv.submit(lv, b, w, (1, 1, 1), tcm, tcm) for lv, b, w, tcm in reqs]
+        mixed = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    finally:
+        srv.close()
+    batch_sizes = [info.batch_size for _, _, info in mixed]
+    assert max(batch_sizes) > 1, "requests were never batched together"
+    for (lv, b, w, tcm), (codes, errs, info) in zip(reqs, mixed):
+        assert info.status == 0
+        alone = gpu.compress_many(lv, b, w, (1, 1, 1), tcm, tcm)
+        want = _oracle(oracle, lv, b, w, tcm)
+        assert np.array_equal(codes, alone[0]) and np.array_equal(errs.view(np.uint32), alone[1].view(np.uint32))
+        assert np.array_equal(codes, want[0]) and np.array_equal(errs.view(np.uint32), want[1].view(np.uint32))
+    assert st["completed"] == len(reqs)
+
+
+def test_budget_admission(gpu):
+    reqs = _requests(2, 60, max_side=512)
+    budget = 32 << 20
+    srv = gpu.Server(memory_budget=budget, max_inflight=2)
+    peak = [0]
+    stop = threading.Event()
+
+    def watch():
+        while not stop.is_set():
+            peak[0] = max(peak[0], srv.stats()["memory_in_use"])
+            time.sleep(0.0005)
+
+    th = threading.Thread(target=watch)
+    th.start()
+    try:
+        tickets = [srv.submit(lv, b, w, (1, 1, 1), tcm, tcm) for lv, b, w, tcm in reqs]
+        infos = [srv.wait(t)[2] for t in tickets]
+    finally:
+        stop.set()
+        th.join()
+        srv.close()
+    assert 0 < peak[0] <= budget
+    # FIFO admission: admit times follow submit order
+    admits = [i.admit_s for i in infos]
+    assert all(a <= b for a, b in zip(admits, admits[1:]))
+    assert all(i.status == 0 for i in infos)
+
+
+def test_launches_overlap(gpu):
+    # a long level-9 texture, then small requests: they are admitted and
+    # launched on the second slot while the long one is still on the device.
+    # The first round warms both launch slots (staging buffers, pool slabs).
+    big, _ = synth.make_image_blocks(6, 2048, 2048, synth.MIX_DEFAULT)
+    small = _requests(7, 16, max_side=64)
+    srv = gpu.Server(memory_budget=1 << 30, max_inflight=2)
+    try:
+        for _ in range(2):
+            tb = srv.submit(9, big, None, (1, 1, 1), True, True)
+            ts = [srv.submit(lv, b, w, (1, 1, 1), tcm, tcm) for lv, b, w, tcm in small]
+            big_info = srv.wait(tb)[2]
+            infos = [srv.wait(t)[2] for t in ts]
+    finally:
+        srv.close()
+    later = [i for i in infos if i.launch_s > big_info.launch_s]
+    detail = (f"big launch {big_info.launch_s:.6f} done {big_info.done_s:.6f}; small "
+              + ", ".join(f"({i.launch_s:.6f},{i.done_s:.6f},b{i.batch_size})" for i in infos))
+    assert later, "small requests all joined the long request's launch: " + detail
+    assert any(i.launch_s < big_info.done_s for i in later), "no launch overlapped the long one: " + detail
+
+
+def test_oversize_request_rejected(gpu):
+    srv = gpu.Server(memory_budget=1 << 20)
+    try:
+        big = np.zeros((8192, 16, 4), np.float32)   # 2 MiB of texels
+        with pytest.raises(gpu.IcbcError) as e:
+            srv.submit(8, big)
+        assert e.value.code == gpu.ERR_BUDGET
+    finally:
+        srv.close()
+
+
+def test_cancel_pending_request(gpu):
+    # a budget that holds one request at a time keeps the later ones pending
+    blocks, _ = synth.make_image_blocks(3, 1024, 1024, synth.MIX_DEFAULT)   # 65536 blocks, ~17 MiB
+    srv = gpu.Server(memory_budget=32 << 20, max_inflight=1)
+    try:
+        t1 = srv.submit(9, blocks, None, (1, 1, 1), True, True)
+        t2 = srv.submit(9, blocks, None, (1, 1, 1), True, True)
+        t3 = srv.submit(8, blocks[:64])
+        srv.cancel(t3)
+        with pytest.raises(gpu.IcbcError) as e:
+            srv.wait(t3)
+        assert e.value.code == gpu.ERR_CANCELLED
+        c1, e1, _ = srv.wait(t1)
+        c2, e2, _ = srv.wait(t2)
+        assert np.array_equal(c1, c2)
+    finally:
+        srv.close()
+
+
+def test_invalid_level_request(gpu):
+    blocks, _ = synth.make_image_blocks(4, 32, 32, synth.MIX_DEFAULT)
+    srv = gpu.Server(memory_budget=16 << 20)
+    try:
+        prior = np.full((len(blocks), 8), 0xab, np.uint8)
+        t = srv.submit(11, blocks, codes=prior)
+        codes, errs, info = srv.wait(t)
+        assert info.status == 0
+        assert np.all(codes == 0xab)
+        assert np.all(errs == np.finfo(np.float32).max)
+    finally:
+        srv.close()
+
+
+def test_concurrent_clients(gpu, oracle):
+    reqs = _requests(5, 24)
+    srv = gpu.Server(memory_budget=32 << 20, max_inflight=2)
+    results = {}
+
+    def client(k):
+        lv, b, w, tcm = reqs[k]
+        t = srv.submit(lv, b, w, (1, 1, 1), tcm, tcm)
+        results[k] = srv.wait(t)
+
+    try:
+        ths = [threading.Thread(target=client, args=(k,)) for k in range(len(reqs))]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+    finally:
+        srv.close()
+    for k, (lv, b, w, tcm) in enumerate(reqs):
+        want = _oracle(oracle, lv, b, w, tcm)
+        assert np.array_equal(results[k][0], want[0])
+
+
+def test_identity_under_load(gpu):
+    # many overlapping requests over a small pool of textures, inputs in
+    # page-locked memory: every served result equals the texture encoded alone
+    reqs = _requests(8, 20, max_side=192)
+    pinned = [gpu.pinned_copy(b) for _, b, _, _ in reqs]
+    alone = [gpu.compress_many(lv, p.array, w, (1, 1, 1), tcm, tcm)[0]
+             for (lv, _, w, tcm), p in zip(reqs, pinned)]
+    srv = gpu.Server(memory_budget=24 << 20, max_inflight=3)
+    rng = np.random.default_rng(9)
+    try:
+        order = rng.integers(0, len(reqs), size=400)
+        tickets = []
+        for k in order:
+            lv, _, w, tcm = reqs[k]
+            codes = np.empty((len(pinned[k].array), 8), np.uint8)
+            tickets.append((srv.submit(lv, pinned[k].array, w, (1, 1, 1), tcm, tcm, codes=codes), k))
+        for t, k in tickets:
+            codes, _, info = srv.wait(t)
+            assert np.array_equal(codes, alone[k]), f"request {t} (texture {k}, batch of {info.batch_size})"
+        assert srv.stats()["launches"] > 3
+    finally:
+        srv.close()
+        for p in pinned:
+            p.close()
