@@ -36,7 +36,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "BC1 Mblocks/s at level 8 (bit-exact), batched vs plain GPU; p99 request latency"
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector, dense (AMD spec)
-PEAK_FP32_NOFMA_TFLOPS = 39.3  # same units with unfused, unpacked FP32 (bit-exact build)
+PEAK_FP32_NOFMA_TFLOPS = 78.65  # one unfused FP32 op per lane per clock (no FMA in the bit-exact build)
 # algorithmic FP32 operations per unit of work (divisions count 1), from the
 # reference's arithmetic: one 4-cluster candidate, one 3-cluster candidate,
 # one refinement step (icbc.h:2685-2742, 2299-2344, 3614-3648)
@@ -224,6 +224,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the short serving run included in the default benchmark")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (HBM-counter calibration "
+                         "under rocprofv3 --pmc)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -276,6 +279,15 @@ def main():
                 for lv, n, o in zip(job_levels, counts, offsets[:-1])]
 
     stream = g.Stream()
+    if args.copy_calibration_mb:
+        nb = args.copy_calibration_mb << 20
+        src, dst = g.DeviceBuffer(nb), g.DeviceBuffer(nb)
+        src.fill(1)
+        g.check(g.lib().icbc_gpu_memcpy_dtod(dst.ptr, src.ptr, nb, stream.handle))
+        stream.synchronize()
+        src.close()
+        dst.close()
+        log(f"calibration copy: {nb} bytes device to device")
     plan = g.Plan(jobs(), schedule=g.SCHEDULE_PERSISTENT, use_graph=True, order_by_cost=True,
                   stream=stream)
     ev = [g.Event() for _ in range(args.steps + 1)]

@@ -122,6 +122,7 @@ _SIGNATURES = {
     "icbc_gpu_host_free": (C.c_int, [C.c_void_p]),
     "icbc_gpu_memcpy_htod": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "icbc_gpu_memcpy_dtoh": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "icbc_gpu_memcpy_dtod": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "icbc_gpu_memset": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t]),
     "icbc_gpu_stream_create": (C.c_void_p, []),
     "icbc_gpu_stream_destroy": (C.c_int, [C.c_void_p]),

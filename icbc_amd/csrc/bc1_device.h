@@ -322,6 +322,70 @@ __device__ __forceinline__ float emit3(const Texels& t, f3 cw, int decoder, bool
     return e;
 }
 
+// output_block4 and, when `with3`, output_block3 (icbc.h:3238-3274) sharing
+// one pass over the texels: per texel both index rules run and each error sum
+// accumulates in texel order, as two separate passes would.
+__device__ __forceinline__ float emit43(const Texels& t, f3 w, int decoder, bool allow_black, bool with3, f3 a4,
+                                        f3 b4, f3 a3, f3 b3, Code& out4, Code& out3, float& err3) {
+    uint32_t c40 = to565(a4), c41 = to565(b4);
+    if (c40 < c41) {
+        uint32_t tmp = c40;
+        c40 = c41;
+        c41 = tmp;
+    }
+    uint32_t c30 = to565(a3), c31 = to565(b3);
+    if (c30 > c31) {
+        uint32_t tmp = c30;
+        c30 = c31;
+        c31 = tmp;
+    }
+    f3 pal4[4], pal3[4];
+    palette_f(c40, c41, decoder, pal4);
+    palette_f(c30, c31, decoder, pal3);
+    const f3 p0 = pal4[0] * w, p1 = pal4[1] * w, p2 = pal4[2] * w, p3 = pal4[3] * w;
+    const f3 q0 = pal3[0] * w, q1 = pal3[1] * w, q2 = pal3[2] * w;
+    uint32_t idx4 = 0, idx3 = 0;
+    float e4 = 0.0f, e3 = 0.0f;
+#pragma unroll 2
+    for (int i = 0; i < 16; i++) {
+        const f3 x = t.color(i);
+        const float wt = t.weight(i);
+        const f3 c = x * w;
+        {  // compute_indices4 rule (icbc.h:3097-3131)
+            const f3 e0 = c - p0, e1 = c - p1, e2 = c - p2, ee3 = c - p3;
+            const float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2), d3 = dot3(ee3, ee3);
+            const uint32_t hi = ((d1 > d2) & (d0 > d2)) | ((d0 > d3) & (d1 > d3));
+            const uint32_t lo = (d0 > d3) & (d2 > d3);
+            const uint32_t k = lo | (hi << 1);
+            idx4 |= k << (2 * i);
+            e4 += wt * texel_error(pick(pal4, k), x, w);
+        }
+        if (with3) {  // compute_indices3 rule (icbc.h:3133-3176)
+            uint32_t k;
+            if (allow_black) {
+                const f3 e0 = q0 - c, e1 = q1 - c, e2 = q2 - c;
+                const float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2), d3 = dot3(c, c);
+                const uint32_t i1 = d1 < d2;
+                const uint32_t i2 = (d2 <= d0) & (d2 <= d1);
+                const uint32_t i3 = (d3 <= d0) & (d3 <= d1) & (d3 <= d2);
+                k = (i1 | i3) | ((i2 | i3) << 1);
+            } else {
+                const f3 e0 = c - q0, e1 = c - q1, e2 = c - q2;
+                const float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2);
+                const uint32_t i1 = d1 < d2;
+                const uint32_t i2 = (d2 <= d0) & (d2 <= d1);
+                k = i1 | (i2 << 1);
+            }
+            idx3 |= k << (2 * i);
+            e3 += wt * texel_error(pick(pal3, k), x, w);
+        }
+    }
+    out4 = make_code(c40, c41, idx4);
+    out3 = make_code(c30, c31, idx3);
+    err3 = e3;
+    return e4;
+}
+
 // ---------------------------------------------------------------------------
 // Colour reduction (icbc.h:1673-1748)
 

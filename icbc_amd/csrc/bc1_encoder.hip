@@ -100,16 +100,21 @@ __device__ __forceinline__ int next_block(int my_n, int tile_blocks, int from, i
 // returned in the owning lane (-1 if none).  SAT rows stream from the
 // workgroup's workspace through a double-buffered LDS slot: the next block's
 // rows are loaded into registers while the current block is searched.
+// Block b's SAT is at ws + b * kSatRows, or alt_offset further when its owner
+// passed my_alt != 0.
 template <int kClusters>
-__device__ __forceinline__ int search(Shared& sh, const float4* ws, int tile_blocks, int my_n, f3 metric,
-                                      const DecoderWeights& dw) {
+__device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offset, int my_alt, int tile_blocks,
+                                      int my_n, f3 metric, const DecoderWeights& dw) {
     constexpr int kMinN = kClusters == 4 ? 2 : 1;
     const int lane = threadIdx.x;
     int my_win = -1;
     int b = next_block(my_n, tile_blocks, 0, kMinN);
     int nb = b < tile_blocks ? __builtin_amdgcn_readlane(my_n, b) : 0;
     float4 row = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (b < tile_blocks && lane <= nb) row = ws[b * kSatRows + lane];
+    auto base = [&](int blk) {
+        return ws + blk * kSatRows + (__builtin_amdgcn_readlane(my_alt, blk) ? alt_offset : 0);
+    };
+    if (b < tile_blocks && lane <= nb) row = base(b)[lane];
     int buf = 0;
     while (b < tile_blocks) {
         if (lane <= nb) sh.sat[buf][lane] = row;
@@ -117,14 +122,14 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws, int tile_blo
         // prefetch the next block's rows while this one is searched
         const int b2 = next_block(my_n, tile_blocks, b + 1, kMinN);
         const int nb2 = b2 < tile_blocks ? __builtin_amdgcn_readlane(my_n, b2) : 0;
-        if (b2 < tile_blocks && lane <= nb2) row = ws[b2 * kSatRows + lane];
+        if (b2 < tile_blocks && lane <= nb2) row = base(b2)[lane];
 
         const float4* sat = sh.sat[buf];
         const float4 tot = sat[nb];
         float best = FLT_MAX;
         int besti = kNoWinner;
         if (kClusters == 4) {
-            const int total = k_four_total[nb - 1];
+            const int total = (nb + 3) * (nb + 2) * (nb + 1) / 6 - 1;  // k_four_total[nb - 1]
             for (int i = lane; i < total; i += kTile) {
                 const uint32_t pk = sh.four[i];
                 f3 a, e;
@@ -136,7 +141,7 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws, int tile_blo
                 }
             }
         } else {
-            const int total = k_three_total[nb - 1];
+            const int total = (nb + 2) * (nb + 1) / 2 - 1;  // k_three_total[nb - 1]
             for (int i = lane; i < total; i += kTile) {
                 const uint32_t pk = sh.three[i];
                 f3 a, e;
@@ -232,7 +237,7 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, S
 #pragma unroll
             for (int i = 0; i < 16; i++) park[i] = make_float4(rc[i].x, rc[i].y, rc[i].z, rw[i]);
         }
-        if (fit) build_sat(rc, rw, n, sat4, (use3 && !skip) ? sat3 : nullptr);
+        if (fit) build_sat(rc, rw, n, sat4, nullptr);
         if (skip) {
 #pragma unroll
             for (int i = 0; i < 16; i++) {
@@ -246,30 +251,29 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, S
         }
         if (use3) n3 = m;
         __syncthreads();
-        const int win4 = search<4>(sh, ws, tile_blocks, fit ? n : 0, metric, dw);
-        const int win3 = search<3>(sh, ws + kTile * kSatRows, tile_blocks, use3 ? m : 0, metric, dw);
+        // 3-cluster SATs: the skip_blacks table where one was built, else the 4-cluster one
+        const int win4 = search<4>(sh, ws, 0, 0, tile_blocks, fit ? n : 0, metric, dw);
+        const int win3 = search<3>(sh, ws, kTile * kSatRows, skip ? 1 : 0, tile_blocks, use3 ? m : 0, metric, dw);
 
         if (fit) {
-            f3 a = mk(0.0f, 0.0f, 0.0f), b = mk(0.0f, 0.0f, 0.0f);
+            f3 a4 = mk(0.0f, 0.0f, 0.0f), b4 = mk(0.0f, 0.0f, 0.0f);
             if (win4 >= 0) {
                 const uint32_t pk = sh.four[win4];
-                candidate4(sat4[pk & 0xff], sat4[(pk >> 8) & 0xff], sat4[pk >> 16], sat4[n], metric, dw, a, b);
+                candidate4(sat4[pk & 0xff], sat4[(pk >> 8) & 0xff], sat4[pk >> 16], sat4[n], metric, dw, a4, b4);
             }
-            Code oc;
-            float ec = emit4(tx, cw, dec, a, b, oc);
-            if (use3) {
-                a = mk(0.0f, 0.0f, 0.0f);
-                b = mk(0.0f, 0.0f, 0.0f);
-                if (win3 >= 0) {
-                    const uint32_t pk = sh.three[win3];
-                    candidate3(sat3[pk & 0xff], sat3[(pk >> 8) & 0xff], sat3[m], metric, a, b);
-                }
-                Code o3;
-                const float e3 = emit3(tx, cw, dec, job.tcb != 0, a, b, o3);
-                if (e3 < ec) {
-                    ec = e3;
-                    oc = o3;
-                }
+            f3 a3 = mk(0.0f, 0.0f, 0.0f), b3 = mk(0.0f, 0.0f, 0.0f);
+            if (use3 && win3 >= 0) {
+                const float4* s3 = skip ? sat3 : sat4;
+                const uint32_t pk = sh.three[win3];
+                candidate3(s3[pk & 0xff], s3[(pk >> 8) & 0xff], s3[m], metric, a3, b3);
+            }
+            // output_block4 and output_block3 in one pass over the texels
+            Code oc, o3;
+            float e3;
+            float ec = emit43(tx, cw, dec, job.tcb != 0, use3, a4, b4, a3, b3, oc, o3, e3);
+            if (use3 && e3 < ec) {
+                ec = e3;
+                oc = o3;
             }
             if (ec < err) {
                 err = ec;

@@ -894,6 +894,11 @@ int icbc_gpu_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
     return ICBC_OK;
 }
 
+int icbc_gpu_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream) {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return ICBC_OK;
+}
+
 int icbc_gpu_memset(void* dst, int value, size_t bytes) {
     HIP_TRY(hipMemset(dst, value, bytes));
     return ICBC_OK;

@@ -164,6 +164,7 @@ void* icbc_gpu_host_alloc(size_t bytes); /* page-locked host memory (fast DMA) *
 int icbc_gpu_host_free(void* ptr);
 int icbc_gpu_memcpy_htod(void* dst, const void* src, size_t bytes);
 int icbc_gpu_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int icbc_gpu_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
 int icbc_gpu_memset(void* dst, int value, size_t bytes);
 void* icbc_gpu_stream_create(void);
 int icbc_gpu_stream_destroy(void* stream);
