@@ -154,12 +154,12 @@ __device__ __forceinline__ void palette8(uint32_t c0, uint32_t c1, int decoder, 
 
 // palette as floats: channel / 255.0f (icbc.h:1656, 2902-2909), from a table
 // holding exactly those quotients.
-__device__ __forceinline__ void palette_f(uint32_t c0, uint32_t c1, int decoder, f3 pal[4]) {
+// `unit` is the 256-entry table of k / 255.0f (an LDS copy in the encoder).
+__device__ __forceinline__ void palette_f(uint32_t c0, uint32_t c1, int decoder, const float* unit, f3 pal[4]) {
     Pal8 p;
     palette8(c0, c1, decoder, p);
 #pragma unroll
-    for (int i = 0; i < 4; i++)
-        pal[i] = mk(k_u8_to_unit[p.r[i]], k_u8_to_unit[p.g[i]], k_u8_to_unit[p.b[i]]);
+    for (int i = 0; i < 4; i++) pal[i] = mk(unit[p.r[i]], unit[p.g[i]], unit[p.b[i]]);
 }
 
 // ---------------------------------------------------------------------------
@@ -198,9 +198,9 @@ __device__ __forceinline__ float palette_error(const Texels& t, f3 cw, const f3 
     return e;
 }
 
-__device__ __forceinline__ float code_error(const Texels& t, f3 cw, int decoder, Code c) {
+__device__ __forceinline__ float code_error(const Texels& t, f3 cw, int decoder, const float* unit, Code c) {
     f3 pal[4];
-    palette_f(col0(c), col1(c), decoder, pal);
+    palette_f(col0(c), col1(c), decoder, unit, pal);
     return palette_error(t, cw, pal, c.idx);
 }
 
@@ -266,32 +266,36 @@ __device__ __forceinline__ float select3(const Texels& t, f3 w, bool allow_black
 // compute_indices rule used by refinement (icbc.h:3179-3235): indices chosen
 // against `sel` (the current best block's palette), error measured against
 // `pal` (the candidate block's palette).
-__device__ __forceinline__ float select_refine(const Texels& t, f3 w, const f3 sel[4], const f3 pal[4],
-                                               uint32_t& indices) {
+// Indices of the 16 texels against `sel` (palette of the current best block):
+// nearest entry under the colour weights, lowest index on ties.
+__device__ __forceinline__ uint32_t select_indices(const Texels& t, f3 w, const f3 sel[4]) {
     const f3 p0 = sel[0] * w, p1 = sel[1] * w, p2 = sel[2] * w, p3 = sel[3] * w;
     uint32_t out = 0;
-    float e = 0.0f;
 #pragma unroll 4
     for (int i = 0; i < 16; i++) {
-        const f3 x = t.color(i);
-        const f3 c = x * w;
+        const f3 c = t.color(i) * w;
         const f3 e0 = c - p0, e1 = c - p1, e2 = c - p2, e3 = c - p3;
         const float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2), d3 = dot3(e3, e3);
         const uint32_t i1 = (d1 <= d0) & (d1 < d2) & (d1 < d3);
         const uint32_t i2 = (d2 <= d0) & (d2 <= d1) & (d2 < d3);
         const uint32_t i3 = (d3 <= d0) & (d3 <= d1) & (d3 <= d2);
-        const uint32_t k = (i1 | i3) | ((i2 | i3) << 1);
-        out |= k << (2 * i);
-        e += t.weight(i) * texel_error(pick(pal, k), x, w);
+        out |= ((i1 | i3) | ((i2 | i3) << 1)) << (2 * i);
     }
-    indices = out;
+    return out;
+}
+
+// Weighted squared error of fixed `indices` against palette `pal`.
+__device__ __forceinline__ float indexed_error(const Texels& t, f3 w, const f3 pal[4], uint32_t indices) {
+    float e = 0.0f;
+#pragma unroll 4
+    for (int i = 0; i < 16; i++) e += t.weight(i) * texel_error(pick(pal, (indices >> (2 * i)) & 3), t.color(i), w);
     return e;
 }
 
 // ---------------------------------------------------------------------------
 // Block output (icbc.h:3238-3274)
 
-__device__ __forceinline__ float emit4(const Texels& t, f3 cw, int decoder, f3 a, f3 b, Code& out) {
+__device__ __forceinline__ float emit4(const Texels& t, f3 cw, int decoder, const float* unit, f3 a, f3 b, Code& out) {
     uint32_t c0 = to565(a), c1 = to565(b);
     if (c0 < c1) {
         uint32_t tmp = c0;
@@ -299,15 +303,15 @@ __device__ __forceinline__ float emit4(const Texels& t, f3 cw, int decoder, f3 a
         c1 = tmp;
     }
     f3 pal[4];
-    palette_f(c0, c1, decoder, pal);
+    palette_f(c0, c1, decoder, unit, pal);
     uint32_t idx;
     const float e = select4(t, cw, pal, idx);
     out = make_code(c0, c1, idx);
     return e;
 }
 
-__device__ __forceinline__ float emit3(const Texels& t, f3 cw, int decoder, bool allow_black, f3 a, f3 b,
-                                       Code& out) {
+__device__ __forceinline__ float emit3(const Texels& t, f3 cw, int decoder, const float* unit, bool allow_black,
+                                       f3 a, f3 b, Code& out) {
     uint32_t c0 = to565(a), c1 = to565(b);
     if (c0 > c1) {
         uint32_t tmp = c0;
@@ -315,7 +319,7 @@ __device__ __forceinline__ float emit3(const Texels& t, f3 cw, int decoder, bool
         c1 = tmp;
     }
     f3 pal[4];
-    palette_f(c0, c1, decoder, pal);
+    palette_f(c0, c1, decoder, unit, pal);
     uint32_t idx;
     const float e = select3(t, cw, allow_black, pal, idx);
     out = make_code(c0, c1, idx);
@@ -325,8 +329,8 @@ __device__ __forceinline__ float emit3(const Texels& t, f3 cw, int decoder, bool
 // output_block4 and, when `with3`, output_block3 (icbc.h:3238-3274) sharing
 // one pass over the texels: per texel both index rules run and each error sum
 // accumulates in texel order, as two separate passes would.
-__device__ __forceinline__ float emit43(const Texels& t, f3 w, int decoder, bool allow_black, bool with3, f3 a4,
-                                        f3 b4, f3 a3, f3 b3, Code& out4, Code& out3, float& err3) {
+__device__ __forceinline__ float emit43(const Texels& t, f3 w, int decoder, const float* unit, bool allow_black,
+                                        bool with3, f3 a4, f3 b4, f3 a3, f3 b3, Code& out4, Code& out3, float& err3) {
     uint32_t c40 = to565(a4), c41 = to565(b4);
     if (c40 < c41) {
         uint32_t tmp = c40;
@@ -340,8 +344,8 @@ __device__ __forceinline__ float emit43(const Texels& t, f3 w, int decoder, bool
         c31 = tmp;
     }
     f3 pal4[4], pal3[4];
-    palette_f(c40, c41, decoder, pal4);
-    palette_f(c30, c31, decoder, pal3);
+    palette_f(c40, c41, decoder, unit, pal4);
+    palette_f(c30, c31, decoder, unit, pal3);
     const f3 p0 = pal4[0] * w, p1 = pal4[1] * w, p2 = pal4[2] * w, p3 = pal4[3] * w;
     const f3 q0 = pal3[0] * w, q1 = pal3[1] * w, q2 = pal3[2] * w;
     uint32_t idx4 = 0, idx3 = 0;
@@ -743,11 +747,16 @@ __device__ __forceinline__ void refine_step(int k, int& dr, int& dg, int& db) {
     db = (int)((kPackB >> (2 * k)) & 3) - 1;
 }
 
-__device__ __forceinline__ float refine(const Texels& t, f3 cw, int decoder, bool three_color_mode,
-                                        float input_error, Code& out, int& steps) {
+__device__ __forceinline__ float refine(const Texels& t, f3 cw, int decoder, const float* unit,
+                                        bool three_color_mode, float input_error, Code& out, int& steps) {
     float best = input_error;
     int last = 0;
     int i = 0;
+    // Every step takes its indices from the palette of the current best
+    // block, so they only change when a step is accepted.
+    f3 sel[4];
+    palette_f(col0(out), col1(out), decoder, unit, sel);
+    uint32_t idx = select_indices(t, cw, sel);
     for (; i < 256; i++) {
         int dr, dg, db;
         refine_step(i & 15, dr, dg, db);
@@ -762,15 +771,14 @@ __device__ __forceinline__ float refine(const Texels& t, f3 cw, int decoder, boo
                 c1 = tmp;
             }
         }
-        f3 sel[4], pal[4];
-        palette_f(col0(out), col1(out), decoder, sel);  // palette of the current best block
-        palette_f(c0, c1, decoder, pal);
-        uint32_t idx;
-        const float e = select_refine(t, cw, sel, pal, idx);
+        f3 pal[4];
+        palette_f(c0, c1, decoder, unit, pal);
+        const float e = indexed_error(t, cw, pal, idx);
         if (e < best) {
             best = e;
             out = make_code(c0, c1, idx);
             last = i;
+            idx = select_indices(t, cw, pal);
         }
         if (i - last > 32) break;
     }

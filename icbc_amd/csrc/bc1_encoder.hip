@@ -84,6 +84,7 @@ constexpr int kNoWinner = 0x7fffffff;
 struct Shared {
     uint32_t four[976];
     uint32_t three[160];
+    float unit[256];  // k / 255.0f
     float4 sat[2][kSatRows];
 };
 
@@ -197,7 +198,7 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, S
     }
     if (!done && n == 1) {  // one colour: optimal single-colour tables (icbc.h:3761-3764)
         out = single_color(to_u8(rc[0].x), to_u8(rc[0].y), to_u8(rc[0].z), dec);
-        err = code_error(tx, cw, dec, out);
+        err = code_error(tx, cw, dec, sh.unit, out);
         written = true;
         done = true;
     }
@@ -206,13 +207,13 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, S
         f3 c0, c1;
         if (p.cluster) box_fit([&](int i) { return rc[i]; }, n, c0, c1);
         else box_fit([&](int i) { return tx.color(i); }, 16, c0, c1);
-        err = emit4(tx, cw, dec, c0, c1, out);
+        err = emit4(tx, cw, dec, sh.unit, c0, c1, out);
         written = true;
         if (p.lsq) {
             f3 a, b;
             if (lsq_endpoints(out.idx, tx, a, b)) {
                 Code o2;
-                const float e2 = emit4(tx, cw, dec, a, b, o2);
+                const float e2 = emit4(tx, cw, dec, sh.unit, a, b, o2);
                 if (e2 < err) {
                     err = e2;
                     out = o2;
@@ -270,7 +271,7 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, S
             // output_block4 and output_block3 in one pass over the texels
             Code oc, o3;
             float e3;
-            float ec = emit43(tx, cw, dec, job.tcb != 0, use3, a4, b4, a3, b3, oc, o3, e3);
+            float ec = emit43(tx, cw, dec, sh.unit, job.tcb != 0, use3, a4, b4, a3, b3, oc, o3, e3);
             if (use3 && e3 < ec) {
                 ec = e3;
                 oc = o3;
@@ -291,7 +292,7 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, S
             out = Code{v.x, v.y};
             written = true;
         }
-        err = refine(tx, cw, dec, job.tcm != 0, err, out, steps);
+        err = refine(tx, cw, dec, sh.unit, job.tcm != 0, err, out, steps);
     }
 
     if (active) {
@@ -307,6 +308,7 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, S
 __device__ __forceinline__ void load_tables(Shared& sh) {
     for (int i = threadIdx.x; i < 976; i += kTile) sh.four[i] = k_four[i];
     for (int i = threadIdx.x; i < 160; i += kTile) sh.three[i] = k_three[i];
+    for (int i = threadIdx.x; i < 256; i += kTile) sh.unit[i] = k_u8_to_unit[i];
     __syncthreads();
 }
 
