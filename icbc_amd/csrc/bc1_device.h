@@ -285,12 +285,31 @@ __device__ __forceinline__ uint32_t select_indices(const Texels& t, f3 w, const 
 }
 
 // Weighted squared error of fixed `indices` against palette `pal`.
-__device__ __forceinline__ float indexed_error(const Texels& t, f3 w, const f3 pal[4], uint32_t indices) {
+template <typename Src>
+__device__ __forceinline__ float indexed_error(const Src& t, f3 w, const f3 pal[4], uint32_t indices) {
     float e = 0.0f;
-#pragma unroll 4
+#pragma unroll
     for (int i = 0; i < 16; i++) e += t.weight(i) * texel_error(pick(pal, (indices >> (2 * i)) & 3), t.color(i), w);
     return e;
 }
+
+// The block's texels held in registers for the refinement walk, which reads
+// them on every step; unweighted blocks keep no weights.
+template <bool kWeighted>
+struct TexelRegs {
+    f3 c[16];
+    float w[kWeighted ? 16 : 1];
+    __device__ __forceinline__ explicit TexelRegs(const Texels& t) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) c[i] = t.color(i);
+        if (kWeighted) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) w[kWeighted ? i : 0] = t.w[i];
+        }
+    }
+    __device__ __forceinline__ f3 color(int i) const { return c[i]; }
+    __device__ __forceinline__ float weight(int i) const { return kWeighted ? w[kWeighted ? i : 0] : 1.0f; }
+};
 
 // ---------------------------------------------------------------------------
 // Block output (icbc.h:3238-3274)
@@ -747,8 +766,10 @@ __device__ __forceinline__ void refine_step(int k, int& dr, int& dg, int& db) {
     db = (int)((kPackB >> (2 * k)) & 3) - 1;
 }
 
-__device__ __forceinline__ float refine(const Texels& t, f3 cw, int decoder, const float* unit,
-                                        bool three_color_mode, float input_error, Code& out, int& steps) {
+template <bool kWeighted>
+__device__ __forceinline__ float refine_walk(const Texels& t, f3 cw, int decoder, const float* unit,
+                                             bool three_color_mode, float input_error, Code& out, int& steps) {
+    const TexelRegs<kWeighted> r(t);
     float best = input_error;
     int last = 0;
     int i = 0;
@@ -773,7 +794,7 @@ __device__ __forceinline__ float refine(const Texels& t, f3 cw, int decoder, con
         }
         f3 pal[4];
         palette_f(c0, c1, decoder, unit, pal);
-        const float e = indexed_error(t, cw, pal, idx);
+        const float e = indexed_error(r, cw, pal, idx);
         if (e < best) {
             best = e;
             out = make_code(c0, c1, idx);
@@ -784,6 +805,12 @@ __device__ __forceinline__ float refine(const Texels& t, f3 cw, int decoder, con
     }
     steps = (i < 256) ? i + 1 : 256;
     return best;
+}
+
+__device__ __forceinline__ float refine(const Texels& t, f3 cw, int decoder, const float* unit,
+                                        bool three_color_mode, float input_error, Code& out, int& steps) {
+    if (t.w) return refine_walk<true>(t, cw, decoder, unit, three_color_mode, input_error, out, steps);
+    return refine_walk<false>(t, cw, decoder, unit, three_color_mode, input_error, out, steps);
 }
 
 // ---------------------------------------------------------------------------

@@ -14,6 +14,8 @@
 // scalar loop, the lowest-index candidate among those with the smallest error
 // below FLT_MAX (NaN never wins); with no such candidate both endpoints are 0.
 
+#include <algorithm>
+
 #include "bc1_device.h"
 #include "bc1_kernels.h"
 
@@ -167,6 +169,7 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offs
 // One tile: the reference's compress_bc1 driver (icbc.h:3734-3808) per lane.
 // `ws` is this workgroup's workspace: 64 blocks x 17 SAT rows.
 
+template <bool kRefine>
 __device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, Shared& sh, float4* ws) {
     const int lane = threadIdx.x;
     const int tile_blocks = job.tile_blocks;
@@ -285,7 +288,7 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, S
         __syncthreads();  // workspace rows are reused by the next tile
     }
 
-    if (p.refine && !done) {  // icbc.h:3803-3805
+    if (kRefine && p.refine && !done) {  // icbc.h:3803-3805
         if (!written) {
             // the reference refines whatever the output block holds
             const uint2 v = job.codes[bi];
@@ -316,10 +319,12 @@ __device__ __forceinline__ void load_tables(Shared& sh) {
 // tiles are taken from a queue in list order (the batched path); without, a
 // workgroup takes tiles blockIdx.x, blockIdx.x + gridDim.x, ...  With `tiles`
 // null the launch covers one job, tile k = blocks [64 k, 64 k + 64).
-__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) void bc1_encode_kernel(const JobDev* __restrict__ jobs,
-                                                           const uint2* __restrict__ tiles,
-                                                           uint32_t ntiles, uint32_t* __restrict__ counter,
-                                                           float4* __restrict__ workspace) {
+// Only the kRefine instance holds the level-9 refinement walk, whose register
+// demand would otherwise weigh on every level.
+template <bool kRefine>
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) void bc1_encode_kernel(
+    const JobDev* __restrict__ jobs, const uint2* __restrict__ tiles, uint32_t ntiles,
+    uint32_t* __restrict__ counter, float4* __restrict__ workspace) {
     __shared__ Shared sh;
     load_tables(sh);
     float4* ws = workspace + (size_t)blockIdx.x * (kWorkspaceBytesPerWorkgroup / sizeof(float4));
@@ -338,7 +343,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) v
             job = tile.x;
             first = tile.y;
         }
-        encode_tile(jobs[job], first, sh, ws);
+        encode_tile<kRefine>(jobs[job], first, sh, ws);
         if (!counter) t += gridDim.x;
     }
 }
@@ -419,15 +424,16 @@ __global__ void bc1_evaluate_kernel(const uint32_t* __restrict__ rgba, const uin
 // launch glue
 
 int encode_workgroups_per_cu() {
-    int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, bc1_encode_kernel, kTile, 0) != hipSuccess ||
-        blocks < 1)
-        blocks = 1;
-    return blocks;
+    int plain = 0, refine = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&plain, bc1_encode_kernel<false>, kTile, 0) != hipSuccess)
+        plain = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&refine, bc1_encode_kernel<true>, kTile, 0) != hipSuccess)
+        refine = 1;
+    return std::max(1, std::min(plain, refine));
 }
 
 hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, uint32_t* counter,
-                         float4* workspace, uint32_t grid, hipStream_t stream) {
+                         float4* workspace, uint32_t grid, bool refine, hipStream_t stream) {
     if (ntiles == 0) return hipSuccess;
     if (grid > ntiles) grid = ntiles;
     if (counter) {
@@ -435,8 +441,12 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(bc1_encode_kernel, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles, counter,
-                       workspace);
+    if (refine)
+        hipLaunchKernelGGL(bc1_encode_kernel<true>, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles, counter,
+                           workspace);
+    else
+        hipLaunchKernelGGL(bc1_encode_kernel<false>, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles,
+                           counter, workspace);
     return hipGetLastError();
 }
 

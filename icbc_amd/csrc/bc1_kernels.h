@@ -51,8 +51,17 @@ int encode_workgroups_per_cu();
 // Per workgroup: two SAT slices (4- and 3-cluster fits) of 64 blocks x 17
 // rows, and the tile's reduced colours (64 x 16 float4).
 constexpr size_t kWorkspaceBytesPerWorkgroup = ((size_t)2 * kTile * kSatRows + (size_t)kTile * 16) * 16;
+// `refine` selects the kernel instance that holds the level-9 refinement
+// walk; it must be set when any job of the launch is at level 9.
 hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, uint32_t* counter,
-                         float4* workspace, uint32_t grid, hipStream_t stream);
+                         float4* workspace, uint32_t grid, bool refine, hipStream_t stream);
+
+// Whether a launch over these (host-side) jobs needs the refining instance.
+inline bool jobs_need_refine(const JobDev* jobs, size_t n) {
+    for (size_t k = 0; k < n; k++)
+        if (jobs[k].level == 9) return true;
+    return false;
+}
 
 // 16 x RGBA8 per block -> 16 x float4 per block (c / 255.0f).
 hipError_t launch_rgba8_to_float(const uint32_t* src, float4* dst, uint64_t blocks, hipStream_t stream);

@@ -534,7 +534,7 @@ int icbc_gpu_compress_bc1_batch(const icbc_bc1_job* jobs, size_t n_jobs, void* s
         r = workspace_alloc(grid, s, w);
         if (r) return r;
         HIP_TRY(launch_encode((const JobDev*)djobs, (const uint2*)dtiles, (uint32_t)tiles.size(), w.counter(),
-                              w.slices(), grid, s));
+                              w.slices(), grid, jobs_need_refine(p.dev.data(), p.dev.size()), s));
         HIP_TRY(hipFreeAsync(w.base, s));
         HIP_TRY(hipFreeAsync(djobs, s));
         HIP_TRY(hipFreeAsync(dtiles, s));
@@ -566,7 +566,7 @@ int icbc_gpu_compress_bc1_plain(const icbc_bc1_job* jobs, size_t n_jobs, void* s
         if (r) return r;
         for (size_t k = 0; k < n_jobs; k++)
             HIP_TRY(launch_encode((const JobDev*)djobs + k, nullptr, tiles_of(p.dev[k].block_count, p.dev[k].tile_blocks),
-                                  nullptr, w.slices(), grid, s));
+                                  nullptr, w.slices(), grid, jobs_need_refine(&p.dev[k], 1), s));
         HIP_TRY(hipFreeAsync(w.base, s));
         HIP_TRY(hipFreeAsync(djobs, s));
     }
@@ -591,6 +591,7 @@ struct icbc_bc1_plan {
     std::vector<Convert> converts;  // RGBA8 jobs, re-read on every execute
     uint32_t ntiles = 0;
     uint32_t grid = 0;
+    bool refine = false;  // some job is at level 9
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -601,7 +602,7 @@ int plan_launch(icbc_bc1_plan* pl, hipStream_t s) {
     for (const auto& c : pl->converts) HIP_TRY(launch_rgba8_to_float(c.src, c.dst, c.blocks, s));
     uint32_t* counter = pl->opt.schedule == ICBC_SCHEDULE_PERSISTENT ? (uint32_t*)pl->dwork : nullptr;
     float4* slices = (float4*)((char*)pl->dwork + 256);
-    HIP_TRY(launch_encode(pl->djobs, pl->dtiles, pl->ntiles, counter, slices, pl->grid, s));
+    HIP_TRY(launch_encode(pl->djobs, pl->dtiles, pl->ntiles, counter, slices, pl->grid, pl->refine, s));
     return ICBC_OK;
 }
 
@@ -635,6 +636,7 @@ int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_pla
     icbc_bc1_plan* pl = new icbc_bc1_plan();
     pl->opt = o;
     pl->ntiles = (uint32_t)tiles.size();
+    pl->refine = jobs_need_refine(dev.data(), dev.size());
     (void)hipGetDevice(&pl->device);
     hipDeviceProp_t prop;
     hipError_t e = hipGetDeviceProperties(&prop, pl->device);
@@ -788,7 +790,8 @@ int compress_one(int level, const float* input_colors, const float* input_weight
     static_assert(sizeof(JobDev) <= 128, "job descriptor must fit the block buffer");
     memcpy(h + 384, &jd, sizeof jd);
     HIP_TRY(hipMemcpyAsync(d, h, 512, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(launch_encode((const JobDev*)(d + 384), nullptr, 1, nullptr, (float4*)(d + 768), 1, ctx->stream));
+    HIP_TRY(launch_encode((const JobDev*)(d + 384), nullptr, 1, nullptr, (float4*)(d + 768), 1, level == 9,
+                          ctx->stream));
     HIP_TRY(hipMemcpyAsync(h + 320, d + 320, 32, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     memcpy(output, h + 320, 8);

@@ -251,7 +251,8 @@ int icbc_server::launch(Slot& s, std::vector<Request*>& reqs) {
     if (hipMemcpyAsync(s.table, s.host_table, need, hipMemcpyHostToDevice, s.stream) != hipSuccess)
         return ICBC_ERR_DEVICE;
     if (launch_encode((const JobDev*)s.table, (const uint2*)((char*)s.table + jbytes), (uint32_t)tiles.size(),
-                      (uint32_t*)s.work, (float4*)((char*)s.work + 256), grid, s.stream) != hipSuccess)
+                      (uint32_t*)s.work, (float4*)((char*)s.work + 256), grid, jobs_need_refine(dev.data(), dev.size()),
+                      s.stream) != hipSuccess)
         return ICBC_ERR_DEVICE;
     // results go to pinned staging (a copy into pageable memory would block
     // this thread until the kernel ends); the reaper hands them over
