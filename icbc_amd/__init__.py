@@ -116,6 +116,7 @@ _SIGNATURES = {
     "icbc_gpu_last_error": (C.c_char_p, []),
     "icbc_gpu_device_name": (C.c_int, [C.c_char_p, C.c_size_t]),
     "icbc_gpu_version": (C.c_int, []),
+    "icbc_gpu_check_reciprocal": (C.c_int, [C.POINTER(C.c_uint64)]),
     "icbc_gpu_malloc": (C.c_void_p, [C.c_size_t]),
     "icbc_gpu_free": (C.c_int, [C.c_void_p]),
     "icbc_gpu_host_alloc": (C.c_void_p, [C.c_size_t]),
@@ -320,6 +321,14 @@ def device_name():
     buf = C.create_string_buffer(256)
     check(lib().icbc_gpu_device_name(buf, 256))
     return buf.value.decode()
+
+
+def check_reciprocal():
+    """Count of float32 inputs where the encoder's fast reciprocal differs from
+    IEEE-754 division on the current device (0 is required for bit-exact codes)."""
+    n = C.c_uint64(0)
+    check(lib().icbc_gpu_check_reciprocal(C.byref(n)))
+    return n.value
 
 
 # ---------------------------------------------------------------------------

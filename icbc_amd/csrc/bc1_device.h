@@ -596,6 +596,21 @@ __device__ __forceinline__ void build_sat(f3 (&c)[16], float (&w)[16], int n, fl
 // Returns the candidate's error (without the constant sum(w x^2) term) and the
 // snapped endpoints.
 
+// Correctly rounded 1/b, the value of IEEE-754 division 1.0f / b, in three
+// instructions where b and 1/b are both normal (biased exponent 1..252): the
+// hardware reciprocal estimate (within 1 ulp) and one Newton step with fused
+// multiply-adds, e = 1 - b*y0 (exact) and y0 + e*y0.  Zero, denormal, huge,
+// infinite and NaN b take the division itself.  bc1_check_reciprocal_kernel
+// compares the two for all 2^32 bit patterns on the device.
+__device__ __forceinline__ float recip_rn(float b) {
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y0, 1.0f);
+    float y1 = __builtin_fmaf(e, y0, y0);
+    const uint32_t ex = (__float_as_uint(b) >> 23) & 0xffu;
+    if (ex - 1u >= 252u) y1 = 1.0f / b;
+    return y1;
+}
+
 __device__ __forceinline__ float candidate4(float4 s0, float4 s1, float4 s2, float4 tot, f3 metric,
                                             const DecoderWeights& dw, f3& a, f3& b) {
     f3 x0 = mk(s0.x, s0.y, s0.z), x1 = mk(s1.x, s1.y, s1.z), x2 = mk(s2.x, s2.y, s2.z);
@@ -609,7 +624,7 @@ __device__ __forceinline__ float candidate4(float4 s0, float4 s1, float4 s2, flo
     const float alpha2 = w2 * dw.w1_9 + (w1 * dw.w4_9 + w0);
     const float beta2 = w1 * dw.w1_9 + (w2 * dw.w4_9 + w3);
     const float ab = (w1 + w2) * dw.w2_9;
-    const float factor = 1.0f / (alpha2 * beta2 - ab * ab);
+    const float factor = recip_rn(alpha2 * beta2 - ab * ab);
 
     const f3 ax = x2 * dw.w1_3 + (x1 * dw.w2_3 + x0);
     const f3 bx = mk(tot.x, tot.y, tot.z) - ax;
@@ -632,7 +647,7 @@ __device__ __forceinline__ float candidate3(float4 s0, float4 s1, float4 tot, f3
     const float ab = w1 * 0.25f;
     const float alpha2 = w0 + ab;
     const float beta2 = w2 + ab;
-    const float factor = 1.0f / (alpha2 * beta2 - ab * ab);
+    const float factor = recip_rn(alpha2 * beta2 - ab * ab);
 
     const f3 ax = x0 + x1 * 0.5f;
     const f3 bx = mk(tot.x, tot.y, tot.z) - ax;

@@ -133,8 +133,11 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offs
         int besti = kNoWinner;
         if (kClusters == 4) {
             const int total = (nb + 3) * (nb + 2) * (nb + 1) / 6 - 1;  // k_four_total[nb - 1]
+            // the next candidate's table entry is read while this one is evaluated
+            uint32_t pk_next = sh.four[lane];
             for (int i = lane; i < total; i += kTile) {
-                const uint32_t pk = sh.four[i];
+                const uint32_t pk = pk_next;
+                pk_next = sh.four[min(i + kTile, 975)];
                 f3 a, e;
                 const float err = candidate4(sat[pk & 0xff], sat[(pk >> 8) & 0xff], sat[pk >> 16], tot, metric,
                                              dw, a, e);
@@ -145,8 +148,10 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offs
             }
         } else {
             const int total = (nb + 2) * (nb + 1) / 2 - 1;  // k_three_total[nb - 1]
+            uint32_t pk_next = sh.three[lane];
             for (int i = lane; i < total; i += kTile) {
-                const uint32_t pk = sh.three[i];
+                const uint32_t pk = pk_next;
+                pk_next = sh.three[min(i + kTile, 159)];
                 f3 a, e;
                 const float err = candidate3(sat[pk & 0xff], sat[(pk >> 8) & 0xff], tot, metric, a, e);
                 if (err < best) {
@@ -350,6 +355,20 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) v
 
 __global__ void reset_counter_kernel(uint32_t* counter) { *counter = 0u; }
 
+// Counts the float bit patterns where recip_rn differs from 1.0f / b (NaN
+// results compare equal to each other, zeros by sign).
+__global__ void bc1_check_reciprocal_kernel(unsigned long long* __restrict__ mismatches) {
+    unsigned long long bad = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (1ull << 32); i += stride) {
+        const float b = __uint_as_float((uint32_t)i);
+        const float fast = recip_rn(b), ieee = 1.0f / b;
+        const bool same = (fast != fast && ieee != ieee) || __float_as_uint(fast) == __float_as_uint(ieee);
+        bad += same ? 0 : 1;
+    }
+    if (bad) atomicAdd(mismatches, bad);
+}
+
 // RGBA8 texels -> float RGBA, each channel c / 255.0f as the reference's
 // harness converts 8-bit images (icbc_test.cpp:401-405); alpha is carried
 // along although the encoder never reads it.
@@ -447,6 +466,13 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
     else
         hipLaunchKernelGGL(bc1_encode_kernel<false>, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles,
                            counter, workspace);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_reciprocal(unsigned long long* mismatches, hipStream_t stream) {
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, (uint32_t*)mismatches);
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, (uint32_t*)mismatches + 1);
+    hipLaunchKernelGGL(bc1_check_reciprocal_kernel, dim3(4096), dim3(256), 0, stream, mismatches);
     return hipGetLastError();
 }
 

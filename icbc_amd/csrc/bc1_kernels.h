@@ -63,6 +63,10 @@ inline bool jobs_need_refine(const JobDev* jobs, size_t n) {
     return false;
 }
 
+// Counts (into *mismatches, zeroed first) the 2^32 float inputs where the
+// encoder's fast reciprocal differs from IEEE-754 division on this device.
+hipError_t launch_check_reciprocal(unsigned long long* mismatches, hipStream_t stream);
+
 // 16 x RGBA8 per block -> 16 x float4 per block (c / 255.0f).
 hipError_t launch_rgba8_to_float(const uint32_t* src, float4* dst, uint64_t blocks, hipStream_t stream);
 

@@ -857,6 +857,22 @@ int icbc_gpu_device_name(char* buf, size_t len) {
 
 int icbc_gpu_version(void) { return 10500; }
 
+int icbc_gpu_check_reciprocal(uint64_t* mismatches) {
+    if (!mismatches) return fail(ICBC_ERR_INVALID_ARGUMENT, "mismatches is null");
+    int decoder;
+    int r = ensure_device_ready(&decoder);
+    if (r) return r;
+    unsigned long long* d = nullptr;
+    HIP_TRY(hipMalloc((void**)&d, sizeof(unsigned long long)));
+    unsigned long long h = 0;
+    hipError_t e = launch_check_reciprocal(d, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof h, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "reciprocal check");
+    *mismatches = (uint64_t)h;
+    return ICBC_OK;
+}
+
 void* icbc_gpu_malloc(size_t bytes) {
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, bytes ? bytes : 1);

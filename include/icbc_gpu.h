@@ -156,6 +156,10 @@ int icbc_gpu_evaluate_bc1_error_batch(const uint8_t* rgba, const void* codes,
 const char* icbc_gpu_last_error(void); /* text of this thread's last failure */
 int icbc_gpu_device_name(char* buf, size_t len);
 int icbc_gpu_version(void); /* 10500 + build revision, mirrors icbc v1.05 */
+/* Self-check of this device: counts the 2^32 float32 inputs where the
+ * encoder's three-instruction reciprocal differs from IEEE-754 division
+ * (no counterpart in the reference; bit-exact encoding needs 0). */
+int icbc_gpu_check_reciprocal(uint64_t* mismatches);
 
 /* ---- device plumbing for callers without their own HIP runtime ---------- */
 void* icbc_gpu_malloc(size_t bytes);

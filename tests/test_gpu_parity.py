@@ -271,3 +271,9 @@ def test_full_size_image_level8_properties(gpu, oracle):
     flat = cat == synth.CATEGORIES.index("flat")
     assert np.all(stats[flat, 0] == 1)          # flat blocks reduce to one colour
     assert np.all((stats[:, 0] >= 1) & (stats[:, 0] <= 16))
+
+
+def test_fast_reciprocal_matches_ieee_division_for_every_float(gpu):
+    # the cluster fits take 1/det from the hardware estimate plus one Newton
+    # step; all 2^32 inputs must give the IEEE-754 quotient bit for bit
+    assert gpu.check_reciprocal() == 0
