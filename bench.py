@@ -58,6 +58,25 @@ def algorithmic_flops(stats):
     return int((FLOPS_FIT4 * fit4 + FLOPS_FIT3 * fit3 + FLOPS_REFINE_STEP * steps).sum())
 
 
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "bc1_encode_kernel_texture_batch_L8_traffic.md")
+
+
+def committed_traffic(blocks, level, mixed):
+    """HBM bytes per launch of the encode kernel from the committed FETCH_SIZE /
+    WRITE_SIZE counter summary (copy-calibrated), when it covers this workload."""
+    if mixed or level != 8 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    vals = {}
+    for line in open(TRAFFIC_PROFILE):
+        parts = line.strip().split(":")
+        if len(parts) == 2 and parts[0].startswith("traffic_"):
+            vals[parts[0]] = int(parts[1])
+    if vals.get("traffic_blocks") != blocks:
+        return None
+    return vals.get("traffic_bytes_per_launch")
+
+
 def cpu_features():
     try:
         flags = open("/proc/cpuinfo").read()
@@ -330,7 +349,11 @@ def main():
     roofline = {"bound": "valu-fp32", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                 "frac_of_unfused_peak": round(achieved / PEAK_FP32_NOFMA_TFLOPS, 4),
-                "traffic": None, "kernel": "bc1_encode_kernel",
+                "traffic": committed_traffic(total, args.level, args.mixed_levels),
+                "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated: "
+                                  "profiles/bc1_encode_kernel_texture_batch_L8_traffic.md",
+                "algorithmic_bytes_per_launch": total * (256 + 8 + 4),
+                "kernel": "bc1_encode_kernel",
                 "algorithmic_gflop_per_launch": round(flops / 1e9, 3),
                 "mean_fit4_candidates_per_block": round(float(np.mean(
                     np.where(stats[:, 0] >= 2, (stats[:, 0] + 3) * (stats[:, 0] + 2) * (stats[:, 0] + 1) // 6 - 1, 0))), 2)}
