@@ -59,6 +59,12 @@ ERR_TIMEOUT = -8
 
 JOB_HOST_MEMORY = 1
 JOB_COLORS_RGBA8 = 2
+JOB_IMAGE = 4
+JOB_WEIGHTS_FROM_ALPHA = 8
+IMAGE_RGBA8 = 0
+IMAGE_RGBA32F = 1
+EDGE_REPLICATE = 0
+EDGE_ROUND_DOWN = 1
 SCHEDULE_GRID = 0
 SCHEDULE_PERSISTENT = 1
 
@@ -69,13 +75,20 @@ class IcbcError(RuntimeError):
         self.code = code
 
 
+class Image(C.Structure):
+    """icbc_image (include/icbc_gpu.h): an RGBA8 or RGBA32F image in device memory."""
+    _fields_ = [("data", C.c_void_p), ("format", C.c_uint32), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("edge", C.c_uint32), ("pitch", C.c_uint64)]
+
+
 class Job(C.Structure):
     """icbc_bc1_job (include/icbc_gpu.h)."""
     _fields_ = [("level", C.c_int32), ("three_color_mode", C.c_int32),
                 ("three_color_black", C.c_int32), ("flags", C.c_uint32),
                 ("color_weights", C.c_float * 3), ("tile_blocks", C.c_uint32),
                 ("block_count", C.c_uint64), ("colors", C.c_void_p), ("weights", C.c_void_p),
-                ("codes", C.c_void_p), ("errors", C.c_void_p), ("stats", C.c_void_p)]
+                ("codes", C.c_void_p), ("errors", C.c_void_p), ("stats", C.c_void_p),
+                ("image", C.POINTER(Image))]
 
 
 class ServerOptions(C.Structure):
@@ -117,6 +130,11 @@ _SIGNATURES = {
     "icbc_gpu_device_name": (C.c_int, [C.c_char_p, C.c_size_t]),
     "icbc_gpu_version": (C.c_int, []),
     "icbc_gpu_check_reciprocal": (C.c_int, [C.POINTER(C.c_uint64)]),
+    "icbc_gpu_image_block_count": (C.c_uint64, [C.POINTER(Image)]),
+    "icbc_gpu_image_to_blocks": (C.c_int, [C.POINTER(Image), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "icbc_gpu_mip_chain_bytes": (C.c_uint64, [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "icbc_gpu_build_mip_chain": (C.c_int, [C.POINTER(Image), C.c_void_p, C.c_uint64, C.POINTER(Image),
+                                           C.c_uint32, C.POINTER(C.c_uint32), C.c_void_p]),
     "icbc_gpu_malloc": (C.c_void_p, [C.c_size_t]),
     "icbc_gpu_free": (C.c_int, [C.c_void_p]),
     "icbc_gpu_host_alloc": (C.c_void_p, [C.c_size_t]),
@@ -337,8 +355,9 @@ def check_reciprocal():
 
 def make_job(level, block_count, colors, codes, weights=None, errors=None, stats=None,
              color_weights=(1.0, 1.0, 1.0), three_color_mode=False, three_color_black=False,
-             host=False, tile_blocks=0, rgba8=False):
-    """Builds an icbc_bc1_job; pointer arguments are ints (device) or numpy arrays (host)."""
+             host=False, tile_blocks=0, rgba8=False, image=None, weights_from_alpha=False):
+    """Builds an icbc_bc1_job; pointer arguments are ints (device) or numpy arrays (host).
+    With ``image`` (an Image), texels are gathered from it and ``colors`` must be None."""
     def addr(x):
         if x is None:
             return None
@@ -351,7 +370,8 @@ def make_job(level, block_count, colors, codes, weights=None, errors=None, stats
     j.level = int(level)
     j.three_color_mode = int(bool(three_color_mode))
     j.three_color_black = int(bool(three_color_black))
-    j.flags = (JOB_HOST_MEMORY if host else 0) | (JOB_COLORS_RGBA8 if rgba8 else 0)
+    j.flags = ((JOB_HOST_MEMORY if host else 0) | (JOB_COLORS_RGBA8 if rgba8 else 0)
+               | (JOB_IMAGE if image is not None else 0) | (JOB_WEIGHTS_FROM_ALPHA if weights_from_alpha else 0))
     for k in range(3):
         j.color_weights[k] = float(color_weights[k])
     j.tile_blocks = int(tile_blocks)
@@ -361,7 +381,95 @@ def make_job(level, block_count, colors, codes, weights=None, errors=None, stats
     j.codes = addr(codes)
     j.errors = addr(errors)
     j.stats = addr(stats)
+    if image is not None:
+        j.image = C.pointer(image)
+        j._image_ref = image  # the job points into it
     return j
+
+
+# ---------------------------------------------------------------------------
+# texture pre-pass
+
+class DeviceImage:
+    """A (h, w, 4) uint8 or float32 image uploaded to device memory, with its
+    icbc_image descriptor in ``.desc``."""
+
+    def __init__(self, img, edge=EDGE_REPLICATE, pitch=None):
+        img = np.ascontiguousarray(img)
+        if img.ndim != 3 or img.shape[2] != 4 or img.dtype not in (np.uint8, np.float32):
+            raise ValueError("image must be (h, w, 4) uint8 or float32")
+        h, w, _ = img.shape
+        tb = img.dtype.itemsize * 4
+        self.pitch = int(pitch) if pitch else w * tb
+        self.buf = DeviceBuffer(self.pitch * h)
+        if self.pitch == w * tb:
+            self.buf.upload(img)
+        else:
+            for y in range(h):
+                self.buf.upload(img[y], offset=y * self.pitch)
+        self.desc = Image(self.buf.ptr, IMAGE_RGBA8 if img.dtype == np.uint8 else IMAGE_RGBA32F,
+                          w, h, int(edge), self.pitch)
+
+    @property
+    def block_count(self):
+        return image_block_count(self.desc)
+
+    def close(self):
+        self.buf.close()
+
+
+def image_block_count(desc):
+    return int(lib().icbc_gpu_image_block_count(C.byref(desc)))
+
+
+def image_to_blocks(desc, alpha_weights=False, stream=None):
+    """Device gather of an Image into (n, 16, 4) float32 blocks (and (n, 16)
+    alpha weights); returns numpy arrays."""
+    n = image_block_count(desc)
+    colors = DeviceBuffer(n * 256)
+    weights = DeviceBuffer(n * 64) if alpha_weights else None
+    try:
+        check(lib().icbc_gpu_image_to_blocks(C.byref(desc), colors.ptr, weights.ptr if weights else None,
+                                             stream.handle if stream else None))
+        if stream:
+            stream.synchronize()
+        out = colors.download(np.float32, (n, 16, 4))
+        w = weights.download(np.float32, (n, 16)) if weights else None
+    finally:
+        colors.close()
+        if weights:
+            weights.close()
+    return (out, w) if alpha_weights else out
+
+
+def mip_chain_bytes(width, height):
+    """(bytes, levels) of the float RGBA chain below a width x height image."""
+    n = C.c_uint32(0)
+    b = lib().icbc_gpu_mip_chain_bytes(int(width), int(height), C.byref(n))
+    return int(b), n.value
+
+
+class MipChain:
+    """Every box-filtered level below ``source`` (an Image) built on the device:
+    ``.levels`` holds their Image descriptors (RGBA32F, tightly packed)."""
+
+    def __init__(self, source, stream=None):
+        nbytes, n = mip_chain_bytes(source.width, source.height)
+        self.buf = DeviceBuffer(max(nbytes, 16))
+        descs = (Image * max(n, 1))()
+        got = C.c_uint32(0)
+        check(lib().icbc_gpu_build_mip_chain(C.byref(source), self.buf.ptr, nbytes, descs, n, C.byref(got),
+                                             stream.handle if stream else None))
+        if stream:
+            stream.synchronize()
+        self.levels = [descs[k] for k in range(got.value)]
+
+    def download(self, k):
+        d = self.levels[k]
+        return self.buf.download(np.float32, (d.height, d.width, 4), offset=d.data - self.buf.ptr)
+
+    def close(self):
+        self.buf.close()
 
 
 def _job_array(jobs):

@@ -26,6 +26,7 @@
 #include "../../include/icbc_gpu.h"
 #include "bc1_kernels.h"
 #include "icbc_internal.h"
+#include "texture_prepass.h"
 
 using namespace icbc_gpu;
 
@@ -218,14 +219,54 @@ int workspace_alloc(uint32_t grid, hipStream_t s, Workspace& w) {
 // ---------------------------------------------------------------------------
 // jobs
 
+// Checks an image descriptor and converts it for the kernels.
+int image_desc(const icbc_image* im, ImageDev& out, const char* what) {
+    if (!im) return fail(ICBC_ERR_INVALID_ARGUMENT, "%s: null image", what);
+    if (im->format != ICBC_IMAGE_RGBA8 && im->format != ICBC_IMAGE_RGBA32F)
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "%s: unknown image format %u", what, im->format);
+    if (im->edge != ICBC_EDGE_REPLICATE && im->edge != ICBC_EDGE_ROUND_DOWN)
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "%s: unknown edge mode %u", what, im->edge);
+    if (!im->data || im->width == 0 || im->height == 0)
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "%s: empty image", what);
+    const uint64_t tb = im->format == ICBC_IMAGE_RGBA8 ? 4 : 16;
+    if (im->pitch < (uint64_t)im->width * tb || im->pitch % tb != 0 || ((uintptr_t)im->data % tb) != 0)
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "%s: pitch must cover a row and be a multiple of %u bytes, "
+                    "data aligned to it", what, (unsigned)tb);
+    out.data = (const uint8_t*)im->data;
+    out.pitch = im->pitch;
+    out.format = im->format;
+    out.width = im->width;
+    out.height = im->height;
+    out.edge = im->edge;
+    return ICBC_OK;
+}
+
 int validate(const icbc_bc1_job& j, size_t k) {
-    if ((j.flags & ~(ICBC_JOB_HOST_MEMORY | ICBC_JOB_COLORS_RGBA8)) != 0)
+    if ((j.flags & ~(ICBC_JOB_HOST_MEMORY | ICBC_JOB_COLORS_RGBA8 | ICBC_JOB_IMAGE | ICBC_JOB_WEIGHTS_FROM_ALPHA)) != 0)
         return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: unknown flags", k);
+    if ((j.flags & ICBC_JOB_WEIGHTS_FROM_ALPHA) && (!(j.flags & ICBC_JOB_IMAGE) || j.weights))
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: weights from alpha need an image source and null weights", k);
+    if (j.flags & ICBC_JOB_IMAGE) {
+        if (j.flags & (ICBC_JOB_HOST_MEMORY | ICBC_JOB_COLORS_RGBA8))
+            return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: image sources are device images, not host or RGBA8 blocks", k);
+        if (j.colors) return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: image job with colors set", k);
+        ImageDev im;
+        char what[32];
+        snprintf(what, sizeof what, "job %zu", k);
+        int r = image_desc(j.image, im, what);
+        if (r) return r;
+        if (image_block_count(im) != j.block_count)
+            return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: block_count %llu, image has %llu blocks", k,
+                        (unsigned long long)j.block_count, (unsigned long long)image_block_count(im));
+    } else if (j.image) {
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: image set without ICBC_JOB_IMAGE", k);
+    }
     if (j.tile_blocks != 0 && j.tile_blocks != 8 && j.tile_blocks != 16 && j.tile_blocks != 32 &&
         j.tile_blocks != 64)
         return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: tile_blocks must be 0, 8, 16, 32 or 64", k);
     if (j.block_count == 0) return ICBC_OK;
-    if (!j.colors || !j.codes) return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: null colors or codes", k);
+    if ((!j.colors && !(j.flags & ICBC_JOB_IMAGE)) || !j.codes)
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: null colors or codes", k);
     if (!(j.flags & ICBC_JOB_HOST_MEMORY)) {
         if (((uintptr_t)j.colors & 15) || ((uintptr_t)j.weights & 15))
             return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: colors/weights must be 16-byte aligned", k);
@@ -347,26 +388,70 @@ int stage_out(const icbc_bc1_job& j, hipStream_t s, Staged& st) {
     return ICBC_OK;
 }
 
+// A job whose texels are not already float blocks: RGBA8 blocks or an image,
+// converted into the encoder's layout before the encode launch.
+struct SourceConvert {
+    bool image = false;
+    const uint32_t* rgba8 = nullptr;  // RGBA8 blocks
+    ImageDev img{};                   // image source
+    float4* colors = nullptr;         // float blocks written here
+    float* weights = nullptr;         // alpha as weight (image jobs with ICBC_JOB_WEIGHTS_FROM_ALPHA)
+    uint64_t blocks = 0;
+};
+
+bool needs_convert(const icbc_bc1_job& j) {
+    return j.block_count && (j.flags & (ICBC_JOB_COLORS_RGBA8 | ICBC_JOB_IMAGE));
+}
+
+// Bytes of float blocks (and weights) a converted job needs.
+size_t convert_bytes(const icbc_bc1_job& j) {
+    return (size_t)j.block_count * ((j.flags & ICBC_JOB_WEIGHTS_FROM_ALPHA) ? 256 + 64 : 256);
+}
+
+// Describes the conversion of job `j` into `buf` (convert_bytes(j) bytes) and
+// points the device job at the converted texels.
+SourceConvert describe_convert(const icbc_bc1_job& j, void* buf, JobDev& d) {
+    SourceConvert c;
+    c.blocks = j.block_count;
+    c.colors = (float4*)buf;
+    if (j.flags & ICBC_JOB_IMAGE) {
+        c.image = true;
+        (void)image_desc(j.image, c.img, "job");  // validated before
+        if (j.flags & ICBC_JOB_WEIGHTS_FROM_ALPHA) {
+            c.weights = (float*)((char*)buf + (size_t)j.block_count * 256);
+            d.weights = (const float4*)c.weights;
+        }
+    } else {
+        c.rgba8 = (const uint32_t*)j.colors;
+    }
+    d.colors = c.colors;
+    return c;
+}
+
+hipError_t run_convert(const SourceConvert& c, hipStream_t s) {
+    if (c.image) return launch_image_gather(c.img, c.colors, c.weights, s);
+    return launch_rgba8_to_float(c.rgba8, c.colors, c.blocks, s);
+}
+
 // Shared front half of batch/plain: validate, stage host jobs, build JobDev list.
 struct Prepared {
     std::vector<icbc_bc1_job> jobs;  // device-pointer view
     std::vector<Staged> staged;      // per job (empty for device jobs)
     std::vector<JobDev> dev;
-    std::vector<void*> temps;        // float copies of RGBA8 inputs, freed after the launch
+    std::vector<void*> temps;        // float copies of RGBA8 / image inputs, freed after the launch
     bool any_host = false;
 };
 
-// RGBA8 jobs: convert the texels to float RGBA in stream-ordered scratch and
-// point the device job at it.
-int convert_rgba8(Prepared& p, hipStream_t s) {
+// RGBA8 and image jobs: convert the texels to float RGBA blocks in
+// stream-ordered scratch and point the device job at them.
+int convert_sources(Prepared& p, hipStream_t s) {
     for (size_t k = 0; k < p.jobs.size(); k++) {
         const icbc_bc1_job& j = p.jobs[k];
-        if (!(j.flags & ICBC_JOB_COLORS_RGBA8) || j.block_count == 0) continue;
+        if (!needs_convert(j)) continue;
         void* f = nullptr;
-        HIP_TRY(hipMallocAsync(&f, j.block_count * 256, s));
+        HIP_TRY(hipMallocAsync(&f, convert_bytes(j), s));
         p.temps.push_back(f);
-        HIP_TRY(launch_rgba8_to_float((const uint32_t*)j.colors, (float4*)f, j.block_count, s));
-        p.dev[k].colors = (const float4*)f;
+        HIP_TRY(run_convert(describe_convert(j, f, p.dev[k]), s));
     }
     return ICBC_OK;
 }
@@ -396,7 +481,7 @@ int prepare(const icbc_bc1_job* jobs, size_t n_jobs, hipStream_t s, int decoder,
         }
         p.dev[k] = to_dev(p.jobs[k], decoder);
     }
-    return convert_rgba8(p, s);
+    return convert_sources(p, s);
 }
 
 int finish(const icbc_bc1_job* jobs, size_t n_jobs, hipStream_t s, Prepared& p) {
@@ -583,12 +668,8 @@ struct icbc_bc1_plan {
     JobDev* djobs = nullptr;
     uint2* dtiles = nullptr;
     void* dwork = nullptr;  // tile-queue counter (256 B) + per-workgroup slices
-    struct Convert {
-        const uint32_t* src;
-        float4* dst;
-        uint64_t blocks;
-    };
-    std::vector<Convert> converts;  // RGBA8 jobs, re-read on every execute
+    std::vector<SourceConvert> converts;  // RGBA8 / image jobs, re-read on every execute
+    std::vector<void*> convert_bufs;
     uint32_t ntiles = 0;
     uint32_t grid = 0;
     bool refine = false;  // some job is at level 9
@@ -599,7 +680,7 @@ struct icbc_bc1_plan {
 namespace {
 
 int plan_launch(icbc_bc1_plan* pl, hipStream_t s) {
-    for (const auto& c : pl->converts) HIP_TRY(launch_rgba8_to_float(c.src, c.dst, c.blocks, s));
+    for (const auto& c : pl->converts) HIP_TRY(run_convert(c, s));
     uint32_t* counter = pl->opt.schedule == ICBC_SCHEDULE_PERSISTENT ? (uint32_t*)pl->dwork : nullptr;
     float4* slices = (float4*)((char*)pl->dwork + 256);
     HIP_TRY(launch_encode(pl->djobs, pl->dtiles, pl->ntiles, counter, slices, pl->grid, pl->refine, s));
@@ -662,11 +743,11 @@ int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_pla
         (e = hipMalloc(&pl->dwork, 256 + (size_t)pl->grid * kWorkspaceBytesPerWorkgroup)) != hipSuccess)
         return cleanup(hip_fail(e, "hipMalloc (plan)"));
     for (size_t k = 0; k < n_jobs; k++) {
-        if (!(jobs[k].flags & ICBC_JOB_COLORS_RGBA8) || jobs[k].block_count == 0) continue;
+        if (!needs_convert(jobs[k])) continue;
         void* f = nullptr;
-        if ((e = hipMalloc(&f, jobs[k].block_count * 256)) != hipSuccess) return cleanup(hip_fail(e, "hipMalloc (plan)"));
-        pl->converts.push_back({(const uint32_t*)jobs[k].colors, (float4*)f, jobs[k].block_count});
-        dev[k].colors = (const float4*)f;
+        if ((e = hipMalloc(&f, convert_bytes(jobs[k]))) != hipSuccess) return cleanup(hip_fail(e, "hipMalloc (plan)"));
+        pl->convert_bufs.push_back(f);
+        pl->converts.push_back(describe_convert(jobs[k], f, dev[k]));
     }
     if (!dev.empty() &&
         (e = hipMemcpy(pl->djobs, dev.data(), dev.size() * sizeof(JobDev), hipMemcpyHostToDevice)) != hipSuccess)
@@ -717,7 +798,7 @@ int icbc_gpu_plan_destroy(icbc_bc1_plan* pl) {
     if (pl->djobs) (void)hipFree(pl->djobs);
     if (pl->dtiles) (void)hipFree(pl->dtiles);
     if (pl->dwork) (void)hipFree(pl->dwork);
-    for (const auto& c : pl->converts) (void)hipFree(c.dst);
+    for (void* b : pl->convert_bufs) (void)hipFree(b);
     delete pl;
     return ICBC_OK;
 }
@@ -814,6 +895,75 @@ float icbc_gpu_compress_bc1(int level, const float* input_colors, const float* i
 }
 
 // ---- decode / evaluate --------------------------------------------------------
+
+// ---- texture pre-pass -------------------------------------------------------------
+
+uint64_t icbc_gpu_image_block_count(const icbc_image* image) {
+    ImageDev im;
+    if (image_desc(image, im, "image")) return 0;
+    return image_block_count(im);
+}
+
+int icbc_gpu_image_to_blocks(const icbc_image* image, float* colors, float* weights, void* stream) {
+    ImageDev im;
+    int r = image_desc(image, im, "image");
+    if (r) return r;
+    if (!colors || ((uintptr_t)colors & 15) || ((uintptr_t)weights & 3))
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "colors must be 16-byte aligned device memory");
+    int decoder;
+    r = ensure_device_ready(&decoder);
+    if (r) return r;
+    HIP_TRY(launch_image_gather(im, (float4*)colors, weights, (hipStream_t)stream));
+    return ICBC_OK;
+}
+
+uint64_t icbc_gpu_mip_chain_bytes(uint32_t width, uint32_t height, uint32_t* levels) {
+    uint64_t bytes = 0;
+    uint32_t n = 0;
+    uint32_t w = width, h = height;
+    while (w > 4 || h > 4) {  // down to the first level that fits one 4x4 block
+        downsampled_size(w, h, w, h);
+        bytes += (uint64_t)w * h * 16;
+        n++;
+    }
+    if (levels) *levels = n;
+    return bytes;
+}
+
+int icbc_gpu_build_mip_chain(const icbc_image* source, float* dst, uint64_t dst_bytes, icbc_image* levels,
+                             uint32_t max_levels, uint32_t* n_levels, void* stream) {
+    ImageDev src;
+    int r = image_desc(source, src, "source");
+    if (r) return r;
+    uint32_t n = 0;
+    const uint64_t need = icbc_gpu_mip_chain_bytes(src.width, src.height, &n);
+    if (need && (!dst || dst_bytes < need || ((uintptr_t)dst & 15)))
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "mip chain needs %llu bytes of 16-byte aligned device memory",
+                    (unsigned long long)need);
+    if (n > max_levels || (n && !levels))
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "mip chain has %u levels, room for %u", n, max_levels);
+    int decoder;
+    r = ensure_device_ready(&decoder);
+    if (r) return r;
+    ImageDev cur = src;
+    char* out = (char*)dst;
+    for (uint32_t k = 0; k < n; k++) {
+        uint32_t w, h;
+        downsampled_size(cur.width, cur.height, w, h);
+        HIP_TRY(launch_downsample(cur, (float4*)out, (hipStream_t)stream));
+        icbc_image& lv = levels[k];
+        lv.data = out;
+        lv.format = ICBC_IMAGE_RGBA32F;
+        lv.width = w;
+        lv.height = h;
+        lv.edge = source->edge;
+        lv.pitch = (uint64_t)w * 16;
+        cur = ImageDev{(const uint8_t*)out, lv.pitch, ICBC_IMAGE_RGBA32F, w, h, source->edge};
+        out += (uint64_t)w * h * 16;
+    }
+    if (n_levels) *n_levels = n;
+    return ICBC_OK;
+}
 
 int icbc_gpu_decode_bc1_batch(const void* codes, uint64_t block_count, uint8_t* rgba, int decoder,
                               void* stream) {

@@ -40,6 +40,29 @@ extern "C" {
 #define ICBC_JOB_COLORS_RGBA8 2u /* colors are 16 x RGBA8 (64 bytes) per block, each channel
                                     read as c / 255.0f like the reference's harness
                                     (icbc_test.cpp:401-405) */
+#define ICBC_JOB_IMAGE 4u /* texels come from `image` (device memory), gathered into 4x4
+                               blocks on the device; `colors` must be NULL and block_count
+                               equal icbc_gpu_image_block_count(image) */
+#define ICBC_JOB_WEIGHTS_FROM_ALPHA 8u /* with ICBC_JOB_IMAGE: per-texel weight = the texel's
+                                          alpha (`weights` must be NULL) */
+
+/* ---- texture pre-pass: images on the device ------------------------------ */
+#define ICBC_IMAGE_RGBA8 0u   /* 4 x uint8 per texel, read as c / 255.0f (icbc_test.cpp:401-405) */
+#define ICBC_IMAGE_RGBA32F 1u /* 4 x float per texel */
+#define ICBC_EDGE_REPLICATE 0u  /* ceil(w/4) x ceil(h/4) blocks; the last row and column are
+                                   repeated into the partial blocks */
+#define ICBC_EDGE_ROUND_DOWN 1u /* floor(w/4) x floor(h/4) blocks, partial blocks dropped, as the
+                                   reference's harness does (icbc_test.cpp:354-380) */
+
+typedef struct icbc_image {
+    const void* data; /* device memory, first texel of the top row */
+    uint32_t format;  /* ICBC_IMAGE_* */
+    uint32_t width;
+    uint32_t height;
+    uint32_t edge;    /* ICBC_EDGE_* */
+    uint64_t pitch;   /* bytes from one row to the next: >= width * texel size, a multiple of
+                         the texel size (RGBA8: 4 B, RGBA32F: 16 B, which is also the alignment) */
+} icbc_image;
 
 typedef struct icbc_bc1_job {
     int32_t level;             /* icbc::Quality 0..9; others give error FLT_MAX, codes untouched */
@@ -58,6 +81,8 @@ typedef struct icbc_bc1_job {
     int32_t* stats;            /* optional, 4 int32 per block: reduced colour count n,
                                   colours given to the 3-cluster fit (-1 = not run),
                                   refinement steps, 0 */
+    const icbc_image* image;   /* texel source with ICBC_JOB_IMAGE, else NULL; read at the call
+                                  (and at every execute of a plan holding the job) */
 } icbc_bc1_job;
 
 /* Replaces icbc::init (icbc.h:19, 3813): selects the hardware decoder whose
@@ -151,6 +176,33 @@ int icbc_gpu_decode_bc1_batch(const void* codes, uint64_t block_count, uint8_t* 
 int icbc_gpu_evaluate_bc1_error_batch(const uint8_t* rgba, const void* codes,
                                       uint64_t block_count, float* errors, int decoder,
                                       void* stream);
+
+/* ---- texture pre-pass (device) --------------------------------------------
+ * The reference's harness turns an RGBA8 image into blocks on the CPU
+ * (icbc_test.cpp:353-380: 4x4 gather, texel j = 4*y + x, c / 255.0f at
+ * 401-405).  These run that step, and box-filtered mip chains, on the device.
+ * All pointers are device memory; work is enqueued on `stream`. */
+
+/* Blocks of an image under its edge mode (0 for an invalid descriptor). */
+uint64_t icbc_gpu_image_block_count(const icbc_image* image);
+
+/* Gathers the image into the encoder's block layout: 64 floats per block
+ * (RGBA, alpha carried through) and, when `weights` is not NULL, 16 floats per
+ * block holding each texel's alpha. */
+int icbc_gpu_image_to_blocks(const icbc_image* image, float* colors, float* weights, void* stream);
+
+/* Bytes of float RGBA needed for every level below `width` x `height` down to
+ * the first that fits one 4x4 block, and that level count. */
+uint64_t icbc_gpu_mip_chain_bytes(uint32_t width, uint32_t height, uint32_t* levels);
+
+/* Builds those levels from `source` into `dst` (tightly packed, level after
+ * level, RGBA32F), each a 2x2 box filter of the one above: ((a + b) + (c + d))
+ * * 0.25 per channel in float, an odd last row or column dropped, a 1-texel
+ * dimension filtered along the other only ((a + b) * 0.5).  `levels`
+ * receives up to `max_levels` descriptors (edge mode copied from `source`),
+ * `*n_levels` the count. */
+int icbc_gpu_build_mip_chain(const icbc_image* source, float* dst, uint64_t dst_bytes, icbc_image* levels,
+                             uint32_t max_levels, uint32_t* n_levels, void* stream);
 
 /* ---- diagnostics -------------------------------------------------------- */
 const char* icbc_gpu_last_error(void); /* text of this thread's last failure */

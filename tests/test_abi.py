@@ -62,10 +62,13 @@ def test_invalid_arguments_are_rejected_without_device():
 
 
 def test_job_struct_layout_matches_header():
-    # icbc_bc1_job: 4 int32, 3 float, 1 uint32, uint64, 5 pointers
-    assert ctypes.sizeof(icbc_amd.Job) == 16 + 16 + 8 + 5 * 8
+    # icbc_bc1_job: 4 int32, 3 float, 1 uint32, uint64, 6 pointers
+    assert ctypes.sizeof(icbc_amd.Job) == 16 + 16 + 8 + 6 * 8
     assert icbc_amd.Job.block_count.offset == 32
     assert icbc_amd.Job.colors.offset == 40
+    assert icbc_amd.Job.image.offset == 80
+    # icbc_image: pointer, 4 uint32, uint64
+    assert ctypes.sizeof(icbc_amd.Image) == 8 + 16 + 8
     assert ctypes.sizeof(icbc_amd.PlanOptions) == 16
 
 
