@@ -243,6 +243,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the short serving run included in the default benchmark")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the per-level autotuning pass before the batched plan is built")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM-counter calibration "
                          "under rocprofv3 --pmc)")
@@ -307,8 +309,25 @@ def main():
         src.close()
         dst.close()
         log(f"calibration copy: {nb} bytes device to device")
-    plan = g.Plan(jobs(), schedule=g.SCHEDULE_PERSISTENT, use_graph=True, order_by_cost=True,
-                  stream=stream)
+    tuning = None
+    if not args.no_autotune:
+        # untimed, like building the plan: tune the workload's levels on its
+        # first textures (about a million blocks)
+        t0 = time.time()
+        all_jobs = jobs()
+        take, acc = [], 0
+        for j, n in zip(all_jobs, counts):
+            take.append(j)
+            acc += int(n)
+            if acc >= (1 << 20):
+                break
+        tuned = g.autotune(take, levels=sorted(set(job_levels)))
+        stream.synchronize()
+        tuning = {"sample_blocks": acc, "seconds": round(time.time() - t0, 2),
+                  "levels": {str(k): v for k, v in tuned.as_dict().items()}}
+        log(f"autotune: {tuning}")
+    plan = g.Plan(jobs(), use_graph=True, stream=stream) if tuning else \
+        g.Plan(jobs(), schedule=g.SCHEDULE_PERSISTENT, use_graph=True, order_by_cost=True, stream=stream)
     ev = [g.Event() for _ in range(args.steps + 1)]
 
     def timed(run_once):
@@ -380,7 +399,9 @@ def main():
                                 + ", three_color_mode+three_color_black, weights 1, cw (1,1,1)"),
                    "textures": args.textures, "images": len(texs), "blocks": total,
                    "level": "1-9" if args.mixed_levels else args.level, "seed": args.seed,
-                   "path": "persistent graph-captured launch", "device": dev},
+                   "path": "persistent graph-captured launch" + (", autotuned" if tuning else ""),
+                   "device": dev},
+        "autotune": tuning,
         "plain": result_plain,
         "batched_over_plain": (round(result_plain["ms_per_step"] / ms_batched, 4) if result_plain else None),
         "roofline": roofline,

@@ -106,6 +106,18 @@ class ServerStats(C.Structure):
                 ("pending", C.c_uint64), ("inflight", C.c_uint64), ("memory_in_use", C.c_uint64)]
 
 
+class Tuning(C.Structure):
+    """icbc_tuning (include/icbc_gpu.h): per-level launch settings."""
+    _fields_ = [("tile_blocks", C.c_uint32 * 10), ("workgroups_per_cu", C.c_int32 * 10),
+                ("schedule", C.c_int32 * 10), ("ns_per_block", C.c_float * 10)]
+
+    def as_dict(self):
+        return {lv: {"tile_blocks": self.tile_blocks[lv], "workgroups_per_cu": self.workgroups_per_cu[lv],
+                     "schedule": "persistent" if self.schedule[lv] == 1 else "grid",
+                     "ns_per_block": round(float(self.ns_per_block[lv]), 4)}
+                for lv in range(10) if self.tile_blocks[lv]}
+
+
 class PlanOptions(C.Structure):
     _fields_ = [("schedule", C.c_int32), ("use_graph", C.c_int32),
                 ("workgroups_per_cu", C.c_int32), ("order_by_cost", C.c_int32)]
@@ -130,6 +142,15 @@ _SIGNATURES = {
     "icbc_gpu_device_name": (C.c_int, [C.c_char_p, C.c_size_t]),
     "icbc_gpu_version": (C.c_int, []),
     "icbc_gpu_check_reciprocal": (C.c_int, [C.POINTER(C.c_uint64)]),
+    "icbc_gpu_autotune": (C.c_int, [C.c_void_p, C.c_size_t, C.c_uint32, C.POINTER(Tuning)]),
+    "icbc_gpu_get_tuning": (C.c_int, [C.POINTER(Tuning)]),
+    "icbc_gpu_set_tuning": (C.c_int, [C.POINTER(Tuning)]),
+    "icbc_container_size": (C.c_uint64, [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "icbc_container_data_bytes": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32]),
+    "icbc_container_encode": (C.c_int, [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                        C.c_uint64]),
+    "icbc_container_write": (C.c_int, [C.c_int, C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "icbc_container_compare_dds": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_void_p]),
     "icbc_gpu_image_block_count": (C.c_uint64, [C.POINTER(Image)]),
     "icbc_gpu_image_to_blocks": (C.c_int, [C.POINTER(Image), C.c_void_p, C.c_void_p, C.c_void_p]),
     "icbc_gpu_mip_chain_bytes": (C.c_uint64, [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]),
@@ -472,6 +493,74 @@ class MipChain:
         self.buf.close()
 
 
+# ---------------------------------------------------------------------------
+# autotuning
+
+def autotune(sample_jobs, levels=range(10)):
+    """Tunes the listed levels on device-memory sample jobs; returns the
+    current device's Tuning."""
+    arr = _job_array(sample_jobs)
+    mask = 0
+    for lv in levels:
+        mask |= 1 << int(lv)
+    t = Tuning()
+    check(lib().icbc_gpu_autotune(C.addressof(arr), len(sample_jobs), mask, C.byref(t)))
+    return t
+
+
+def get_tuning():
+    t = Tuning()
+    check(lib().icbc_gpu_get_tuning(C.byref(t)))
+    return t
+
+
+def set_tuning(t):
+    check(lib().icbc_gpu_set_tuning(C.byref(t) if t is not None else None))
+
+
+# ---------------------------------------------------------------------------
+# containers
+
+CONTAINER_DDS = 0
+CONTAINER_KTX = 1
+
+
+def _level_codes(level_codes):
+    if isinstance(level_codes, np.ndarray):
+        level_codes = [level_codes]
+    return np.ascontiguousarray(np.concatenate([np.asarray(c, np.uint8).reshape(-1) for c in level_codes])), len(level_codes)
+
+
+def container_bytes(kind, width, height, level_codes):
+    """The DDS / KTX file of a BC1 texture (``level_codes``: one (n, 8) uint8
+    array per mip level, level 0 first) as bytes."""
+    data, levels = _level_codes(level_codes)
+    need = int(lib().icbc_container_data_bytes(width, height, levels))
+    if need != data.nbytes:
+        raise ValueError(f"{levels} levels of {width}x{height} hold {need} bytes of blocks, got {data.nbytes}")
+    size = int(lib().icbc_container_size(int(kind), width, height, levels))
+    out = np.empty(size, np.uint8)
+    check(lib().icbc_container_encode(int(kind), width, height, levels, data.ctypes.data, out.ctypes.data, size))
+    return out.tobytes()
+
+
+def write_container(path, kind, width, height, level_codes):
+    data, levels = _level_codes(level_codes)
+    need = int(lib().icbc_container_data_bytes(width, height, levels))
+    if need != data.nbytes:
+        raise ValueError(f"{levels} levels of {width}x{height} hold {need} bytes of blocks, got {data.nbytes}")
+    check(lib().icbc_container_write(int(kind), os.fsencode(path), width, height, levels, data.ctypes.data))
+
+
+def compare_dds(path, width, height, codes):
+    """True if the top level of the DDS file equals ``codes`` (the reference's -cmp check)."""
+    codes = np.ascontiguousarray(codes, np.uint8)
+    r = lib().icbc_container_compare_dds(os.fsencode(path), width, height, codes.ctypes.data)
+    if r < 0:
+        check(r)
+    return r == 1
+
+
 def _job_array(jobs):
     arr = (Job * len(jobs))(*jobs)
     return arr
@@ -506,12 +595,18 @@ def compress_many(level, colors, weights=None, color_weights=(1.0, 1.0, 1.0),
 class Plan:
     """Persistent batched plan over device-memory jobs (icbc_gpu_plan_*)."""
 
-    def __init__(self, jobs, schedule=SCHEDULE_PERSISTENT, use_graph=True, workgroups_per_cu=0,
+    def __init__(self, jobs, schedule=None, use_graph=True, workgroups_per_cu=0,
                  order_by_cost=True, stream=None):
+        """With every option at its default the library picks them (the
+        device's tuning when one is installed, else persistent + graph)."""
         self._jobs = _job_array(jobs)
-        opt = PlanOptions(schedule, int(bool(use_graph)), workgroups_per_cu, int(bool(order_by_cost)))
+        if schedule is None and use_graph and not workgroups_per_cu and order_by_cost:
+            opt = None
+        else:
+            opt = PlanOptions(SCHEDULE_PERSISTENT if schedule is None else schedule, int(bool(use_graph)),
+                              workgroups_per_cu, int(bool(order_by_cost)))
         h = C.c_void_p()
-        check(lib().icbc_gpu_plan_create(C.addressof(self._jobs), len(jobs), C.byref(opt),
+        check(lib().icbc_gpu_plan_create(C.addressof(self._jobs), len(jobs), C.byref(opt) if opt else None,
                                          stream.handle if stream else None, C.byref(h)))
         self.handle = h
 

@@ -56,6 +56,11 @@ constexpr size_t kWorkspaceBytesPerWorkgroup = ((size_t)2 * kTile * kSatRows + (
 hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, uint32_t* counter,
                          float4* workspace, uint32_t grid, bool refine, hipStream_t stream);
 
+// The one-workgroup-per-tile schedule gives every tile its own workspace
+// slice; tuned defaults use it only for launches of at most this many waves
+// of resident workgroups.
+constexpr uint64_t kGridScheduleMaxWaves = 4;
+
 // Whether a launch over these (host-side) jobs needs the refining instance.
 inline bool jobs_need_refine(const JobDev* jobs, size_t n) {
     for (size_t k = 0; k < n; k++)

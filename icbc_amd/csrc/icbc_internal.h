@@ -17,6 +17,9 @@ int icbc_internal_ready(int* decoder);
 int icbc_internal_resident(uint32_t* grid);
 // Argument checks shared by every entry point taking a job.
 int icbc_internal_validate(const icbc_bc1_job& job);
+// The current device's tuning (false when no level is tuned) / replace it.
+bool icbc_internal_get_tuning(icbc_tuning* out);
+void icbc_internal_set_tuning(const icbc_tuning* t);
 
 namespace icbc_gpu {
 

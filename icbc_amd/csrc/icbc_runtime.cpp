@@ -278,6 +278,39 @@ int validate(const icbc_bc1_job& j, size_t k) {
     return ICBC_OK;
 }
 
+// Per-device tuning tables (icbc_gpu_autotune / icbc_gpu_set_tuning).
+std::mutex g_tuning_mutex;
+std::vector<std::pair<int, icbc_tuning>> g_tuning;
+
+bool tuning_for_current(icbc_tuning* out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    std::lock_guard<std::mutex> lock(g_tuning_mutex);
+    for (auto& t : g_tuning)
+        if (t.first == dev) {
+            *out = t.second;
+            for (int l = 0; l < 10; l++)
+                if (out->tile_blocks[l]) return true;
+            return false;
+        }
+    return false;
+}
+
+void set_tuning_for_current(const icbc_tuning* t) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    std::lock_guard<std::mutex> lock(g_tuning_mutex);
+    for (auto& e : g_tuning)
+        if (e.first == dev) {
+            if (t) e.second = *t;
+            else memset(&e.second, 0, sizeof e.second);
+            return;
+        }
+    icbc_tuning z;
+    memset(&z, 0, sizeof z);
+    g_tuning.push_back({dev, t ? *t : z});
+}
+
 // Blocks per tile when the caller leaves it to the library.  With enough work
 // to give every resident workgroup 64-block tiles, use 64 (shrinking only for
 // jobs smaller than a tile); otherwise split the blocks over about one tile per
@@ -504,23 +537,47 @@ int choose_tiles(std::vector<JobDev>& dev, const icbc_bc1_job* jobs, bool per_jo
     if (r) return r;
     uint64_t total = 0;
     for (const JobDev& d : dev) total += d.block_count;
+    icbc_tuning tune;
+    const bool tuned = tuning_for_current(&tune);
     for (size_t k = 0; k < dev.size(); k++)
-        if (jobs[k].tile_blocks == 0)
-            dev[k].tile_blocks = (int32_t)auto_tile_blocks_impl(dev[k].block_count,
-                                                           per_job_launch ? dev[k].block_count : total, resident);
+        if (jobs[k].tile_blocks == 0) {
+            const uint64_t launch_blocks = per_job_launch ? dev[k].block_count : total;
+            uint32_t t = auto_tile_blocks_impl(dev[k].block_count, launch_blocks, resident);
+            // a launch that fills every workgroup many times over takes the tuned
+            // tile of its level, still no larger than the job needs
+            const int lv = dev[k].level;
+            if (tuned && lv >= 0 && lv <= 9 && tune.tile_blocks[lv] && launch_blocks >= 64ull * resident) {
+                t = tune.tile_blocks[lv];
+                while (t > 8 && t / 2 >= dev[k].block_count) t /= 2;
+            }
+            dev[k].tile_blocks = (int32_t)t;
+        }
     return ICBC_OK;
 }
 
 // Relative cost of one block at each level, used to issue expensive tiles
 // first (larger candidate searches and refinement at the top levels).
-float level_cost(int level) {
+// Without a tuning, a fixed estimate; with one, the measured ns per block of
+// every level present in the launch (when all of them are tuned).
+float static_level_cost(int level) {
     static const float cost[10] = {1, 2, 40, 40, 60, 60, 60, 60, 70, 110};
     return (level >= 0 && level <= 9) ? cost[level] : 0.0f;
 }
 
 void order_tiles(std::vector<uint2>& tiles, const std::vector<JobDev>& dev) {
+    float cost[10];
+    for (int l = 0; l < 10; l++) cost[l] = static_level_cost(l);
+    icbc_tuning tune;
+    if (tuning_for_current(&tune)) {
+        bool all = true;
+        for (const JobDev& d : dev)
+            if (d.level >= 0 && d.level <= 9 && !(tune.ns_per_block[d.level] > 0.0f)) all = false;
+        if (all)
+            for (int l = 0; l < 10; l++) cost[l] = tune.ns_per_block[l];
+    }
+    auto c = [&](const JobDev& d) { return (d.level >= 0 && d.level <= 9) ? cost[d.level] : 0.0f; };
     std::stable_sort(tiles.begin(), tiles.end(), [&](const uint2& a, const uint2& b) {
-        return level_cost(dev[a.x].level) * dev[a.x].tile_blocks > level_cost(dev[b.x].level) * dev[b.x].tile_blocks;
+        return c(dev[a.x]) * dev[a.x].tile_blocks > c(dev[b.x]) * dev[b.x].tile_blocks;
     });
 }
 
@@ -553,6 +610,10 @@ int icbc_internal_ready(int* decoder) { return ensure_device_ready(decoder); }
 int icbc_internal_resident(uint32_t* grid) { return resident_workgroups(grid); }
 
 int icbc_internal_validate(const icbc_bc1_job& job) { return validate(job, 0); }
+
+bool icbc_internal_get_tuning(icbc_tuning* out) { return tuning_for_current(out); }
+
+void icbc_internal_set_tuning(const icbc_tuning* t) { set_tuning_for_current(t); }
 
 namespace icbc_gpu {
 
@@ -712,7 +773,28 @@ int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_pla
     std::vector<uint2> tiles = make_tiles(dev);
     if (tiles.size() > 0xffffffffu) return fail(ICBC_ERR_INVALID_ARGUMENT, "plan too large");
     icbc_plan_options o{ICBC_SCHEDULE_PERSISTENT, 1, 0, 1};
-    if (opt) o = *opt;
+    if (opt) {
+        o = *opt;
+    } else {
+        // defaults: the tuned occupancy and schedule of the level that carries
+        // most of the plan's measured work
+        icbc_tuning tune;
+        if (tuning_for_current(&tune)) {
+            double share[10] = {0};
+            for (const JobDev& d : dev)
+                if (d.level >= 0 && d.level <= 9) share[d.level] += (double)d.block_count * tune.ns_per_block[d.level];
+            int top = -1;
+            for (int l = 0; l < 10; l++)
+                if (tune.tile_blocks[l] && (top < 0 || share[l] > share[top])) top = l;
+            uint32_t resident = 0;
+            if (top >= 0 && share[top] > 0 && resident_workgroups(&resident) == ICBC_OK) {
+                o.workgroups_per_cu = tune.workgroups_per_cu[top];
+                if (tune.schedule[top] == ICBC_SCHEDULE_PERSISTENT ||
+                    tiles.size() <= kGridScheduleMaxWaves * (uint64_t)resident)
+                    o.schedule = tune.schedule[top];
+            }
+        }
+    }
     if (o.order_by_cost) order_tiles(tiles, dev);
     icbc_bc1_plan* pl = new icbc_bc1_plan();
     pl->opt = o;

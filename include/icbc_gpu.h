@@ -177,6 +177,32 @@ int icbc_gpu_evaluate_bc1_error_batch(const uint8_t* rgba, const void* codes,
                                       uint64_t block_count, float* errors, int decoder,
                                       void* stream);
 
+/* ---- autotuning -----------------------------------------------------------
+ * Per quality level: blocks per wavefront tile, resident workgroups per CU and
+ * the schedule, measured on the caller's own sample.  The current device's
+ * tuning then drives jobs with tile_blocks = 0 in large launches (smaller
+ * launches keep spreading their blocks over every workgroup), the defaults of
+ * plans created with opt = NULL, and the cost order in which mixed-level
+ * launches hand out tiles (dearest first). */
+typedef struct icbc_tuning {
+    uint32_t tile_blocks[10];      /* 8, 16, 32 or 64; 0 = level not tuned */
+    int32_t workgroups_per_cu[10]; /* resident workgroups per CU (persistent schedule) */
+    int32_t schedule[10];          /* ICBC_SCHEDULE_* */
+    float ns_per_block[10];        /* measured time of the chosen setting */
+} icbc_tuning;
+
+/* Times every setting (tile 8/16/32/64 x {persistent at full and half
+ * occupancy, one workgroup per tile}) for each level in `level_mask` (bit L =
+ * level L) on `sample` (device-memory jobs; their `level` and `tile_blocks`
+ * are overridden, their outputs are overwritten), keeps the fastest per level
+ * as the current device's tuning (merged with levels tuned before) and
+ * returns the whole table in `out` (may be NULL). */
+int icbc_gpu_autotune(const icbc_bc1_job* sample, size_t n_jobs, uint32_t level_mask, icbc_tuning* out);
+/* The current device's tuning (all zero when none). */
+int icbc_gpu_get_tuning(icbc_tuning* out);
+/* Replaces it (e.g. with a table saved from an earlier run); NULL clears it. */
+int icbc_gpu_set_tuning(const icbc_tuning* tuning);
+
 /* ---- texture pre-pass (device) --------------------------------------------
  * The reference's harness turns an RGBA8 image into blocks on the CPU
  * (icbc_test.cpp:353-380: 4x4 gather, texel j = 4*y + x, c / 255.0f at
@@ -203,6 +229,28 @@ uint64_t icbc_gpu_mip_chain_bytes(uint32_t width, uint32_t height, uint32_t* lev
  * `*n_levels` the count. */
 int icbc_gpu_build_mip_chain(const icbc_image* source, float* dst, uint64_t dst_bytes, icbc_image* levels,
                              uint32_t max_levels, uint32_t* n_levels, void* stream);
+
+/* ---- containers (host memory) ---------------------------------------------
+ * BC1 files with mip levels.  Level k is max(1, width >> k) x max(1, height >>
+ * k) texels in ceil(w/4) x ceil(h/4) blocks; `codes` holds the levels' blocks
+ * back to back, level 0 first (8 bytes per block, as the encoder writes them). */
+#define ICBC_CONTAINER_DDS 0 /* header of the reference's output_dxt_dds (icbc_test.cpp:155-211);
+                                mip count and mipmap caps set when levels > 1 */
+#define ICBC_CONTAINER_KTX 1 /* KTX 1.1 as the reference's output_dxt_ktx (icbc_test.cpp:233-279),
+                                with one imageSize word per level */
+
+/* Bytes of the file (0 for invalid arguments) / of the block data alone. */
+uint64_t icbc_container_size(int kind, uint32_t width, uint32_t height, uint32_t levels);
+uint64_t icbc_container_data_bytes(uint32_t width, uint32_t height, uint32_t levels);
+/* Serialises into `out` (at least icbc_container_size bytes). */
+int icbc_container_encode(int kind, uint32_t width, uint32_t height, uint32_t levels, const void* codes, void* out,
+                          uint64_t out_bytes);
+/* Writes the file to `path`. */
+int icbc_container_write(int kind, const char* path, uint32_t width, uint32_t height, uint32_t levels,
+                         const void* codes);
+/* The reference's regression check (icbc_test.cpp:213-231): 1 if the top-level
+ * block data of the DDS file at `path` equals `codes`, 0 if not, < 0 on error. */
+int icbc_container_compare_dds(const char* path, uint32_t width, uint32_t height, const void* codes);
 
 /* ---- diagnostics -------------------------------------------------------- */
 const char* icbc_gpu_last_error(void); /* text of this thread's last failure */

@@ -18,8 +18,16 @@ Contents (all inputs are seeded synthetic blocks, block-major 16 x RGBA):
   tables_*, midpoints*      the reference's init tables (decoder 0)
   dec_codes, dec_rgba       decode_bc1 outputs for every decoder
   eval_rgba, eval_err       evaluate_bc1_error outputs for every decoder
+
+and tests/golden/c1_512x512_L8.dds: the 512x512 plain-content image (seed 1,
+50/30/20 gradient / value noise / flat) encoded by the reference at level 8
+with three_color_mode off, weights 1, channel weights 1, in the DDS layout of
+the reference's output_dxt_dds (icbc_test.cpp:155-211; its uninitialised
+reserved words written as zeros) -- the file its -cmp regression mode compares
+against (icbc_test.cpp:213-231).
 """
 import os
+import struct
 import sys
 
 import numpy as np
@@ -31,6 +39,28 @@ from icbc_amd import synth  # noqa: E402
 from oracle import Reference  # noqa: E402
 
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "bc1_golden.npz")
+DDS_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "c1_512x512_L8.dds")
+
+
+def dds_header(width, height):
+    """DDS_HEADER of icbc_test.cpp:157-196 for one DXT1 level."""
+    fourcc = lambda s: struct.unpack("<I", s)[0]  # noqa: E731
+    flags = 0x1 | 0x1000 | 0x4 | 0x2 | 0x80000  # CAPS | PIXELFORMAT | WIDTH | HEIGHT | LINEARSIZE
+    linear = 8 * ((width + 3) // 4) * ((height + 3) // 4)
+    head = struct.pack("<7I", fourcc(b"DDS "), 124, flags, height, width, linear, 0)
+    head += struct.pack("<I", 1) + b"\0" * 44                     # mipmapcount, reserved[11]
+    head += struct.pack("<8I", 32, 0x4, fourcc(b"DXT1"), 0, 0, 0, 0, 0)
+    head += struct.pack("<5I", 0x1000, 0, 0, 0, 0)                  # caps1..4, notused
+    assert len(head) == 128
+    return head
+
+
+def write_c1_dds(ref):
+    colors, _ = synth.make_image_blocks(1, 512, 512, synth.MIX_PLAIN)
+    ref.init(0)
+    codes, _, _ = ref.compress_many(8, colors, None, np.ones(3, np.float32), 0, 0)
+    with open(DDS_OUT, "wb") as f:
+        f.write(dds_header(512, 512) + np.ascontiguousarray(codes, np.uint8).tobytes())
 
 
 def inputs():
@@ -112,6 +142,8 @@ def main():
         midpoints5=m5, midpoints6=m6, dec_codes=dec_codes, dec_rgba=dec_rgba,
         eval_rgba=eval_rgba, eval_err=eval_err)
     print(f"wrote {OUT}: {n} blocks x {len(configs)} configs, {os.path.getsize(OUT)} bytes")
+    write_c1_dds(ref)
+    print(f"wrote {DDS_OUT}: {os.path.getsize(DDS_OUT)} bytes")
 
 
 if __name__ == "__main__":

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared_c_functions():
     text = open(os.path.join(ROOT, "include", "icbc_gpu.h")).read()
-    return sorted(set(re.findall(r"\b(icbc_(?:gpu|server)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(icbc_(?:gpu|server|container)_\w+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_c_function():
@@ -42,7 +42,7 @@ def test_no_internal_c_symbols_leak():
     out = subprocess.check_output(["nm", "-D", "--defined-only", icbc_amd.LIB_PATH], text=True)
     plain = [l.split()[-1] for l in out.splitlines() if " T " in l]
     for s in plain:
-        assert s.startswith("icbc_gpu_") or s.startswith("icbc_server_") or s.startswith("_Z"), s
+        assert s.startswith(("icbc_gpu_", "icbc_server_", "icbc_container_", "_Z")), s
 
 
 def test_library_links_hip_runtime_and_has_gfx950_code():

@@ -1,0 +1,80 @@
+"""GPU tests of the autotuner: it returns a valid per-level table, installs
+it for the device, and changes only speed -- codes and errors from tuned
+launches equal the untuned ones."""
+import numpy as np
+import pytest
+
+from icbc_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _sample(gpu, n_tex=6):
+    texs = synth.texture_set(17, count=n_tex, min_side=64, max_side=512)
+    counts = [len(t[5]) for t in texs]
+    total = sum(counts)
+    colors = np.concatenate([t[5] for t in texs])
+    dcol = gpu.DeviceBuffer.from_array(colors)
+    dcode = gpu.DeviceBuffer(8 * total)
+    derr = gpu.DeviceBuffer(4 * total)
+    jobs, off = [], 0
+    for t, n in zip(texs, counts):
+        jobs.append(gpu.make_job(t[4], n, dcol.ptr + 256 * off, dcode.ptr + 8 * off, None, derr.ptr + 4 * off,
+                                 None, (1, 1, 1), True, True))
+        off += n
+    return jobs, total, (dcol, dcode, derr)
+
+
+def test_autotune_table_and_identical_results(gpu):
+    jobs, total, (dcol, dcode, derr) = _sample(gpu)
+    s = gpu.Stream()
+    try:
+        gpu.set_tuning(None)
+        plan = gpu.Plan(jobs, stream=s)
+        plan.execute(s)
+        s.synchronize()
+        base = dcode.download(np.uint8, (total, 8)), derr.download(np.float32, (total,))
+        plan.close()
+
+        t = gpu.autotune(jobs, levels=[1, 8, 9])
+        table = t.as_dict()
+        assert sorted(table) == [1, 8, 9]
+        for lv, e in table.items():
+            assert e["tile_blocks"] in (8, 16, 32, 64) and e["ns_per_block"] > 0
+        assert gpu.get_tuning().as_dict() == table
+        # level 9 costs more per block than level 1
+        assert table[9]["ns_per_block"] > table[1]["ns_per_block"]
+
+        # tuned defaults (tile sizes, plan occupancy/schedule, cost order): same results
+        dcode.fill(0)
+        plan = gpu.Plan(jobs, stream=s)
+        for _ in range(2):
+            plan.execute(s)
+            s.synchronize()
+            assert np.array_equal(dcode.download(np.uint8, (total, 8)), base[0])
+            assert np.array_equal(derr.download(np.float32, (total,)).view(np.uint32), base[1].view(np.uint32))
+        plan.close()
+        dcode.fill(0)
+        gpu.compress_batch(jobs, stream=s)
+        s.synchronize()
+        assert np.array_equal(dcode.download(np.uint8, (total, 8)), base[0])
+    finally:
+        gpu.set_tuning(None)
+        for b in (dcol, dcode, derr):
+            b.close()
+    assert gpu.get_tuning().as_dict() == {}
+
+
+def test_set_tuning_validates(gpu):
+    t = gpu.Tuning()
+    t.tile_blocks[3] = 12
+    with pytest.raises(gpu.IcbcError):
+        gpu.set_tuning(t)
+    t.tile_blocks[3] = 16
+    t.schedule[3] = 1
+    t.ns_per_block[3] = 1.0
+    try:
+        gpu.set_tuning(t)
+        assert gpu.get_tuning().as_dict()[3]["tile_blocks"] == 16
+    finally:
+        gpu.set_tuning(None)

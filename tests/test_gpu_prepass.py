@@ -173,3 +173,21 @@ def test_image_job_argument_checks(gpu):
     finally:
         dcode.close()
         di.close()
+
+
+def test_golden_dds_regression(gpu, tmp_path):
+    # the reference harness's -cmp mode (icbc_test.cpp:213-231): encode the
+    # 512x512 plain-content image at level 8 and compare the block data with
+    # the reference-encoded DDS fixture; the whole written file must match too
+    import os
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c1_512x512_L8.dds")
+    img = synth.blocks_to_image(synth.make_image_blocks(1, 512, 512, synth.MIX_PLAIN)[0], 512, 512)
+    di = gpu.DeviceImage(img)
+    try:
+        codes, _ = _encode_image_job(gpu, 8, di.desc, di.block_count, tcm=False, tcb=False)
+    finally:
+        di.close()
+    assert gpu.compare_dds(golden, 512, 512, codes)
+    out = tmp_path / "c1.dds"
+    gpu.write_container(str(out), gpu.CONTAINER_DDS, 512, 512, codes)
+    assert out.read_bytes() == open(golden, "rb").read()
