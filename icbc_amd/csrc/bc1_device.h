@@ -637,6 +637,62 @@ __device__ __forceinline__ float candidate4(float4 s0, float4 s1, float4 s2, flo
     return dot3(e1, metric);
 }
 
+// ---------------------------------------------------------------------------
+// Lower bound for pruning the four-cluster search.  A candidate's error, as
+// candidate4 computes it, is the quadratic
+//   E(a, b) = sum_c m_c (a^2 A + b^2 B + 2 a b C - 2 a X_c - 2 b Y_c)
+// at its snapped endpoints, with A + B + 2C = W (total weight: the palette
+// weights of each cluster sum to 1), A + C = Wa (sum of w * alpha) and
+// Y = t - X (t = block totals).  Over all real (a, b) its minimum is
+//   -K0 - W * sum_c m_c (X_c - (Wa / W) t_c)^2 / (A W - Wa^2),  K0 = sum_c m_c t_c^2 / W,
+// so no candidate can score below that.  The bound is computed here with
+// fused multiply-adds and the hardware reciprocal (it only decides which
+// candidates are evaluated, never a result); the rounding of this form and
+// of candidate4's own arithmetic is covered by the block's margin
+// (bound_row).  Candidates with a poorly conditioned system
+// (A W - Wa^2 < A W / 16) get -inf: never pruned.
+__device__ __forceinline__ float candidate4_bound(float4 s0, float4 s1, float4 s2, float W, f3 t, float invW,
+                                                  float K0, f3 m, const DecoderWeights& dw) {
+    const float W1 = s1.w - s0.w, W2 = s2.w - s1.w;
+    const float A = __builtin_fmaf(W2, dw.w1_9, __builtin_fmaf(W1, dw.w4_9, s0.w));
+    const float Wa = __builtin_fmaf(W2, dw.w1_3, __builtin_fmaf(W1, dw.w2_3, s0.w));
+    const float AW = A * W;
+    const float D = __builtin_fmaf(-Wa, Wa, AW);
+    const float mu = Wa * invW;
+    const float dx = __builtin_fmaf(-mu, t.x, __builtin_fmaf(s2.x - s1.x, dw.w1_3, __builtin_fmaf(s1.x - s0.x, dw.w2_3, s0.x)));
+    const float dy = __builtin_fmaf(-mu, t.y, __builtin_fmaf(s2.y - s1.y, dw.w1_3, __builtin_fmaf(s1.y - s0.y, dw.w2_3, s0.y)));
+    const float dz = __builtin_fmaf(-mu, t.z, __builtin_fmaf(s2.z - s1.z, dw.w1_3, __builtin_fmaf(s1.z - s0.z, dw.w2_3, s0.z)));
+    const float sd = __builtin_fmaf(m.z * dz, dz, __builtin_fmaf(m.y * dy, dy, (m.x * dx) * dx));
+    const float lb = __builtin_fmaf(-W * sd, __builtin_amdgcn_rcpf(D), -K0);
+    return (D >= AW * 0.0625f) ? lb : -INFINITY;
+}
+
+// Margin of a block's four-cluster pruning: 2^-13 sum_c m_c P_c (P = sum of
+// w x^2, the scale of the errors and of the bound's rounding) + 2^-18 sum_c m_c
+// (4 W + 4 sum w |x_c|) (the rounding of candidate4's error, whose terms are
+// bounded by the weights and weighted colour sums since the snapped endpoints
+// lie in [0, 1]).  Blocks whose sums are not finite are never pruned.
+__device__ __forceinline__ float4 bound_row(const f3 (&c)[16], const float (&w)[16], int n, f3 m) {
+    float px = 0.0f, py = 0.0f, pz = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f, W = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        if (i < n) {
+            px = __builtin_fmaf(w[i] * c[i].x, c[i].x, px);
+            py = __builtin_fmaf(w[i] * c[i].y, c[i].y, py);
+            pz = __builtin_fmaf(w[i] * c[i].z, c[i].z, pz);
+            ax = __builtin_fmaf(w[i], fabsf(c[i].x), ax);
+            ay = __builtin_fmaf(w[i], fabsf(c[i].y), ay);
+            az = __builtin_fmaf(w[i], fabsf(c[i].z), az);
+            W += w[i];
+        }
+    }
+    const float scale = (m.x * px + m.y * py) + m.z * pz;
+    const float terms = 4.0f * W * ((m.x + m.y) + m.z) + 4.0f * ((m.x * ax + m.y * ay) + m.z * az);
+    const float margin = 0x1p-13f * scale + 0x1p-18f * terms;
+    const bool ok = margin == margin && margin < 1e30f && W > 0.0f;
+    return make_float4(margin, ok ? 1.0f : 0.0f, 0.0f, 0.0f);
+}
+
 __device__ __forceinline__ float candidate3(float4 s0, float4 s1, float4 tot, f3 metric, f3& a, f3& b) {
     f3 x0 = mk(s0.x, s0.y, s0.z), x1 = mk(s1.x, s1.y, s1.z);
     float w0 = s0.w, w1 = s1.w;

@@ -86,9 +86,26 @@ constexpr int kNoWinner = 0x7fffffff;
 struct Shared {
     uint32_t four[976];
     uint32_t three[160];
-    float unit[256];  // k / 255.0f
+    float unit[256];         // k / 255.0f
     float4 sat[2][kSatRows];
+    uint16_t survivors[976];  // four-cluster candidates left by the bound pass
 };
+
+// Four-cluster searches with at least this many colours run the bound pass
+// first (smaller ones have too few candidates to gain from it).
+constexpr int kPruneMinColors = 9;
+
+// Wave-wide minimum (all lanes get it); NaN inputs must be mapped out first.
+__device__ __forceinline__ float wave_min(float e) {
+    int dummy = 0;
+    argmin_step<0xB1, 0xf>(e, dummy);
+    argmin_step<0x4E, 0xf>(e, dummy);
+    argmin_step<0x141, 0xf>(e, dummy);
+    argmin_step<0x140, 0xf>(e, dummy);
+    argmin_step<0x142, 0xa>(e, dummy);
+    argmin_step<0x143, 0xc>(e, dummy);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e), 63));
+}
 
 // First block b >= from (wave-uniform) whose owner lane holds my_n >= min_n.
 __device__ __forceinline__ int next_block(int my_n, int tile_blocks, int from, int min_n) {
@@ -117,21 +134,81 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offs
     auto base = [&](int blk) {
         return ws + blk * kSatRows + (__builtin_amdgcn_readlane(my_alt, blk) ? alt_offset : 0);
     };
-    if (b < tile_blocks && lane <= nb) row = base(b)[lane];
+    if (b < tile_blocks && (lane <= nb || lane == kBoundRow)) row = base(b)[lane];
     int buf = 0;
     while (b < tile_blocks) {
-        if (lane <= nb) sh.sat[buf][lane] = row;
+        if (lane <= nb || lane == kBoundRow) sh.sat[buf][lane] = row;
         __syncthreads();
         // prefetch the next block's rows while this one is searched
         const int b2 = next_block(my_n, tile_blocks, b + 1, kMinN);
         const int nb2 = b2 < tile_blocks ? __builtin_amdgcn_readlane(my_n, b2) : 0;
-        if (b2 < tile_blocks && lane <= nb2) row = base(b2)[lane];
+        if (b2 < tile_blocks && (lane <= nb2 || lane == kBoundRow)) row = base(b2)[lane];
 
         const float4* sat = sh.sat[buf];
         const float4 tot = sat[nb];
         float best = FLT_MAX;
         int besti = kNoWinner;
-        if (kClusters == 4) {
+        const float4 brow = sat[kBoundRow];
+        if (kClusters == 4 && nb >= kPruneMinColors && brow.y != 0.0f) {
+            const int total = (nb + 3) * (nb + 2) * (nb + 1) / 6 - 1;  // >= 219: every lane has 3+
+            const float margin = brow.x;
+            const f3 t = mk(tot.x, tot.y, tot.z);
+            const float invW = __builtin_amdgcn_rcpf(tot.w);
+            const float K0 = ((metric.x * t.x) * t.x + (metric.y * t.y) * t.y + (metric.z * t.z) * t.z) * invW;
+            // bound pass: lane l holds the bounds of candidates l, l + 64, ...
+            float lb[16];
+            float key_min = INFINITY;
+            int k_min = 0;
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const int i = lane + kTile * k;
+                lb[k] = INFINITY;
+                if (i < total) {
+                    const uint32_t pk = sh.four[i];
+                    lb[k] = candidate4_bound(sat[pk & 0xff], sat[(pk >> 8) & 0xff], sat[pk >> 16], tot.w, t, invW,
+                                             K0, metric, dw);
+                    if (lb[k] > -INFINITY && lb[k] < key_min) {
+                        key_min = lb[k];
+                        k_min = k;
+                    }
+                }
+            }
+            // threshold: the best true error among the lanes' lowest-bound candidates
+            float thr;
+            {
+                const uint32_t pk = sh.four[lane + kTile * k_min];
+                f3 a, e;
+                float err = candidate4(sat[pk & 0xff], sat[(pk >> 8) & 0xff], sat[pk >> 16], tot, metric, dw, a, e);
+                thr = wave_min(err == err ? err : FLT_MAX);
+            }
+            // survivors (bound - margin <= threshold), in candidate order, to LDS
+            int count = 0;
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const int i = lane + kTile * k;
+                const bool keep = i < total && !(lb[k] - margin > thr);
+                const uint64_t mask = __ballot(keep);
+                if (keep)
+                    sh.survivors[count + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0))] =
+                        (uint16_t)i;
+                count += __popcll(mask);
+            }
+            __syncthreads();
+            // the survivors get the exact evaluation; every candidate that could
+            // reach the minimum error is among them
+            for (int j = lane; j < count; j += kTile) {
+                const int i = sh.survivors[j];
+                const uint32_t pk = sh.four[i];
+                f3 a, e;
+                const float err = candidate4(sat[pk & 0xff], sat[(pk >> 8) & 0xff], sat[pk >> 16], tot, metric,
+                                             dw, a, e);
+                if (err < best) {
+                    best = err;
+                    besti = i;
+                }
+            }
+        } else if (kClusters == 4) {
             const int total = (nb + 3) * (nb + 2) * (nb + 1) / 6 - 1;  // k_four_total[nb - 1]
             // the next candidate's table entry is read while this one is evaluated
             uint32_t pk_next = sh.four[lane];
@@ -246,7 +323,10 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, S
 #pragma unroll
             for (int i = 0; i < 16; i++) park[i] = make_float4(rc[i].x, rc[i].y, rc[i].z, rw[i]);
         }
-        if (fit) build_sat(rc, rw, n, sat4, nullptr);
+        if (fit) {
+            build_sat(rc, rw, n, sat4, nullptr);
+            sat4[kBoundRow] = bound_row(rc, rw, n, metric);
+        }
         if (skip) {
 #pragma unroll
             for (int i = 0; i < 16; i++) {

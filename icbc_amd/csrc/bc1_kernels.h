@@ -7,7 +7,8 @@
 namespace icbc_gpu {
 
 constexpr int kTile = 64;     // blocks per tile = lanes per wavefront
-constexpr int kSatRows = 17;  // zero row + up to 16 prefix rows per block
+constexpr int kSatRows = 18;  // zero row + up to 16 prefix rows + the bound row per block
+constexpr int kBoundRow = 17; // {search margin, prune allowed, 0, 0} of the four-cluster search
 
 // One job as the kernels see it (device memory, built by the runtime).
 struct JobDev {
