@@ -277,3 +277,45 @@ def test_fast_reciprocal_matches_ieee_division_for_every_float(gpu):
     # the cluster fits take 1/det from the hardware estimate plus one Newton
     # step; all 2^32 inputs must give the IEEE-754 quotient bit for bit
     assert gpu.check_reciprocal() == 0
+
+
+def _adversarial_blocks(seed, n):
+    """Blocks built to stress the four-cluster pruning: many distinct colours
+    (n >= 9 after reduction) on near-degenerate lines, weights spread over
+    seven decades, colours outside [0, 1], tiny perturbations."""
+    rng = np.random.default_rng(seed)
+    out = np.empty((n, 16, 4), np.float32)
+    wts = np.empty((n, 16), np.float32)
+    for b in range(n):
+        kind = b % 5
+        if kind == 0:    # points on a line with tiny off-line jitter
+            a, d = rng.random(3), rng.random(3) - 0.5
+            tt = np.sort(rng.random(16))
+            c = a + tt[:, None] * d + rng.normal(0, 1e-4, (16, 3))
+        elif kind == 1:  # two tight clusters plus a few stragglers
+            p, q = rng.random(3), rng.random(3)
+            c = np.where(rng.random((16, 1)) < 0.5, p, q) + rng.normal(0, 3e-3, (16, 3))
+        elif kind == 2:  # out-of-range colours
+            c = rng.uniform(-2, 3, (16, 3))
+        elif kind == 3:  # a gradient with one texel far away
+            a, d = rng.random(3), rng.random(3) - 0.5
+            c = a + np.linspace(0, 1, 16)[:, None] * d
+            c[rng.integers(16)] = rng.random(3) * 4 - 1
+        else:            # white noise
+            c = rng.random((16, 3))
+        out[b, :, :3] = c
+        out[b, :, 3] = 1
+        w = 10.0 ** rng.uniform(-7, 0, 16) if b % 3 == 0 else rng.random(16) + 0.01
+        wts[b] = w
+    return out, wts
+
+
+@pytest.mark.parametrize("cw", [(1, 1, 1), (0.3, 0.59, 0.11), (0, 1, 0), (5, 0.001, 1)])
+def test_pruned_search_adversarial_blocks(gpu, oracle, cw):
+    blocks, weights = _adversarial_blocks(31, 6000)
+    for level, tcm in ((5, False), (8, True), (8, False)):
+        want = oracle.compress_many(level, blocks, weights, cw, tcm, tcm, nthreads=8)
+        got = gpu.compress_many(level, blocks, weights, cw, tcm, tcm)
+        bad = np.nonzero(np.any(got[0] != want[0], axis=1))[0]
+        assert len(bad) == 0, f"level {level} cw {cw}: {len(bad)} blocks differ, first {bad[:5]}"
+        assert np.array_equal(got[1].view(np.uint32), want[1].view(np.uint32))
