@@ -184,6 +184,36 @@ __device__ __forceinline__ float texel_error(f3 p, f3 c, f3 w) {
     return dot3(d, d);
 }
 
+// The block's texels held in registers, for stages that read them many
+// times (box fit + least squares, the refinement walk); unweighted blocks keep
+// no weights.
+template <bool kWeighted>
+struct TexelRegs {
+    f3 c[16];
+    float w[kWeighted ? 16 : 1];
+    __device__ __forceinline__ explicit TexelRegs(const Texels& t) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) c[i] = t.color(i);
+        if (kWeighted) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) w[kWeighted ? i : 0] = t.w[i];
+        }
+    }
+    __device__ __forceinline__ f3 color(int i) const { return c[i]; }
+    __device__ __forceinline__ float weight(int i) const { return kWeighted ? w[kWeighted ? i : 0] : 1.0f; }
+};
+
+// Loops over the 16 texels unroll fully for register-held texels (static
+// indices keep them in registers) and partly for texels read from memory.
+template <typename Src>
+struct TexelUnroll {
+    static constexpr int value = 4;
+};
+template <bool kWeighted>
+struct TexelUnroll<TexelRegs<kWeighted>> {
+    static constexpr int value = 16;
+};
+
 __device__ __forceinline__ f3 pick(const f3 pal[4], uint32_t k) {
     f3 lo = (k & 1) ? pal[1] : pal[0];
     f3 hi = (k & 1) ? pal[3] : pal[2];
@@ -211,11 +241,12 @@ __device__ __forceinline__ float code_error(const Texels& t, f3 cw, int decoder,
 // summing the errors would.
 
 // compute_indices4 rule (icbc.h:3097-3131)
-__device__ __forceinline__ float select4(const Texels& t, f3 w, const f3 pal[4], uint32_t& indices) {
+template <typename Src>
+__device__ __forceinline__ float select4(const Src& t, f3 w, const f3 pal[4], uint32_t& indices) {
     const f3 p0 = pal[0] * w, p1 = pal[1] * w, p2 = pal[2] * w, p3 = pal[3] * w;
     uint32_t out = 0;
     float e = 0.0f;
-#pragma unroll 4
+#pragma unroll TexelUnroll<Src>::value
     for (int i = 0; i < 16; i++) {
         const f3 x = t.color(i);
         const f3 c = x * w;
@@ -293,28 +324,12 @@ __device__ __forceinline__ float indexed_error(const Src& t, f3 w, const f3 pal[
     return e;
 }
 
-// The block's texels held in registers for the refinement walk, which reads
-// them on every step; unweighted blocks keep no weights.
-template <bool kWeighted>
-struct TexelRegs {
-    f3 c[16];
-    float w[kWeighted ? 16 : 1];
-    __device__ __forceinline__ explicit TexelRegs(const Texels& t) {
-#pragma unroll
-        for (int i = 0; i < 16; i++) c[i] = t.color(i);
-        if (kWeighted) {
-#pragma unroll
-            for (int i = 0; i < 16; i++) w[kWeighted ? i : 0] = t.w[i];
-        }
-    }
-    __device__ __forceinline__ f3 color(int i) const { return c[i]; }
-    __device__ __forceinline__ float weight(int i) const { return kWeighted ? w[kWeighted ? i : 0] : 1.0f; }
-};
 
 // ---------------------------------------------------------------------------
 // Block output (icbc.h:3238-3274)
 
-__device__ __forceinline__ float emit4(const Texels& t, f3 cw, int decoder, const float* unit, f3 a, f3 b, Code& out) {
+template <typename Src>
+__device__ __forceinline__ float emit4(const Src& t, f3 cw, int decoder, const float* unit, f3 a, f3 b, Code& out) {
     uint32_t c0 = to565(a), c1 = to565(b);
     if (c0 < c1) {
         uint32_t tmp = c0;
@@ -720,10 +735,11 @@ __device__ __forceinline__ float candidate3(float4 s0, float4 s1, float4 tot, f3
 // Least-squares endpoints for fixed indices (icbc.h:3278-3310) and box fit
 // (icbc.h:3313-3368).
 
-__device__ __forceinline__ bool lsq_endpoints(uint32_t indices, const Texels& t, f3& a, f3& b) {
+template <typename Src>
+__device__ __forceinline__ bool lsq_endpoints(uint32_t indices, const Src& t, f3& a, f3& b) {
     float a2 = 0.0f, b2 = 0.0f, ab = 0.0f;
     f3 ax = mk(0.0f, 0.0f, 0.0f), bx = mk(0.0f, 0.0f, 0.0f);
-#pragma unroll 4
+#pragma unroll TexelUnroll<Src>::value
     for (int i = 0; i < 16; i++) {
         const uint32_t bits = indices >> (2 * i);
         float beta = (float)(bits & 1);

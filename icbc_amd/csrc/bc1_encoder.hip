@@ -251,8 +251,36 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offs
 // One tile: the reference's compress_bc1 driver (icbc.h:3734-3808) per lane.
 // `ws` is this workgroup's workspace: 64 blocks x 17 SAT rows.
 
-template <bool kRefine>
+// Levels 0-1 from register-held texels: box fit, output block, and at level 1
+// least-squares endpoints and a second output block, all reading the block
+// once (icbc.h:3769-3786 without reduce_colors, as at these levels).
+template <bool kWeighted>
+__device__ __forceinline__ float encode_box(const Texels& tx, bool lsq, f3 cw, int dec, const float* unit,
+                                            Code& out) {
+    const TexelRegs<kWeighted> r(tx);
+    f3 c0, c1;
+    box_fit([&](int i) { return r.color(i); }, 16, c0, c1);
+    float err = emit4(r, cw, dec, unit, c0, c1, out);
+    if (lsq) {
+        f3 a, b;
+        if (lsq_endpoints(out.idx, r, a, b)) {
+            Code o2;
+            const float e2 = emit4(r, cw, dec, unit, a, b, o2);
+            if (e2 < err) {
+                err = e2;
+                out = o2;
+            }
+        }
+    }
+    return err;
+}
+
+// Kernel instances: kModeBox serves launches whose jobs are all at levels
+// 0-1 (or invalid levels), kModeCluster launches without level 9, kModeAll
+// everything (the refinement walk's registers weigh only on that one).
+template <int kMode>
 __device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, Shared& sh, float4* ws) {
+    constexpr bool kRefine = kMode == kModeAll;
     const int lane = threadIdx.x;
     const int tile_blocks = job.tile_blocks;
     const uint64_t bi = first + lane;
@@ -268,6 +296,17 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, S
     Code out{0u, 0u};
     bool written = false;
     float err = FLT_MAX;
+    if constexpr (kMode == kModeBox) {
+        if (!active) return;
+        if (p.box) {
+            err = tx.w ? encode_box<true>(tx, p.lsq, cw, dec, sh.unit, out)
+                       : encode_box<false>(tx, p.lsq, cw, dec, sh.unit, out);
+            job.codes[bi] = make_uint2(out.cols, out.idx);
+        }
+        if (job.errors) job.errors[bi] = err;
+        if (job.stats) job.stats[bi] = make_int4(16, -1, 0, 0);
+        return;
+    }
     int n = 16, n3 = -1, steps = 0;
     f3 rc[16];
     float rw[16];
@@ -393,9 +432,12 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, S
 // ---------------------------------------------------------------------------
 // kernels
 
+template <int kMode>
 __device__ __forceinline__ void load_tables(Shared& sh) {
-    for (int i = threadIdx.x; i < 976; i += kTile) sh.four[i] = k_four[i];
-    for (int i = threadIdx.x; i < 160; i += kTile) sh.three[i] = k_three[i];
+    if (kMode != kModeBox) {
+        for (int i = threadIdx.x; i < 976; i += kTile) sh.four[i] = k_four[i];
+        for (int i = threadIdx.x; i < 160; i += kTile) sh.three[i] = k_three[i];
+    }
     for (int i = threadIdx.x; i < 256; i += kTile) sh.unit[i] = k_u8_to_unit[i];
     __syncthreads();
 }
@@ -404,23 +446,27 @@ __device__ __forceinline__ void load_tables(Shared& sh) {
 // tiles are taken from a queue in list order (the batched path); without, a
 // workgroup takes tiles blockIdx.x, blockIdx.x + gridDim.x, ...  With `tiles`
 // null the launch covers one job, tile k = blocks [64 k, 64 k + 64).
-// Only the kRefine instance holds the level-9 refinement walk, whose register
-// demand would otherwise weigh on every level.
-template <bool kRefine>
+template <int kMode>
 __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) void bc1_encode_kernel(
     const JobDev* __restrict__ jobs, const uint2* __restrict__ tiles, uint32_t ntiles,
     uint32_t* __restrict__ counter, float4* __restrict__ workspace) {
     __shared__ Shared sh;
-    load_tables(sh);
+    load_tables<kMode>(sh);
     float4* ws = workspace + (size_t)blockIdx.x * (kWorkspaceBytesPerWorkgroup / sizeof(float4));
+    // queue of this workgroup's XCD: its share is tiles x, x + 8, ...; the
+    // first claim of each workgroup is static, later ones come from the counter
+    const uint32_t x = blockIdx.x % kQueues;
+    const uint32_t share = ntiles > x ? (ntiles - x + kQueues - 1) / kQueues : 0;
+    const uint32_t static_claims = (gridDim.x - x + kQueues - 1) / kQueues;
+    uint32_t* queue = counter ? counter + x * (kQueueStride / sizeof(uint32_t)) : nullptr;
+    uint32_t j = blockIdx.x / kQueues;
     uint32_t t = blockIdx.x;
     for (;;) {
-        if (counter) {
-            uint32_t v = 0;
-            if (threadIdx.x == 0) v = atomicAdd(counter, 1u);
-            t = __builtin_amdgcn_readfirstlane(__shfl(v, 0, 64));
+        if (queue) {
+            if (j >= share) break;  // every wave reaches this once its XCD's share is taken
+            t = x + j * kQueues;
         }
-        if (t >= ntiles) break;  // every wave reaches this once the tiles run out
+        if (t >= ntiles) break;
         uint32_t job = 0;
         uint64_t first = (uint64_t)t * (uint32_t)jobs[0].tile_blocks;
         if (tiles) {
@@ -428,12 +474,22 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) v
             job = tile.x;
             first = tile.y;
         }
-        encode_tile<kRefine>(jobs[job], first, sh, ws);
-        if (!counter) t += gridDim.x;
+        encode_tile<kMode>(jobs[job], first, sh, ws);
+        if (queue) {
+            uint32_t v = 0;
+            if (threadIdx.x == 0) v = atomicAdd(queue, 1u);
+            j = static_claims + __builtin_amdgcn_readfirstlane(__shfl(v, 0, 64));
+        } else {
+            t += gridDim.x;
+        }
     }
 }
 
-__global__ void reset_counter_kernel(uint32_t* counter) { *counter = 0u; }
+// Zeroes `n` words `stride` words apart (queue counters; a kernel rather
+// than a memset node, so that captured graphs replay it correctly).
+__global__ void zero_words_kernel(uint32_t* p, uint32_t n, uint32_t stride) {
+    if (threadIdx.x < n) p[threadIdx.x * stride] = 0u;
+}
 
 // Counts the float bit patterns where recip_rn differs from 1.0f / b (NaN
 // results compare equal to each other, zeros by sign).
@@ -523,35 +579,41 @@ __global__ void bc1_evaluate_kernel(const uint32_t* __restrict__ rgba, const uin
 // launch glue
 
 int encode_workgroups_per_cu() {
-    int plain = 0, refine = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&plain, bc1_encode_kernel<false>, kTile, 0) != hipSuccess)
-        plain = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&refine, bc1_encode_kernel<true>, kTile, 0) != hipSuccess)
-        refine = 1;
-    return std::max(1, std::min(plain, refine));
+    int box = 0, cluster = 0, all = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&box, bc1_encode_kernel<kModeBox>, kTile, 0) != hipSuccess)
+        box = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&cluster, bc1_encode_kernel<kModeCluster>, kTile, 0) !=
+        hipSuccess)
+        cluster = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&all, bc1_encode_kernel<kModeAll>, kTile, 0) != hipSuccess)
+        all = 1;
+    return std::max(1, std::min(box, std::min(cluster, all)));
 }
 
 hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, uint32_t* counter,
-                         float4* workspace, uint32_t grid, bool refine, hipStream_t stream) {
+                         float4* workspace, uint32_t grid, int mode, hipStream_t stream) {
     if (ntiles == 0) return hipSuccess;
     if (grid > ntiles) grid = ntiles;
     if (counter) {
-        hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
+        hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, counter, kQueues,
+                           (uint32_t)(kQueueStride / sizeof(uint32_t)));
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    if (refine)
-        hipLaunchKernelGGL(bc1_encode_kernel<true>, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles, counter,
-                           workspace);
+    if (mode == kModeAll)
+        hipLaunchKernelGGL(bc1_encode_kernel<kModeAll>, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles,
+                           counter, workspace);
+    else if (mode == kModeCluster)
+        hipLaunchKernelGGL(bc1_encode_kernel<kModeCluster>, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles,
+                           counter, workspace);
     else
-        hipLaunchKernelGGL(bc1_encode_kernel<false>, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles,
+        hipLaunchKernelGGL(bc1_encode_kernel<kModeBox>, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles,
                            counter, workspace);
     return hipGetLastError();
 }
 
 hipError_t launch_check_reciprocal(unsigned long long* mismatches, hipStream_t stream) {
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, (uint32_t*)mismatches);
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, (uint32_t*)mismatches + 1);
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, (uint32_t*)mismatches, 2u, 1u);
     hipLaunchKernelGGL(bc1_check_reciprocal_kernel, dim3(4096), dim3(256), 0, stream, mismatches);
     return hipGetLastError();
 }

@@ -46,28 +46,50 @@ int encode_workgroups_per_cu();
 //   tiles != null: tile k covers blocks [tiles[k].y, +T) of job tiles[k].x
 //   tiles == null: one job (jobs[0]), tile k covers blocks [T k, T k + T)
 // where T is the job's tile_blocks.
-//   counter != null: tiles are handed out in order from a queue (reset by this
-//   call); otherwise workgroup g takes tiles g, g + grid, ...
+//   counter != null (kQueueBytes of queue counters): workgroup g takes tile g,
+//   then claims further tiles of its XCD's share from that XCD's queue (reset
+//   by this call); otherwise workgroup g takes tiles g, g + grid, ...
 // `workspace` holds kWorkspaceBytesPerWorkgroup bytes per workgroup of `grid`.
-// Per workgroup: two SAT slices (4- and 3-cluster fits) of 64 blocks x 17
+// Per workgroup: two SAT slices (4- and 3-cluster fits) of 64 blocks x 18
 // rows, and the tile's reduced colours (64 x 16 float4).
 constexpr size_t kWorkspaceBytesPerWorkgroup = ((size_t)2 * kTile * kSatRows + (size_t)kTile * 16) * 16;
-// `refine` selects the kernel instance that holds the level-9 refinement
-// walk; it must be set when any job of the launch is at level 9.
+
+// Tile queues: one counter per XCD (workgroups are dispatched round-robin
+// over the 8 XCDs, and XCD x takes tiles x, x + 8, x + 16, ... of the list),
+// each in its own 128-byte line, ahead of the workgroup slices.
+constexpr uint32_t kQueues = 8;
+constexpr size_t kQueueStride = 128;
+constexpr size_t kQueueBytes = kQueues * kQueueStride;
+
+// Kernel instances: every job of a kModeBox launch is at level 0 or 1 (or an
+// invalid level); kModeCluster covers levels 0-8, kModeAll also level 9.
+constexpr int kModeBox = 0;
+constexpr int kModeCluster = 1;
+constexpr int kModeAll = 2;
+
+// `mode` selects the kernel instance; encode_mode() gives the one a set of
+// jobs needs.
 hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, uint32_t* counter,
-                         float4* workspace, uint32_t grid, bool refine, hipStream_t stream);
+                         float4* workspace, uint32_t grid, int mode, hipStream_t stream);
+
+inline int encode_mode_of_level(int level) {
+    return level == 9 ? kModeAll : (level >= 2 && level <= 8) ? kModeCluster : kModeBox;
+}
+
+// The kernel instance a launch over these (host-side) jobs needs.
+inline int encode_mode(const JobDev* jobs, size_t n) {
+    int mode = kModeBox;
+    for (size_t k = 0; k < n; k++) {
+        const int m = encode_mode_of_level(jobs[k].level);
+        if (m > mode) mode = m;
+    }
+    return mode;
+}
 
 // The one-workgroup-per-tile schedule gives every tile its own workspace
 // slice; tuned defaults use it only for launches of at most this many waves
 // of resident workgroups.
 constexpr uint64_t kGridScheduleMaxWaves = 4;
-
-// Whether a launch over these (host-side) jobs needs the refining instance.
-inline bool jobs_need_refine(const JobDev* jobs, size_t n) {
-    for (size_t k = 0; k < n; k++)
-        if (jobs[k].level == 9) return true;
-    return false;
-}
 
 // Counts (into *mismatches, zeroed first) the 2^32 float inputs where the
 // encoder's fast reciprocal differs from IEEE-754 division on this device.

@@ -203,16 +203,16 @@ int resident_workgroups(uint32_t* grid) {
     return ICBC_OK;
 }
 
-// Stream-ordered workspace for one encode launch: `grid` workgroup slices
-// and a tile-queue counter.
+// Stream-ordered workspace for one encode launch: the tile-queue counters
+// and `grid` workgroup slices.
 struct Workspace {
     void* base = nullptr;
-    float4* slices() const { return (float4*)((char*)base + 256); }
+    float4* slices() const { return (float4*)((char*)base + kQueueBytes); }
     uint32_t* counter() const { return (uint32_t*)base; }
 };
 
 int workspace_alloc(uint32_t grid, hipStream_t s, Workspace& w) {
-    HIP_TRY(hipMallocAsync(&w.base, 256 + (size_t)grid * kWorkspaceBytesPerWorkgroup, s));
+    HIP_TRY(hipMallocAsync(&w.base, kQueueBytes + (size_t)grid * kWorkspaceBytesPerWorkgroup, s));
     return ICBC_OK;
 }
 
@@ -680,7 +680,7 @@ int icbc_gpu_compress_bc1_batch(const icbc_bc1_job* jobs, size_t n_jobs, void* s
         r = workspace_alloc(grid, s, w);
         if (r) return r;
         HIP_TRY(launch_encode((const JobDev*)djobs, (const uint2*)dtiles, (uint32_t)tiles.size(), w.counter(),
-                              w.slices(), grid, jobs_need_refine(p.dev.data(), p.dev.size()), s));
+                              w.slices(), grid, encode_mode(p.dev.data(), p.dev.size()), s));
         HIP_TRY(hipFreeAsync(w.base, s));
         HIP_TRY(hipFreeAsync(djobs, s));
         HIP_TRY(hipFreeAsync(dtiles, s));
@@ -712,7 +712,7 @@ int icbc_gpu_compress_bc1_plain(const icbc_bc1_job* jobs, size_t n_jobs, void* s
         if (r) return r;
         for (size_t k = 0; k < n_jobs; k++)
             HIP_TRY(launch_encode((const JobDev*)djobs + k, nullptr, tiles_of(p.dev[k].block_count, p.dev[k].tile_blocks),
-                                  nullptr, w.slices(), grid, jobs_need_refine(&p.dev[k], 1), s));
+                                  nullptr, w.slices(), grid, encode_mode(&p.dev[k], 1), s));
         HIP_TRY(hipFreeAsync(w.base, s));
         HIP_TRY(hipFreeAsync(djobs, s));
     }
@@ -728,12 +728,12 @@ struct icbc_bc1_plan {
     icbc_plan_options opt{};
     JobDev* djobs = nullptr;
     uint2* dtiles = nullptr;
-    void* dwork = nullptr;  // tile-queue counter (256 B) + per-workgroup slices
+    void* dwork = nullptr;  // tile-queue counters (kQueueBytes) + per-workgroup slices
     std::vector<SourceConvert> converts;  // RGBA8 / image jobs, re-read on every execute
     std::vector<void*> convert_bufs;
     uint32_t ntiles = 0;
     uint32_t grid = 0;
-    bool refine = false;  // some job is at level 9
+    int mode = kModeBox;  // kernel instance (encode_mode)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -743,8 +743,8 @@ namespace {
 int plan_launch(icbc_bc1_plan* pl, hipStream_t s) {
     for (const auto& c : pl->converts) HIP_TRY(run_convert(c, s));
     uint32_t* counter = pl->opt.schedule == ICBC_SCHEDULE_PERSISTENT ? (uint32_t*)pl->dwork : nullptr;
-    float4* slices = (float4*)((char*)pl->dwork + 256);
-    HIP_TRY(launch_encode(pl->djobs, pl->dtiles, pl->ntiles, counter, slices, pl->grid, pl->refine, s));
+    float4* slices = (float4*)((char*)pl->dwork + kQueueBytes);
+    HIP_TRY(launch_encode(pl->djobs, pl->dtiles, pl->ntiles, counter, slices, pl->grid, pl->mode, s));
     return ICBC_OK;
 }
 
@@ -799,7 +799,7 @@ int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_pla
     icbc_bc1_plan* pl = new icbc_bc1_plan();
     pl->opt = o;
     pl->ntiles = (uint32_t)tiles.size();
-    pl->refine = jobs_need_refine(dev.data(), dev.size());
+    pl->mode = encode_mode(dev.data(), dev.size());
     (void)hipGetDevice(&pl->device);
     hipDeviceProp_t prop;
     hipError_t e = hipGetDeviceProperties(&prop, pl->device);
@@ -822,7 +822,7 @@ int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_pla
     };
     if ((e = hipMalloc(&pl->djobs, std::max<size_t>(1, dev.size()) * sizeof(JobDev))) != hipSuccess ||
         (e = hipMalloc(&pl->dtiles, std::max<size_t>(1, tiles.size()) * sizeof(uint2))) != hipSuccess ||
-        (e = hipMalloc(&pl->dwork, 256 + (size_t)pl->grid * kWorkspaceBytesPerWorkgroup)) != hipSuccess)
+        (e = hipMalloc(&pl->dwork, kQueueBytes + (size_t)pl->grid * kWorkspaceBytesPerWorkgroup)) != hipSuccess)
         return cleanup(hip_fail(e, "hipMalloc (plan)"));
     for (size_t k = 0; k < n_jobs; k++) {
         if (!needs_convert(jobs[k])) continue;
@@ -953,7 +953,7 @@ int compress_one(int level, const float* input_colors, const float* input_weight
     static_assert(sizeof(JobDev) <= 128, "job descriptor must fit the block buffer");
     memcpy(h + 384, &jd, sizeof jd);
     HIP_TRY(hipMemcpyAsync(d, h, 512, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(launch_encode((const JobDev*)(d + 384), nullptr, 1, nullptr, (float4*)(d + 768), 1, level == 9,
+    HIP_TRY(launch_encode((const JobDev*)(d + 384), nullptr, 1, nullptr, (float4*)(d + 768), 1, encode_mode_of_level(level),
                           ctx->stream));
     HIP_TRY(hipMemcpyAsync(h + 320, d + 320, 32, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));

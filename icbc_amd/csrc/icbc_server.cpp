@@ -251,7 +251,7 @@ int icbc_server::launch(Slot& s, std::vector<Request*>& reqs) {
     if (hipMemcpyAsync(s.table, s.host_table, need, hipMemcpyHostToDevice, s.stream) != hipSuccess)
         return ICBC_ERR_DEVICE;
     if (launch_encode((const JobDev*)s.table, (const uint2*)((char*)s.table + jbytes), (uint32_t)tiles.size(),
-                      (uint32_t*)s.work, (float4*)((char*)s.work + 256), grid, jobs_need_refine(dev.data(), dev.size()),
+                      (uint32_t*)s.work, (float4*)((char*)s.work + kQueueBytes), grid, encode_mode(dev.data(), dev.size()),
                       s.stream) != hipSuccess)
         return ICBC_ERR_DEVICE;
     // results go to pinned staging (a copy into pageable memory would block
@@ -453,7 +453,7 @@ int icbc_server_create(const icbc_server_options* opt, icbc_server** out) {
     for (Slot& s : sv->slots) {
         if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
-            hipMalloc(&s.work, 256 + (size_t)grid * kWorkspaceBytesPerWorkgroup) != hipSuccess) {
+            hipMalloc(&s.work, kQueueBytes + (size_t)grid * kWorkspaceBytesPerWorkgroup) != hipSuccess) {
             icbc_server_destroy(sv);
             return icbc_internal_fail(ICBC_ERR_OUT_OF_MEMORY, "server launch slot allocation failed");
         }
