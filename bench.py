@@ -35,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "BC1 Mblocks/s at level 8 (bit-exact), batched vs plain GPU; p99 request latency"
+PEAK_HBM_GBPS = 8000.0
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector, dense (AMD spec)
 PEAK_FP32_NOFMA_TFLOPS = 78.65  # one unfused FP32 op per lane per clock (no FMA in the bit-exact build)
 # algorithmic FP32 operations per unit of work (divisions count 1), from the
@@ -226,6 +227,53 @@ def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64
             "input": "RGBA8 texels (64 B/block) in page-locked host memory, copied per request"}
 
 
+def single_image_levels(g, synth, levels=(1, 5, 9), steps=10, seed=7):
+    """The 4096x4096 single-image configuration: one plan per level over its
+    1,048,576 blocks (40/30/20/10 gradient / white noise / near-black / flat,
+    three-colour mode and transparent black on), timed with HIP events; level
+    1 against the HBM roofline (268 B per block: texels in, code and error
+    out), the cluster levels against the FP32 one."""
+    colors, _ = synth.make_image_blocks(seed, 4096, 4096, synth.MIX_DEFAULT)
+    n = len(colors)
+    dcol = g.DeviceBuffer.from_array(colors)
+    dcode, derr, dstat = g.DeviceBuffer(8 * n), g.DeviceBuffer(4 * n), g.DeviceBuffer(16 * n)
+    s = g.Stream()
+    e0, e1 = g.Event(), g.Event()
+    out = {}
+    try:
+        for lv in levels:
+            job = g.make_job(lv, n, dcol, dcode, None, derr, None, (1, 1, 1), True, True)
+            plan = g.Plan([job], stream=s)
+            for _ in range(2):
+                plan.execute(s)
+            e0.record(s)
+            for _ in range(steps):
+                plan.execute(s)
+            e1.record(s)
+            e1.synchronize()
+            ms = e0.elapsed_ms(e1) / steps
+            plan.close()
+            r = {"ms": round(ms, 4), "mblocks_per_s": round(n / ms / 1e3, 1)}
+            if lv <= 1:
+                gbps = n * (256 + 8 + 4) / (ms * 1e-3) / 1e9
+                r.update({"bound": "hbm", "achieved_gb_per_s": round(gbps, 1), "peak_gb_per_s": PEAK_HBM_GBPS,
+                          "frac": round(gbps / PEAK_HBM_GBPS, 4)})
+            else:
+                g.compress_batch([g.make_job(lv, n, dcol, dcode, None, derr, dstat, (1, 1, 1), True, True)],
+                                 stream=s)
+                s.synchronize()
+                tf = algorithmic_flops(dstat.download(np.int32, (n, 4))) / (ms * 1e-3) / 1e12
+                r.update({"bound": "valu-fp32", "achieved_tflops": round(tf, 3), "peak_tflops": PEAK_FP32_TFLOPS,
+                          "frac": round(tf / PEAK_FP32_TFLOPS, 4)})
+            out[str(lv)] = r
+            log(f"4096x4096 level {lv}: {r}")
+    finally:
+        for b in (dcol, dcode, derr, dstat):
+            b.close()
+    return {"workload": "one 4096x4096 image, 1,048,576 blocks, seed 7, three_color_mode+three_color_black",
+            "path": "plan (graph-captured launch) per level", "levels": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,6 +291,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the short serving run included in the default benchmark")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-levels", action="store_true",
+                    help="skip the 4096x4096 single-image run at levels 1, 5 and 9")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the per-level autotuning pass before the batched plan is built")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -384,6 +434,8 @@ def main():
         cpu = cpu_baseline(colors, sample_lv, tcm, tcb, threads)
         log(f"cpu baseline: {cpu}")
 
+    levels_run = None if args.no_levels else single_image_levels(g, synth)
+
     serving = None
     if not args.no_serve:
         serving = serving_bench(g, synth, args.serve_seconds, (0.8,))
@@ -407,6 +459,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "serving": serving,
+        "single_image_levels": levels_run,
     }
     print(json.dumps(out), flush=True)
     plan.close()

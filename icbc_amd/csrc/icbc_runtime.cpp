@@ -674,8 +674,11 @@ int icbc_gpu_compress_bc1_batch(const icbc_bc1_job* jobs, size_t n_jobs, void* s
         void *djobs = nullptr, *dtiles = nullptr;
         r = upload_blob(p.dev.data(), p.dev.size() * sizeof(JobDev), s, &djobs);
         if (r) return r;
-        r = upload_blob(tiles.data(), tiles.size() * sizeof(uint2), s, &dtiles);
-        if (r) return r;
+        // one job: its tiles are implied (tile k = blocks [T k, T k + T)), no list
+        if (n_jobs > 1) {
+            r = upload_blob(tiles.data(), tiles.size() * sizeof(uint2), s, &dtiles);
+            if (r) return r;
+        }
         Workspace w;
         r = workspace_alloc(grid, s, w);
         if (r) return r;
@@ -683,7 +686,7 @@ int icbc_gpu_compress_bc1_batch(const icbc_bc1_job* jobs, size_t n_jobs, void* s
                               w.slices(), grid, encode_mode(p.dev.data(), p.dev.size()), s));
         HIP_TRY(hipFreeAsync(w.base, s));
         HIP_TRY(hipFreeAsync(djobs, s));
-        HIP_TRY(hipFreeAsync(dtiles, s));
+        if (dtiles) HIP_TRY(hipFreeAsync(dtiles, s));
     }
     return finish(jobs, n_jobs, s, p);
 }
