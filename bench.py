@@ -274,6 +274,83 @@ def single_image_levels(g, synth, levels=(1, 5, 9), steps=10, seed=7):
             "path": "plan (graph-captured launch) per level", "levels": out}
 
 
+def mixed_level_batch(g, jobs_for, counts, texs, stream, steps=3, seed=11):
+    """The batching configuration with its per-texture quality drawn from 1-9:
+    the same textures and mips, an autotuned persistent graph launch against
+    one launch per image."""
+    rng = np.random.default_rng(seed)
+    tex_level = rng.integers(1, 10, size=max(t[0] for t in texs) + 1)
+    levels = [int(tex_level[t[0]]) for t in texs]
+    jobs = jobs_for(levels)
+    take, acc = [], 0
+    for j, n in zip(jobs, counts):
+        take.append(j)
+        acc += int(n)
+        if acc >= (1 << 20):
+            break
+    tuned = g.autotune(take, levels=sorted(set(levels)))
+    plan = g.Plan(jobs, stream=stream)
+    e = [g.Event() for _ in range(2)]
+
+    def timed(run):
+        run()
+        stream.synchronize()
+        e[0].record(stream)
+        for _ in range(steps):
+            run()
+        e[1].record(stream)
+        e[1].synchronize()
+        return e[0].elapsed_ms(e[1]) / steps
+
+    ms_b = timed(lambda: plan.execute(stream))
+    plan.close()
+    ms_p = timed(lambda: g.compress_batch(jobs, stream=stream, plain=True))
+    total = int(np.sum(counts))
+    blocks_per_level = {str(lv): int(sum(int(n) for n, l in zip(counts, levels) if l == lv)) for lv in range(1, 10)}
+    out = {"workload": "same 256 textures + mips, level drawn uniformly from 1-9 per texture (seed 11)",
+           "blocks": total, "blocks_per_level": blocks_per_level,
+           "batched_ms": round(ms_b, 3), "batched_mblocks_per_s": round(total / ms_b / 1e3, 2),
+           "plain_ms": round(ms_p, 3), "plain_mblocks_per_s": round(total / ms_p / 1e3, 2),
+           "batched_over_plain": round(ms_p / ms_b, 3),
+           "tuning": {str(k): v for k, v in tuned.as_dict().items()}}
+    log(f"mixed levels: {out}")
+    return out
+
+
+def weighted_level9(g, synth, steps=5, seed=4):
+    """The weighted configuration: a 2048x2048 RGBA float image whose alpha (30%
+    of texels at 0) is each texel's weight, level 9, channel weights (0.3,
+    0.59, 0.11), three-colour mode and transparent black on; blocks gathered
+    on the device from the image (ICBC_JOB_IMAGE + weights from alpha)."""
+    img = synth.make_image(seed, 2048, 2048, synth.MIX_DEFAULT)
+    img[..., 3] = synth.make_weighted_alpha(seed, 2048, 2048)
+    di = g.DeviceImage(img)
+    n = di.block_count
+    dcode, derr = g.DeviceBuffer(8 * n), g.DeviceBuffer(4 * n)
+    s = g.Stream()
+    e0, e1 = g.Event(), g.Event()
+    try:
+        job = g.make_job(9, n, None, dcode, None, derr, None, (0.3, 0.59, 0.11), True, True, image=di.desc,
+                         weights_from_alpha=True)
+        plan = g.Plan([job], stream=s)
+        plan.execute(s)
+        e0.record(s)
+        for _ in range(steps):
+            plan.execute(s)
+        e1.record(s)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / steps
+        plan.close()
+    finally:
+        for b in (dcode, derr, di):
+            b.close()
+    out = {"workload": "2048x2048 RGBA, alpha = weight (30% zero), level 9, cw (0.3, 0.59, 0.11), "
+                       "gathered on the device", "blocks": n, "ms": round(ms, 4),
+           "mblocks_per_s": round(n / ms / 1e3, 2)}
+    log(f"weighted level 9: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,6 +368,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the short serving run included in the default benchmark")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-level (1-9) batched-vs-plain run on the same textures")
     ap.add_argument("--no-levels", action="store_true",
                     help="skip the 4096x4096 single-image run at levels 1, 5 and 9")
     ap.add_argument("--no-autotune", action="store_true",
@@ -434,7 +513,18 @@ def main():
         cpu = cpu_baseline(colors, sample_lv, tcm, tcb, threads)
         log(f"cpu baseline: {cpu}")
 
-    levels_run = None if args.no_levels else single_image_levels(g, synth)
+    levels_run = None
+    weighted = None
+    if not args.no_levels:
+        levels_run = single_image_levels(g, synth)
+        weighted = weighted_level9(g, synth)
+    mixed = None
+    if not args.no_mixed and not args.mixed_levels:
+        def jobs_for(levels):
+            return [g.make_job(lv, int(n), dcol.ptr + 256 * int(o), dcode.ptr + 8 * int(o), None,
+                               derr.ptr + 4 * int(o), None, (1, 1, 1), tcm, tcb)
+                    for lv, n, o in zip(levels, counts, offsets[:-1])]
+        mixed = mixed_level_batch(g, jobs_for, counts, texs, stream)
 
     serving = None
     if not args.no_serve:
@@ -460,6 +550,8 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
         "single_image_levels": levels_run,
+        "weighted_level9": weighted,
+        "mixed_level_batch": mixed,
     }
     print(json.dumps(out), flush=True)
     plan.close()
