@@ -117,49 +117,65 @@ __device__ __forceinline__ int interp_ch(int decoder, bool four, int p0, int p1)
     return (p0 + p1) / 2;
 }
 
-__device__ __forceinline__ void palette8(uint32_t c0, uint32_t c1, int decoder, Pal8& p) {
-    expand565(c0, p.r[0], p.g[0], p.b[0]);
-    expand565(c1, p.r[1], p.g[1], p.b[1]);
+// The four entries of a palette as named scalars (no array, so the compiler
+// keeps them in registers on every decoder path).
+struct PalInts {
+    int r0, g0, b0, r1, g1, b1, r2, g2, b2, r3, g3, b3;
+};
+
+__device__ __forceinline__ PalInts palette_ints(uint32_t c0, uint32_t c1, int decoder) {
+    PalInts p;
+    expand565(c0, p.r0, p.g0, p.b0);
+    expand565(c1, p.r1, p.g1, p.b1);
     const bool four = c0 > c1;
+    p.r3 = p.g3 = p.b3 = 0;
     if (decoder == 1) {
         // NVIDIA: red and blue interpolate the 5-bit values, green the 8-bit ones.
-        int r0 = (c0 >> 11) & 31, b0 = c0 & 31, r1 = (c1 >> 11) & 31, b1 = c1 & 31;
-        int gd = p.g[1] - p.g[0];
+        const int r0 = (c0 >> 11) & 31, b0 = c0 & 31, r1 = (c1 >> 11) & 31, b1 = c1 & 31;
+        const int gd = p.g1 - p.g0;
         if (four) {
-            p.r[2] = (uint8_t)(((2 * r0 + r1) * 22) / 8);
-            p.g[2] = (uint8_t)((256 * p.g[0] + gd / 4 + 128 + gd * 80) / 256);
-            p.b[2] = (uint8_t)(((2 * b0 + b1) * 22) / 8);
-            p.r[3] = (uint8_t)(((2 * r1 + r0) * 22) / 8);
-            p.g[3] = (uint8_t)((256 * p.g[1] - gd / 4 + 128 - gd * 80) / 256);
-            p.b[3] = (uint8_t)(((2 * b1 + b0) * 22) / 8);
+            p.r2 = (uint8_t)(((2 * r0 + r1) * 22) / 8);
+            p.g2 = (uint8_t)((256 * p.g0 + gd / 4 + 128 + gd * 80) / 256);
+            p.b2 = (uint8_t)(((2 * b0 + b1) * 22) / 8);
+            p.r3 = (uint8_t)(((2 * r1 + r0) * 22) / 8);
+            p.g3 = (uint8_t)((256 * p.g1 - gd / 4 + 128 - gd * 80) / 256);
+            p.b3 = (uint8_t)(((2 * b1 + b0) * 22) / 8);
         } else {
-            p.r[2] = (uint8_t)(((r0 + r1) * 33) / 8);
-            p.g[2] = (uint8_t)((256 * p.g[0] + gd / 4 + 128 + gd * 128) / 256);
-            p.b[2] = (uint8_t)(((b0 + b1) * 33) / 8);
-            p.r[3] = p.g[3] = p.b[3] = 0;
+            p.r2 = (uint8_t)(((r0 + r1) * 33) / 8);
+            p.g2 = (uint8_t)((256 * p.g0 + gd / 4 + 128 + gd * 128) / 256);
+            p.b2 = (uint8_t)(((b0 + b1) * 33) / 8);
         }
-        return;
+        return p;
     }
-    p.r[2] = interp_ch(decoder, four, p.r[0], p.r[1]);
-    p.g[2] = interp_ch(decoder, four, p.g[0], p.g[1]);
-    p.b[2] = interp_ch(decoder, four, p.b[0], p.b[1]);
+    p.r2 = interp_ch(decoder, four, p.r0, p.r1);
+    p.g2 = interp_ch(decoder, four, p.g0, p.g1);
+    p.b2 = interp_ch(decoder, four, p.b0, p.b1);
     if (four) {
-        p.r[3] = interp_ch(decoder, true, p.r[1], p.r[0]);
-        p.g[3] = interp_ch(decoder, true, p.g[1], p.g[0]);
-        p.b[3] = interp_ch(decoder, true, p.b[1], p.b[0]);
-    } else {
-        p.r[3] = p.g[3] = p.b[3] = 0;
+        p.r3 = interp_ch(decoder, true, p.r1, p.r0);
+        p.g3 = interp_ch(decoder, true, p.g1, p.g0);
+        p.b3 = interp_ch(decoder, true, p.b1, p.b0);
     }
+    return p;
+}
+
+// Array form for the decode / evaluate kernels, which index entries per texel.
+__device__ __forceinline__ void palette8(uint32_t c0, uint32_t c1, int decoder, Pal8& p) {
+    const PalInts q = palette_ints(c0, c1, decoder);
+    p.r[0] = q.r0; p.g[0] = q.g0; p.b[0] = q.b0;
+    p.r[1] = q.r1; p.g[1] = q.g1; p.b[1] = q.b1;
+    p.r[2] = q.r2; p.g[2] = q.g2; p.b[2] = q.b2;
+    p.r[3] = q.r3; p.g[3] = q.g3; p.b[3] = q.b3;
 }
 
 // palette as floats: channel / 255.0f (icbc.h:1656, 2902-2909), from a table
 // holding exactly those quotients.
 // `unit` is the 256-entry table of k / 255.0f (an LDS copy in the encoder).
 __device__ __forceinline__ void palette_f(uint32_t c0, uint32_t c1, int decoder, const float* unit, f3 pal[4]) {
-    Pal8 p;
-    palette8(c0, c1, decoder, p);
-#pragma unroll
-    for (int i = 0; i < 4; i++) pal[i] = mk(unit[p.r[i]], unit[p.g[i]], unit[p.b[i]]);
+    const PalInts p = palette_ints(c0, c1, decoder);
+    pal[0] = mk(unit[p.r0], unit[p.g0], unit[p.b0]);
+    pal[1] = mk(unit[p.r1], unit[p.g1], unit[p.b1]);
+    pal[2] = mk(unit[p.r2], unit[p.g2], unit[p.b2]);
+    pal[3] = mk(unit[p.r3], unit[p.g3], unit[p.b3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -191,12 +207,15 @@ template <bool kWeighted>
 struct TexelRegs {
     f3 c[16];
     float w[kWeighted ? 16 : 1];
-    __device__ __forceinline__ explicit TexelRegs(const Texels& t) {
+    __device__ __forceinline__ TexelRegs() {}
+    __device__ __forceinline__ explicit TexelRegs(const Texels& t) { load(t); }
+    // With kWeighted, a block without weights holds 1.0 (w * x == x exactly).
+    __device__ __forceinline__ void load(const Texels& t) {
 #pragma unroll
         for (int i = 0; i < 16; i++) c[i] = t.color(i);
         if (kWeighted) {
 #pragma unroll
-            for (int i = 0; i < 16; i++) w[kWeighted ? i : 0] = t.w[i];
+            for (int i = 0; i < 16; i++) w[kWeighted ? i : 0] = t.w ? t.w[i] : 1.0f;
         }
     }
     __device__ __forceinline__ f3 color(int i) const { return c[i]; }
@@ -299,10 +318,11 @@ __device__ __forceinline__ float select3(const Texels& t, f3 w, bool allow_black
 // `pal` (the candidate block's palette).
 // Indices of the 16 texels against `sel` (palette of the current best block):
 // nearest entry under the colour weights, lowest index on ties.
-__device__ __forceinline__ uint32_t select_indices(const Texels& t, f3 w, const f3 sel[4]) {
+template <typename Src>
+__device__ __forceinline__ uint32_t select_indices(const Src& t, f3 w, const f3 sel[4]) {
     const f3 p0 = sel[0] * w, p1 = sel[1] * w, p2 = sel[2] * w, p3 = sel[3] * w;
     uint32_t out = 0;
-#pragma unroll 4
+#pragma unroll TexelUnroll<Src>::value
     for (int i = 0; i < 16; i++) {
         const f3 c = t.color(i) * w;
         const f3 e0 = c - p0, e1 = c - p1, e2 = c - p2, e3 = c - p3;
@@ -851,53 +871,6 @@ __device__ __forceinline__ void refine_step(int k, int& dr, int& dg, int& db) {
     dr = (int)((kPackR >> (2 * k)) & 3) - 1;
     dg = (int)((kPackG >> (2 * k)) & 3) - 1;
     db = (int)((kPackB >> (2 * k)) & 3) - 1;
-}
-
-template <bool kWeighted>
-__device__ __forceinline__ float refine_walk(const Texels& t, f3 cw, int decoder, const float* unit,
-                                             bool three_color_mode, float input_error, Code& out, int& steps) {
-    const TexelRegs<kWeighted> r(t);
-    float best = input_error;
-    int last = 0;
-    int i = 0;
-    // Every step takes its indices from the palette of the current best
-    // block, so they only change when a step is accepted.
-    f3 sel[4];
-    palette_f(col0(out), col1(out), decoder, unit, sel);
-    uint32_t idx = select_indices(t, cw, sel);
-    for (; i < 256; i++) {
-        int dr, dg, db;
-        refine_step(i & 15, dr, dg, db);
-        uint32_t c0 = col0(out), c1 = col1(out);
-        if ((i >> 4) & 1) c0 = nudge565(c0, dr, dg, db);
-        else c1 = nudge565(c1, dr, dg, db);
-        if (!three_color_mode) {
-            if (c0 == c1) c1 = nudge565(c1, 0, 1, 0);
-            if (c0 < c1) {
-                uint32_t tmp = c0;
-                c0 = c1;
-                c1 = tmp;
-            }
-        }
-        f3 pal[4];
-        palette_f(c0, c1, decoder, unit, pal);
-        const float e = indexed_error(r, cw, pal, idx);
-        if (e < best) {
-            best = e;
-            out = make_code(c0, c1, idx);
-            last = i;
-            idx = select_indices(t, cw, pal);
-        }
-        if (i - last > 32) break;
-    }
-    steps = (i < 256) ? i + 1 : 256;
-    return best;
-}
-
-__device__ __forceinline__ float refine(const Texels& t, f3 cw, int decoder, const float* unit,
-                                        bool three_color_mode, float input_error, Code& out, int& steps) {
-    if (t.w) return refine_walk<true>(t, cw, decoder, unit, three_color_mode, input_error, out, steps);
-    return refine_walk<false>(t, cw, decoder, unit, three_color_mode, input_error, out, steps);
 }
 
 // ---------------------------------------------------------------------------

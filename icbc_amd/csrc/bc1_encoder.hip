@@ -15,6 +15,9 @@
 // below FLT_MAX (NaN never wins); with no such candidate both endpoints are 0.
 
 #include <algorithm>
+#include <mutex>
+#include <utility>
+#include <vector>
 
 #include "bc1_device.h"
 #include "bc1_kernels.h"
@@ -279,8 +282,8 @@ __device__ __forceinline__ float encode_box(const Texels& tx, bool lsq, f3 cw, i
 // 0-1 (or invalid levels), kModeCluster launches without level 9, kModeAll
 // everything (the refinement walk's registers weigh only on that one).
 template <int kMode>
-__device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, Shared& sh, float4* ws) {
-    constexpr bool kRefine = kMode == kModeAll;
+__device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_index, uint64_t first, Shared& sh,
+                                            float4* ws, const RefineList& rl) {
     const int lane = threadIdx.x;
     const int tile_blocks = job.tile_blocks;
     const uint64_t bi = first + lane;
@@ -412,14 +415,20 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint64_t first, S
         __syncthreads();  // workspace rows are reused by the next tile
     }
 
-    if (kRefine && p.refine && !done) {  // icbc.h:3803-3805
-        if (!written) {
-            // the reference refines whatever the output block holds
-            const uint2 v = job.codes[bi];
-            out = Code{v.x, v.y};
-            written = true;
+    // level 9 (icbc.h:3803-3805): the block goes to the refinement kernel,
+    // which starts from the code written here (or, if no fit was written, the
+    // one the output buffer held, as the reference refines whatever it holds)
+    const bool hand_off = p.refine && !done;
+    const uint64_t hand_mask = __ballot(hand_off);
+    if (hand_mask) {
+        uint32_t at = 0;
+        if (lane == 0) at = atomicAdd(rl.counters, (uint32_t)__popcll(hand_mask));
+        at = __builtin_amdgcn_readfirstlane(__shfl(at, 0, 64));
+        if (hand_off) {
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(hand_mask >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)hand_mask, 0));
+            rl.items[at + rank] = RefineItem{job_index, (uint32_t)bi, err, 0u};
         }
-        err = refine(tx, cw, dec, sh.unit, job.tcm != 0, err, out, steps);
     }
 
     if (active) {
@@ -449,7 +458,7 @@ __device__ __forceinline__ void load_tables(Shared& sh) {
 template <int kMode>
 __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) void bc1_encode_kernel(
     const JobDev* __restrict__ jobs, const uint2* __restrict__ tiles, uint32_t ntiles,
-    uint32_t* __restrict__ counter, float4* __restrict__ workspace) {
+    uint32_t* __restrict__ counter, float4* __restrict__ workspace, RefineList refine) {
     __shared__ Shared sh;
     load_tables<kMode>(sh);
     float4* ws = workspace + (size_t)blockIdx.x * (kWorkspaceBytesPerWorkgroup / sizeof(float4));
@@ -474,7 +483,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) v
             job = tile.x;
             first = tile.y;
         }
-        encode_tile<kMode>(jobs[job], first, sh, ws);
+        encode_tile<kMode>(jobs[job], job, first, sh, ws, refine);
         if (queue) {
             uint32_t v = 0;
             if (threadIdx.x == 0) v = atomicAdd(queue, 1u);
@@ -489,6 +498,144 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) v
 // than a memset node, so that captured graphs replay it correctly).
 __global__ void zero_words_kernel(uint32_t* p, uint32_t n, uint32_t stride) {
     if (threadIdx.x < n) p[threadIdx.x * stride] = 0u;
+}
+
+// refine_endpoints (icbc.h:3574-3651) for the level-9 blocks handed over by
+// the encode kernel.  Each step nudges one endpoint of the current best code,
+// takes the indices from the current best palette (they only change when a
+// step is accepted) and keeps the step if its error is strictly lower; a walk
+// ends 33 steps after its last improvement or after 256 steps.
+//
+// A walk runs on a group of K lanes (K a power of two, 1..16): since every
+// step until the next acceptance starts from the same best code, the group
+// evaluates steps i .. i+K-1 at once and keeps the first accepted one (the
+// later ones are discarded, as the sequential walk would never have tried
+// them from that code), or, with none accepted, moves on K steps -- never past
+// the step where the walk ends.  Results are those of the sequential walk.
+// K grows when there are few walks for the lanes, so short launches are not
+// bound by their longest walk.  A group whose walk has ended takes the next
+// block from the list (claims are gathered per wavefront).
+constexpr int kRefineThreads = 256;
+
+__global__ __launch_bounds__(kRefineThreads) void bc1_refine_kernel(const JobDev* __restrict__ jobs,
+                                                                     RefineList rl) {
+    __shared__ float unit[256];
+    for (int i = threadIdx.x; i < 256; i += kRefineThreads) unit[i] = k_u8_to_unit[i];
+    __syncthreads();
+    const uint32_t total = rl.counters[0];
+    uint32_t* claims = rl.counters + kQueueStride / sizeof(uint32_t);
+    const int lane = threadIdx.x & (kTile - 1);
+    const int decoder = jobs[0].decoder;  // one decoder per device (init)
+    // group size: the largest K <= 16 with total * K <= 4 * lanes of the launch
+    const uint64_t lanes = (uint64_t)gridDim.x * kRefineThreads;
+    int K = 1;
+    while (K < 16 && (uint64_t)total * (uint64_t)(2 * K) <= 4 * lanes) K *= 2;
+    const int sub = lane & (K - 1);
+    const int leader = lane - sub;
+    const uint64_t group_bits = (K == 64 ? ~0ull : ((1ull << K) - 1)) << leader;
+    uint64_t leaders = 0;
+    for (int l = 0; l < kTile; l += K) leaders |= 1ull << l;
+
+    TexelRegs<true> r;
+    f3 cw = mk(0.0f, 0.0f, 0.0f);
+    bool tcm = false, has = false, drained = false;
+    Code out{0u, 0u};
+    float best = 0.0f;
+    int last = 0, i = 0;
+    uint32_t idx = 0, job = 0, block = 0;
+    // blocks claimed by this wavefront and not started yet: [lo, hi); the
+    // list counter is touched once per kTile / K blocks, not per free group
+    const uint32_t chunk = (uint32_t)(kTile / K);
+    uint32_t lo = 0, hi = 0;
+    for (;;) {
+        // free groups (their leader lanes) start the wavefront's next blocks
+        const uint64_t free_mask = __ballot(!has) & leaders;
+        if (free_mask && lo == hi && !drained) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(claims, chunk);
+            base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
+            lo = min(base, total);
+            hi = min(base + chunk, total);
+            if (base + chunk >= total) drained = true;
+        }
+        if (free_mask && lo < hi) {
+            const uint32_t rank = (uint32_t)__popcll(free_mask & ((1ull << leader) - 1));
+            const uint32_t k = lo + rank;
+            const uint32_t taken = min((uint32_t)__popcll(free_mask), hi - lo);
+            lo += taken;
+            if (!has && ((free_mask >> leader) & 1) && k < lo) {
+                const RefineItem it = rl.items[k];
+                const JobDev& jd = jobs[it.job];
+                job = it.job;
+                block = it.block;
+                const Texels tx{jd.colors + (uint64_t)block * 16,
+                                jd.weights ? reinterpret_cast<const float*>(jd.weights + (uint64_t)block * 4)
+                                           : nullptr};
+                r.load(tx);
+                cw = mk(jd.cw0, jd.cw1, jd.cw2);
+                tcm = jd.tcm != 0;
+                const uint2 v = jd.codes[block];
+                out = Code{v.x, v.y};
+                best = it.error;
+                last = 0;
+                i = 0;
+                f3 sel[4];
+                palette_f(col0(out), col1(out), decoder, unit, sel);
+                idx = select_indices(r, cw, sel);
+                has = true;
+            }
+        }
+        if (!__ballot(has) && lo == hi && drained) break;  // every lane idle, nothing left to start
+        if (has) {
+            const int limit = min(last + 33, 255);  // the walk's last step unless one is accepted
+            const int step = i + sub;
+            const bool valid = step <= limit;
+            int dr, dg, db;
+            refine_step(step & 15, dr, dg, db);
+            uint32_t c0 = col0(out), c1 = col1(out);
+            if ((step >> 4) & 1) c0 = nudge565(c0, dr, dg, db);
+            else c1 = nudge565(c1, dr, dg, db);
+            if (!tcm) {
+                if (c0 == c1) c1 = nudge565(c1, 0, 1, 0);
+                if (c0 < c1) {
+                    const uint32_t tmp = c0;
+                    c0 = c1;
+                    c1 = tmp;
+                }
+            }
+            f3 pal[4];
+            palette_f(c0, c1, decoder, unit, pal);
+            const float e = indexed_error(r, cw, pal, idx);
+            const uint64_t acc = __ballot(valid && e < best) & group_bits;
+            bool finished;
+            if (acc) {
+                // the first accepted step of the group becomes the best code
+                const int j = __builtin_ctzll(acc >> leader);
+                const uint32_t cols = __shfl(c0 | (c1 << 16), leader + j, 64);
+                const float ej = __shfl(e, leader + j, 64);
+                best = ej;
+                out = Code{cols, idx};
+                last = i + j;
+                palette_f(cols & 0xffffu, cols >> 16, decoder, unit, pal);
+                idx = select_indices(r, cw, pal);
+                i = last + 1;
+                finished = last == 255;
+            } else {
+                finished = i + K - 1 >= limit;
+                i += K;
+            }
+            if (finished) {
+                if (sub == 0) {
+                    const JobDev& jd = jobs[job];
+                    const int steps = acc ? 256 : limit + 1;
+                    jd.codes[block] = make_uint2(out.cols, out.idx);
+                    if (jd.errors) jd.errors[block] = best;
+                    if (jd.stats) reinterpret_cast<int32_t*>(jd.stats + block)[2] = steps;
+                }
+                has = false;
+            }
+        }
+    }
 }
 
 // Counts the float bit patterns where recip_rn differs from 1.0f / b (NaN
@@ -579,36 +726,62 @@ __global__ void bc1_evaluate_kernel(const uint32_t* __restrict__ rgba, const uin
 // launch glue
 
 int encode_workgroups_per_cu() {
-    int box = 0, cluster = 0, all = 0;
+    int box = 0, cluster = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&box, bc1_encode_kernel<kModeBox>, kTile, 0) != hipSuccess)
         box = 1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&cluster, bc1_encode_kernel<kModeCluster>, kTile, 0) !=
         hipSuccess)
         cluster = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&all, bc1_encode_kernel<kModeAll>, kTile, 0) != hipSuccess)
-        all = 1;
-    return std::max(1, std::min(box, std::min(cluster, all)));
+    return std::max(1, std::min(box, cluster));
 }
 
+namespace {
+
+// Resident workgroups of the refinement kernel on the current device.
+uint32_t refine_grid() {
+    static std::mutex m;
+    static std::vector<std::pair<int, uint32_t>> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    std::lock_guard<std::mutex> lock(m);
+    for (auto& c : cache)
+        if (c.first == dev) return c.second;
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bc1_refine_kernel, kRefineThreads, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const uint32_t g = (uint32_t)cus * (uint32_t)per_cu;
+    cache.push_back({dev, g});
+    return g;
+}
+
+}  // namespace
+
 hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, uint32_t* counter,
-                         float4* workspace, uint32_t grid, int mode, hipStream_t stream) {
+                         float4* workspace, uint32_t grid, int mode, RefineList refine, hipStream_t stream) {
     if (ntiles == 0) return hipSuccess;
     if (grid > ntiles) grid = ntiles;
+    const uint32_t stride = (uint32_t)(kQueueStride / sizeof(uint32_t));
     if (counter) {
-        hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, counter, kQueues,
-                           (uint32_t)(kQueueStride / sizeof(uint32_t)));
+        hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, counter, kQueues, stride);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    if (mode == kModeAll)
-        hipLaunchKernelGGL(bc1_encode_kernel<kModeAll>, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles,
-                           counter, workspace);
-    else if (mode == kModeCluster)
-        hipLaunchKernelGGL(bc1_encode_kernel<kModeCluster>, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles,
-                           counter, workspace);
-    else
+    if (mode == kModeAll) {
+        hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, refine.counters, 2u, stride);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (mode == kModeBox)
         hipLaunchKernelGGL(bc1_encode_kernel<kModeBox>, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles,
-                           counter, workspace);
+                           counter, workspace, refine);
+    else
+        hipLaunchKernelGGL(bc1_encode_kernel<kModeCluster>, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles,
+                           counter, workspace, refine);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || mode != kModeAll) return e;
+    hipLaunchKernelGGL(bc1_refine_kernel, dim3(refine_grid()), dim3(kRefineThreads), 0, stream, jobs, refine);
     return hipGetLastError();
 }
 

@@ -61,19 +61,48 @@ constexpr uint32_t kQueues = 8;
 constexpr size_t kQueueStride = 128;
 constexpr size_t kQueueBytes = kQueues * kQueueStride;
 
+// Level-9 blocks handed from the encode kernel to the refinement kernel: the
+// encode kernel writes the block's fitted code (or leaves the prior one) and
+// appends {job, block, error after the fits}; the refinement kernel walks
+// each one with a lane of its own.
+struct RefineItem {
+    uint32_t job;
+    uint32_t block;
+    float error;
+    uint32_t pad;
+};
+struct RefineList {
+    RefineItem* items;   // room for every level-9 block of the launch
+    uint32_t* counters;  // [0] appended, [kQueueStride / 4] claimed
+};
+constexpr size_t kRefineCounterBytes = 2 * kQueueStride;
+inline size_t refine_list_bytes(uint64_t level9_blocks) {
+    return kRefineCounterBytes + (size_t)level9_blocks * sizeof(RefineItem);
+}
+
 // Kernel instances: every job of a kModeBox launch is at level 0 or 1 (or an
-// invalid level); kModeCluster covers levels 0-8, kModeAll also level 9.
+// invalid level); kModeCluster covers levels 0-9; kModeAll is a kModeCluster
+// launch followed by the refinement kernel over its level-9 blocks.
 constexpr int kModeBox = 0;
 constexpr int kModeCluster = 1;
 constexpr int kModeAll = 2;
 
 // `mode` selects the kernel instance; encode_mode() gives the one a set of
 // jobs needs.
+// With kModeAll, `refine` holds room for the launch's level-9 blocks.
 hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, uint32_t* counter,
-                         float4* workspace, uint32_t grid, int mode, hipStream_t stream);
+                         float4* workspace, uint32_t grid, int mode, RefineList refine, hipStream_t stream);
 
 inline int encode_mode_of_level(int level) {
     return level == 9 ? kModeAll : (level >= 2 && level <= 8) ? kModeCluster : kModeBox;
+}
+
+// Level-9 blocks of a set of (host-side) jobs.
+inline uint64_t level9_blocks(const JobDev* jobs, size_t n) {
+    uint64_t b = 0;
+    for (size_t k = 0; k < n; k++)
+        if (jobs[k].level == 9) b += jobs[k].block_count;
+    return b;
 }
 
 // The kernel instance a launch over these (host-side) jobs needs.

@@ -203,16 +203,27 @@ int resident_workgroups(uint32_t* grid) {
     return ICBC_OK;
 }
 
-// Stream-ordered workspace for one encode launch: the tile-queue counters
-// and `grid` workgroup slices.
+// The level-9 handoff list laid out at `p` (refine_list_bytes() bytes).
+RefineList refine_list_at(void* p) {
+    return RefineList{(RefineItem*)((char*)p + kRefineCounterBytes), (uint32_t*)p};
+}
+
+// Stream-ordered workspace for one encode launch: the tile-queue counters,
+// `grid` workgroup slices and room to hand `level9` blocks to refinement.
 struct Workspace {
     void* base = nullptr;
+    uint32_t grid = 0;
     float4* slices() const { return (float4*)((char*)base + kQueueBytes); }
     uint32_t* counter() const { return (uint32_t*)base; }
+    RefineList refine() const {
+        return refine_list_at((char*)base + kQueueBytes + (size_t)grid * kWorkspaceBytesPerWorkgroup);
+    }
 };
 
-int workspace_alloc(uint32_t grid, hipStream_t s, Workspace& w) {
-    HIP_TRY(hipMallocAsync(&w.base, kQueueBytes + (size_t)grid * kWorkspaceBytesPerWorkgroup, s));
+int workspace_alloc(uint32_t grid, uint64_t level9, hipStream_t s, Workspace& w) {
+    w.grid = grid;
+    HIP_TRY(hipMallocAsync(&w.base, kQueueBytes + (size_t)grid * kWorkspaceBytesPerWorkgroup + refine_list_bytes(level9),
+                           s));
     return ICBC_OK;
 }
 
@@ -680,10 +691,10 @@ int icbc_gpu_compress_bc1_batch(const icbc_bc1_job* jobs, size_t n_jobs, void* s
             if (r) return r;
         }
         Workspace w;
-        r = workspace_alloc(grid, s, w);
+        r = workspace_alloc(grid, level9_blocks(p.dev.data(), p.dev.size()), s, w);
         if (r) return r;
         HIP_TRY(launch_encode((const JobDev*)djobs, (const uint2*)dtiles, (uint32_t)tiles.size(), w.counter(),
-                              w.slices(), grid, encode_mode(p.dev.data(), p.dev.size()), s));
+                              w.slices(), grid, encode_mode(p.dev.data(), p.dev.size()), w.refine(), s));
         HIP_TRY(hipFreeAsync(w.base, s));
         HIP_TRY(hipFreeAsync(djobs, s));
         if (dtiles) HIP_TRY(hipFreeAsync(dtiles, s));
@@ -711,11 +722,13 @@ int icbc_gpu_compress_bc1_plain(const icbc_bc1_job* jobs, size_t n_jobs, void* s
         r = upload_blob(p.dev.data(), p.dev.size() * sizeof(JobDev), s, &djobs);
         if (r) return r;
         Workspace w;
-        r = workspace_alloc(grid, s, w);
+        uint64_t l9 = 0;  // one launch at a time reuses the handoff list
+        for (const JobDev& d : p.dev) l9 = std::max<uint64_t>(l9, level9_blocks(&d, 1));
+        r = workspace_alloc(grid, l9, s, w);
         if (r) return r;
         for (size_t k = 0; k < n_jobs; k++)
             HIP_TRY(launch_encode((const JobDev*)djobs + k, nullptr, tiles_of(p.dev[k].block_count, p.dev[k].tile_blocks),
-                                  nullptr, w.slices(), grid, encode_mode(&p.dev[k], 1), s));
+                                  nullptr, w.slices(), grid, encode_mode(&p.dev[k], 1), w.refine(), s));
         HIP_TRY(hipFreeAsync(w.base, s));
         HIP_TRY(hipFreeAsync(djobs, s));
     }
@@ -747,7 +760,8 @@ int plan_launch(icbc_bc1_plan* pl, hipStream_t s) {
     for (const auto& c : pl->converts) HIP_TRY(run_convert(c, s));
     uint32_t* counter = pl->opt.schedule == ICBC_SCHEDULE_PERSISTENT ? (uint32_t*)pl->dwork : nullptr;
     float4* slices = (float4*)((char*)pl->dwork + kQueueBytes);
-    HIP_TRY(launch_encode(pl->djobs, pl->dtiles, pl->ntiles, counter, slices, pl->grid, pl->mode, s));
+    const RefineList rl = refine_list_at((char*)pl->dwork + kQueueBytes + (size_t)pl->grid * kWorkspaceBytesPerWorkgroup);
+    HIP_TRY(launch_encode(pl->djobs, pl->dtiles, pl->ntiles, counter, slices, pl->grid, pl->mode, rl, s));
     return ICBC_OK;
 }
 
@@ -825,7 +839,8 @@ int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_pla
     };
     if ((e = hipMalloc(&pl->djobs, std::max<size_t>(1, dev.size()) * sizeof(JobDev))) != hipSuccess ||
         (e = hipMalloc(&pl->dtiles, std::max<size_t>(1, tiles.size()) * sizeof(uint2))) != hipSuccess ||
-        (e = hipMalloc(&pl->dwork, kQueueBytes + (size_t)pl->grid * kWorkspaceBytesPerWorkgroup)) != hipSuccess)
+        (e = hipMalloc(&pl->dwork, kQueueBytes + (size_t)pl->grid * kWorkspaceBytesPerWorkgroup +
+                                       refine_list_bytes(level9_blocks(dev.data(), dev.size())))) != hipSuccess)
         return cleanup(hip_fail(e, "hipMalloc (plan)"));
     for (size_t k = 0; k < n_jobs; k++) {
         if (!needs_convert(jobs[k])) continue;
@@ -916,7 +931,7 @@ int block_context(BlockContext*& ctx) {
     if (ctx->device != dev) {
         ctx->device = dev;
         HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-        HIP_TRY(hipMalloc(&ctx->dbuf, 512 + 256 + kWorkspaceBytesPerWorkgroup));
+        HIP_TRY(hipMalloc(&ctx->dbuf, 768 + kWorkspaceBytesPerWorkgroup + refine_list_bytes(1)));
         HIP_TRY(hipHostMalloc(&ctx->hbuf, 512, hipHostMallocDefault));
     }
     return ICBC_OK;
@@ -957,6 +972,7 @@ int compress_one(int level, const float* input_colors, const float* input_weight
     memcpy(h + 384, &jd, sizeof jd);
     HIP_TRY(hipMemcpyAsync(d, h, 512, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(launch_encode((const JobDev*)(d + 384), nullptr, 1, nullptr, (float4*)(d + 768), 1, encode_mode_of_level(level),
+                          refine_list_at(d + 768 + kWorkspaceBytesPerWorkgroup),
                           ctx->stream));
     HIP_TRY(hipMemcpyAsync(h + 320, d + 320, 32, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -158,7 +158,9 @@ struct Request {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    void* work = nullptr;  // counter + workspace slices
+    void* work = nullptr;  // queue counters + workspace slices
+    void* refine = nullptr;  // level-9 handoff list, grown to the largest launch
+    size_t refine_bytes = 0;
     void* table = nullptr;  // job table + tile list
     size_t table_bytes = 0;
     void* host_table = nullptr;  // pinned staging for the table
@@ -250,9 +252,20 @@ int icbc_server::launch(Slot& s, std::vector<Request*>& reqs) {
     memcpy((char*)s.host_table + jbytes, tiles.data(), tiles.size() * sizeof(uint2));
     if (hipMemcpyAsync(s.table, s.host_table, need, hipMemcpyHostToDevice, s.stream) != hipSuccess)
         return ICBC_ERR_DEVICE;
+    const size_t rbytes = refine_list_bytes(level9_blocks(dev.data(), dev.size()));
+    if (rbytes > s.refine_bytes) {
+        // the slot's stream has finished its previous launch (the slot was free)
+        if (s.refine) (void)hipFree(s.refine);
+        s.refine = nullptr;
+        s.refine_bytes = 0;
+        const size_t cap = std::max<size_t>(rbytes * 2, 64 << 10);
+        if (hipMalloc(&s.refine, cap) != hipSuccess) return ICBC_ERR_OUT_OF_MEMORY;
+        s.refine_bytes = cap;
+    }
+    const RefineList rl{(RefineItem*)((char*)s.refine + kRefineCounterBytes), (uint32_t*)s.refine};
     if (launch_encode((const JobDev*)s.table, (const uint2*)((char*)s.table + jbytes), (uint32_t)tiles.size(),
                       (uint32_t*)s.work, (float4*)((char*)s.work + kQueueBytes), grid, encode_mode(dev.data(), dev.size()),
-                      s.stream) != hipSuccess)
+                      rl, s.stream) != hipSuccess)
         return ICBC_ERR_DEVICE;
     // results go to pinned staging (a copy into pageable memory would block
     // this thread until the kernel ends); the reaper hands them over
@@ -565,6 +578,7 @@ int icbc_server_destroy(icbc_server* sv) {
         if (s.done) (void)hipEventDestroy(s.done);
         if (s.stream) (void)hipStreamDestroy(s.stream);
         if (s.work) (void)hipFree(s.work);
+        if (s.refine) (void)hipFree(s.refine);
         if (s.table) (void)hipFree(s.table);
         if (s.host_table) (void)hipHostFree(s.host_table);
     }
