@@ -680,23 +680,40 @@ __device__ __forceinline__ float candidate4(float4 s0, float4 s1, float4 s2, flo
 // weights of each cluster sum to 1), A + C = Wa (sum of w * alpha) and
 // Y = t - X (t = block totals).  Over all real (a, b) its minimum is
 //   -K0 - W * sum_c m_c (X_c - (Wa / W) t_c)^2 / (A W - Wa^2),  K0 = sum_c m_c t_c^2 / W,
-// so no candidate can score below that.  The bound is computed here with
-// fused multiply-adds and the hardware reciprocal (it only decides which
-// candidates are evaluated, never a result); the rounding of this form and
-// of candidate4's own arithmetic is covered by the block's margin
-// (bound_row).  Candidates with a poorly conditioned system
+// so no candidate can score below that.  X, Wa and A are fixed combinations
+// of the candidate's three SAT rows S0, S1, S2 (X = w1_3 S2 + (w2_3 - w1_3) S1
+// + (1 - w2_3) S0, likewise Wa on the weights, A with the squared palette
+// weights), so X - (Wa / W) t is the same combination of the centred rows
+// R = (S.xyz - (S.w / W) t, S.w), staged once per block.  The bound is
+// computed with fused multiply-adds and the hardware reciprocal (it only
+// decides which candidates are evaluated, never a result); the rounding of
+// this form and of candidate4's own arithmetic is covered by the block's
+// margin (bound_row).  Candidates with a poorly conditioned system
 // (A W - Wa^2 < A W / 16) get -inf: never pruned.
-__device__ __forceinline__ float candidate4_bound(float4 s0, float4 s1, float4 s2, float W, f3 t, float invW,
-                                                  float K0, f3 m, const DecoderWeights& dw) {
-    const float W1 = s1.w - s0.w, W2 = s2.w - s1.w;
-    const float A = __builtin_fmaf(W2, dw.w1_9, __builtin_fmaf(W1, dw.w4_9, s0.w));
-    const float Wa = __builtin_fmaf(W2, dw.w1_3, __builtin_fmaf(W1, dw.w2_3, s0.w));
+struct BoundCoefs {
+    float c0, c1, c2;  // rows' weights in X and Wa
+    float a0, a1, a2;  // rows' weights in A
+};
+
+__device__ __forceinline__ BoundCoefs bound_coefs(const DecoderWeights& dw) {
+    return BoundCoefs{1.0f - dw.w2_3, dw.w2_3 - dw.w1_3, dw.w1_3, 1.0f - dw.w4_9, dw.w4_9 - dw.w1_9, dw.w1_9};
+}
+
+// Centred SAT row (`invW` = 1 / total weight, `t` = colour totals).
+__device__ __forceinline__ float4 centred_row(float4 s, float invW, f3 t) {
+    const float f = s.w * invW;
+    return make_float4(__builtin_fmaf(-f, t.x, s.x), __builtin_fmaf(-f, t.y, s.y), __builtin_fmaf(-f, t.z, s.z), s.w);
+}
+
+__device__ __forceinline__ float candidate4_bound(float4 r0, float4 r1, float4 r2, float W, float K0, f3 m,
+                                                  const BoundCoefs& k) {
+    const float A = __builtin_fmaf(k.a2, r2.w, __builtin_fmaf(k.a1, r1.w, k.a0 * r0.w));
+    const float Wa = __builtin_fmaf(k.c2, r2.w, __builtin_fmaf(k.c1, r1.w, k.c0 * r0.w));
     const float AW = A * W;
     const float D = __builtin_fmaf(-Wa, Wa, AW);
-    const float mu = Wa * invW;
-    const float dx = __builtin_fmaf(-mu, t.x, __builtin_fmaf(s2.x - s1.x, dw.w1_3, __builtin_fmaf(s1.x - s0.x, dw.w2_3, s0.x)));
-    const float dy = __builtin_fmaf(-mu, t.y, __builtin_fmaf(s2.y - s1.y, dw.w1_3, __builtin_fmaf(s1.y - s0.y, dw.w2_3, s0.y)));
-    const float dz = __builtin_fmaf(-mu, t.z, __builtin_fmaf(s2.z - s1.z, dw.w1_3, __builtin_fmaf(s1.z - s0.z, dw.w2_3, s0.z)));
+    const float dx = __builtin_fmaf(k.c2, r2.x, __builtin_fmaf(k.c1, r1.x, k.c0 * r0.x));
+    const float dy = __builtin_fmaf(k.c2, r2.y, __builtin_fmaf(k.c1, r1.y, k.c0 * r0.y));
+    const float dz = __builtin_fmaf(k.c2, r2.z, __builtin_fmaf(k.c1, r1.z, k.c0 * r0.z));
     const float sd = __builtin_fmaf(m.z * dz, dz, __builtin_fmaf(m.y * dy, dy, (m.x * dx) * dx));
     const float lb = __builtin_fmaf(-W * sd, __builtin_amdgcn_rcpf(D), -K0);
     return (D >= AW * 0.0625f) ? lb : -INFINITY;

@@ -91,7 +91,8 @@ struct Shared {
     uint32_t three[160];
     float unit[256];         // k / 255.0f
     float4 sat[2][kSatRows];
-    uint16_t survivors[976];  // four-cluster candidates left by the bound pass
+    float4 centred[2][kSatRows];  // four-cluster search: the rows centred for the bound pass
+    uint16_t survivors[976];      // four-cluster candidates left by the bound pass
 };
 
 // Four-cluster searches with at least this many colours run the bound pass
@@ -141,6 +142,11 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offs
     int buf = 0;
     while (b < tile_blocks) {
         if (lane <= nb || lane == kBoundRow) sh.sat[buf][lane] = row;
+        if (kClusters == 4 && nb >= kPruneMinColors) {
+            const f3 t = mk(__shfl(row.x, nb, 64), __shfl(row.y, nb, 64), __shfl(row.z, nb, 64));
+            const float invW = __builtin_amdgcn_rcpf(__shfl(row.w, nb, 64));
+            if (lane <= nb) sh.centred[buf][lane] = centred_row(row, invW, t);
+        }
         __syncthreads();
         // prefetch the next block's rows while this one is searched
         const int b2 = next_block(my_n, tile_blocks, b + 1, kMinN);
@@ -158,6 +164,8 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offs
             const f3 t = mk(tot.x, tot.y, tot.z);
             const float invW = __builtin_amdgcn_rcpf(tot.w);
             const float K0 = ((metric.x * t.x) * t.x + (metric.y * t.y) * t.y + (metric.z * t.z) * t.z) * invW;
+            const BoundCoefs coefs = bound_coefs(dw);
+            const float4* cen = sh.centred[buf];
             // bound pass: lane l holds the bounds of candidates l, l + 64, ...
             float lb[16];
             float key_min = INFINITY;
@@ -168,8 +176,8 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offs
                 lb[k] = INFINITY;
                 if (i < total) {
                     const uint32_t pk = sh.four[i];
-                    lb[k] = candidate4_bound(sat[pk & 0xff], sat[(pk >> 8) & 0xff], sat[pk >> 16], tot.w, t, invW,
-                                             K0, metric, dw);
+                    lb[k] = candidate4_bound(cen[pk & 0xff], cen[(pk >> 8) & 0xff], cen[pk >> 16], tot.w, K0, metric,
+                                             coefs);
                     if (lb[k] > -INFINITY && lb[k] < key_min) {
                         key_min = lb[k];
                         k_min = k;
