@@ -319,3 +319,20 @@ def test_pruned_search_adversarial_blocks(gpu, oracle, cw):
         bad = np.nonzero(np.any(got[0] != want[0], axis=1))[0]
         assert len(bad) == 0, f"level {level} cw {cw}: {len(bad)} blocks differ, first {bad[:5]}"
         assert np.array_equal(got[1].view(np.uint32), want[1].view(np.uint32))
+
+
+def test_refinement_group_size_does_not_change_results(gpu, oracle):
+    # a 4096x2048 level-9 image hands ~0.5 M walks to the refinement kernel,
+    # which then walks one block per lane; a few of its blocks alone are
+    # walked by groups of lanes evaluating steps speculatively.  Both must give
+    # the sequential walk's codes and errors.
+    blocks, _ = synth.make_image_blocks(17, 4096, 2048, synth.MIX_DEFAULT)
+    big = gpu.compress_many(9, blocks, None, (1, 1, 1), True, True)
+    rng = np.random.default_rng(3)
+    pick = np.sort(rng.choice(len(blocks), 2048, replace=False))
+    small = gpu.compress_many(9, blocks[pick], None, (1, 1, 1), True, True)
+    assert np.array_equal(big[0][pick], small[0])
+    assert np.array_equal(big[1][pick].view(np.uint32), small[1].view(np.uint32))
+    want = oracle.compress_many(9, blocks[pick], None, (1, 1, 1), True, True, nthreads=8)
+    assert np.array_equal(small[0], want[0])
+    assert np.array_equal(small[1].view(np.uint32), want[1].view(np.uint32))
