@@ -164,6 +164,19 @@ void build_tables(HostTables& t) {
 }
 
 // Makes sure the current device holds the tables; returns the decoder.
+// Per-device memory pool for the library's stream-ordered scratch (workspace,
+// job tables, staging): it keeps up to kPoolKeepBytes reserved across stream
+// synchronisations, so repeated calls reuse their memory instead of mapping
+// it again (the default pool hands everything back at each synchronisation).
+constexpr uint64_t kPoolKeepBytes = 8ull << 30;
+std::vector<std::pair<int, hipMemPool_t>> g_pools;
+
+hipMemPool_t pool_for(int dev) {
+    for (auto& p : g_pools)
+        if (p.first == dev) return p.second;
+    return nullptr;
+}
+
 int ensure_device_ready(int* decoder) {
     std::lock_guard<std::mutex> lock(g_mutex);
     if (g_decoder < 0) return fail(ICBC_ERR_NOT_INITIALIZED, "icbc_gpu_init has not been called");
@@ -177,6 +190,20 @@ int ensure_device_ready(int* decoder) {
                         prop.gcnArchName);
         hipError_t e = upload_tables(g_tables, nullptr);
         if (e != hipSuccess) return hip_fail(e, "upload_tables");
+        if (!pool_for(dev)) {
+            hipMemPoolProps props;
+            memset(&props, 0, sizeof props);
+            props.allocType = hipMemAllocationTypePinned;
+            props.location.type = hipMemLocationTypeDevice;
+            props.location.id = dev;
+            hipMemPool_t pool = nullptr;
+            e = hipMemPoolCreate(&pool, &props);
+            if (e != hipSuccess) return hip_fail(e, "hipMemPoolCreate");
+            uint64_t keep = kPoolKeepBytes;
+            e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            if (e != hipSuccess) return hip_fail(e, "hipMemPoolSetAttribute");
+            g_pools.push_back({dev, pool});
+        }
         g_uploaded.push_back(dev);
     }
     *decoder = g_decoder;
@@ -203,6 +230,19 @@ int resident_workgroups(uint32_t* grid) {
     return ICBC_OK;
 }
 
+// Stream-ordered allocation from the current device's pool.
+hipError_t scratch_alloc(void** p, size_t bytes, hipStream_t s) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    hipMemPool_t pool = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        pool = pool_for(dev);
+    }
+    return pool ? hipMallocFromPoolAsync(p, bytes, pool, s) : hipMallocAsync(p, bytes, s);
+}
+
 // The level-9 handoff list laid out at `p` (refine_list_bytes() bytes).
 RefineList refine_list_at(void* p) {
     return RefineList{(RefineItem*)((char*)p + kRefineCounterBytes), (uint32_t*)p};
@@ -222,7 +262,7 @@ struct Workspace {
 
 int workspace_alloc(uint32_t grid, uint64_t level9, hipStream_t s, Workspace& w) {
     w.grid = grid;
-    HIP_TRY(hipMallocAsync(&w.base, kQueueBytes + (size_t)grid * kWorkspaceBytesPerWorkgroup + refine_list_bytes(level9),
+    HIP_TRY(scratch_alloc((void**)&w.base, kQueueBytes + (size_t)grid * kWorkspaceBytesPerWorkgroup + refine_list_bytes(level9),
                            s));
     return ICBC_OK;
 }
@@ -368,7 +408,7 @@ void free_host_blob(void* p) { free(p); }
 
 // Copies `bytes` from host into fresh stream-ordered device memory.
 int upload_blob(const void* src, size_t bytes, hipStream_t s, void** dptr) {
-    HIP_TRY(hipMallocAsync(dptr, bytes, s));
+    HIP_TRY(scratch_alloc((void**)dptr, bytes, s));
     void* keep = malloc(bytes);
     if (!keep) return fail(ICBC_ERR_OUT_OF_MEMORY, "host allocation of %zu bytes", bytes);
     memcpy(keep, src, bytes);
@@ -397,24 +437,24 @@ int stage_in(const icbc_bc1_job& j, hipStream_t s, Staged& st, icbc_bc1_job& dj)
     const uint64_t n = j.block_count;
     if (n == 0) return ICBC_OK;
     const size_t cbytes = n * ((j.flags & ICBC_JOB_COLORS_RGBA8) ? 64 : 256);
-    HIP_TRY(hipMallocAsync(&st.colors, cbytes, s));
+    HIP_TRY(scratch_alloc((void**)&st.colors, cbytes, s));
     HIP_TRY(hipMemcpyAsync(st.colors, j.colors, cbytes, hipMemcpyHostToDevice, s));
     dj.colors = (const float*)st.colors;
     if (j.weights) {
-        HIP_TRY(hipMallocAsync(&st.weights, n * 64, s));
+        HIP_TRY(scratch_alloc((void**)&st.weights, n * 64, s));
         HIP_TRY(hipMemcpyAsync(st.weights, j.weights, n * 64, hipMemcpyHostToDevice, s));
         dj.weights = (const float*)st.weights;
     }
     // codes go in too: blocks the reference leaves untouched keep their bytes
-    HIP_TRY(hipMallocAsync(&st.codes, n * 8, s));
+    HIP_TRY(scratch_alloc((void**)&st.codes, n * 8, s));
     HIP_TRY(hipMemcpyAsync(st.codes, j.codes, n * 8, hipMemcpyHostToDevice, s));
     dj.codes = st.codes;
     if (j.errors) {
-        HIP_TRY(hipMallocAsync(&st.errors, n * 4, s));
+        HIP_TRY(scratch_alloc((void**)&st.errors, n * 4, s));
         dj.errors = (float*)st.errors;
     }
     if (j.stats) {
-        HIP_TRY(hipMallocAsync(&st.stats, n * 16, s));
+        HIP_TRY(scratch_alloc((void**)&st.stats, n * 16, s));
         dj.stats = (int32_t*)st.stats;
     }
     return ICBC_OK;
@@ -493,7 +533,7 @@ int convert_sources(Prepared& p, hipStream_t s) {
         const icbc_bc1_job& j = p.jobs[k];
         if (!needs_convert(j)) continue;
         void* f = nullptr;
-        HIP_TRY(hipMallocAsync(&f, convert_bytes(j), s));
+        HIP_TRY(scratch_alloc((void**)&f, convert_bytes(j), s));
         p.temps.push_back(f);
         HIP_TRY(run_convert(describe_convert(j, f, p.dev[k]), s));
     }
@@ -575,7 +615,14 @@ float static_level_cost(int level) {
     return (level >= 0 && level <= 9) ? cost[level] : 0.0f;
 }
 
-void order_tiles(std::vector<uint2>& tiles, const std::vector<JobDev>& dev) {
+// Jobs in launch order: with `by_cost`, largest per-tile cost first (cost per
+// block of the level, measured by the autotuner where it has timed every level
+// present, else a static estimate, times the tile size), ties in submission
+// order; otherwise submission order.
+std::vector<uint32_t> job_order(const std::vector<JobDev>& dev, bool by_cost) {
+    std::vector<uint32_t> order(dev.size());
+    for (size_t k = 0; k < dev.size(); k++) order[k] = (uint32_t)k;
+    if (!by_cost) return order;
     float cost[10];
     for (int l = 0; l < 10; l++) cost[l] = static_level_cost(l);
     icbc_tuning tune;
@@ -586,20 +633,29 @@ void order_tiles(std::vector<uint2>& tiles, const std::vector<JobDev>& dev) {
         if (all)
             for (int l = 0; l < 10; l++) cost[l] = tune.ns_per_block[l];
     }
-    auto c = [&](const JobDev& d) { return (d.level >= 0 && d.level <= 9) ? cost[d.level] : 0.0f; };
-    std::stable_sort(tiles.begin(), tiles.end(), [&](const uint2& a, const uint2& b) {
-        return c(dev[a.x]) * dev[a.x].tile_blocks > c(dev[b.x]) * dev[b.x].tile_blocks;
-    });
+    auto c = [&](uint32_t k) {
+        const JobDev& d = dev[k];
+        return ((d.level >= 0 && d.level <= 9) ? cost[d.level] : 0.0f) * d.tile_blocks;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return c(a) > c(b); });
+    return order;
 }
 
-std::vector<uint2> make_tiles(const std::vector<JobDev>& dev) {
-    std::vector<uint2> tiles;
-    size_t total = 0;
+uint64_t tile_count(const std::vector<JobDev>& dev) {
+    uint64_t total = 0;
     for (const JobDev& d : dev) total += tiles_of(d.block_count, d.tile_blocks);
-    tiles.reserve(total);
-    for (size_t k = 0; k < dev.size(); k++)
+    return total;
+}
+
+// The tile list (job, first block): every tile of a job together, jobs in
+// job_order().  Tiles of one job cost the same, so this is the list sorted by
+// per-tile cost with ties kept in submission order.
+std::vector<uint2> make_tiles(const std::vector<JobDev>& dev, bool by_cost) {
+    std::vector<uint2> tiles;
+    tiles.reserve(tile_count(dev));
+    for (uint32_t k : job_order(dev, by_cost))
         for (uint32_t t = 0; t < tiles_of(dev[k].block_count, dev[k].tile_blocks); t++)
-            tiles.push_back(make_uint2((uint32_t)k, t * (uint32_t)dev[k].tile_blocks));
+            tiles.push_back(make_uint2(k, t * (uint32_t)dev[k].tile_blocks));
     return tiles;
 }
 
@@ -635,9 +691,7 @@ uint32_t auto_tile_blocks(uint64_t job_blocks, uint64_t total_blocks, uint32_t r
 }
 
 std::vector<uint2> tiles_for(const std::vector<JobDev>& dev, bool order_by_cost) {
-    std::vector<uint2> t = make_tiles(dev);
-    if (order_by_cost) order_tiles(t, dev);
-    return t;
+    return make_tiles(dev, order_by_cost);
 }
 
 }  // namespace icbc_gpu
@@ -674,26 +728,26 @@ int icbc_gpu_compress_bc1_batch(const icbc_bc1_job* jobs, size_t n_jobs, void* s
     if (r) return r;
     r = choose_tiles(p.dev, jobs, false);
     if (r) return r;
-    std::vector<uint2> tiles = make_tiles(p.dev);
-    if (tiles.size() > 0xffffffffu) return fail(ICBC_ERR_INVALID_ARGUMENT, "batch too large");
-    order_tiles(tiles, p.dev);
-    if (!tiles.empty()) {
+    const uint64_t ntiles = tile_count(p.dev);
+    if (ntiles > 0xffffffffu) return fail(ICBC_ERR_INVALID_ARGUMENT, "batch too large");
+    if (ntiles) {
         uint32_t grid;
         r = resident_workgroups(&grid);
         if (r) return r;
-        grid = std::min<uint32_t>(grid, (uint32_t)tiles.size());
+        grid = std::min<uint32_t>(grid, (uint32_t)ntiles);
         void *djobs = nullptr, *dtiles = nullptr;
         r = upload_blob(p.dev.data(), p.dev.size() * sizeof(JobDev), s, &djobs);
         if (r) return r;
         // one job: its tiles are implied (tile k = blocks [T k, T k + T)), no list
         if (n_jobs > 1) {
+            const std::vector<uint2> tiles = make_tiles(p.dev, true);
             r = upload_blob(tiles.data(), tiles.size() * sizeof(uint2), s, &dtiles);
             if (r) return r;
         }
         Workspace w;
         r = workspace_alloc(grid, level9_blocks(p.dev.data(), p.dev.size()), s, w);
         if (r) return r;
-        HIP_TRY(launch_encode((const JobDev*)djobs, (const uint2*)dtiles, (uint32_t)tiles.size(), w.counter(),
+        HIP_TRY(launch_encode((const JobDev*)djobs, (const uint2*)dtiles, (uint32_t)ntiles, w.counter(),
                               w.slices(), grid, encode_mode(p.dev.data(), p.dev.size()), w.refine(), s));
         HIP_TRY(hipFreeAsync(w.base, s));
         HIP_TRY(hipFreeAsync(djobs, s));
@@ -787,8 +841,8 @@ int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_pla
     }
     r = choose_tiles(dev, jobs, false);
     if (r) return r;
-    std::vector<uint2> tiles = make_tiles(dev);
-    if (tiles.size() > 0xffffffffu) return fail(ICBC_ERR_INVALID_ARGUMENT, "plan too large");
+    const uint64_t ntiles = tile_count(dev);
+    if (ntiles > 0xffffffffu) return fail(ICBC_ERR_INVALID_ARGUMENT, "plan too large");
     icbc_plan_options o{ICBC_SCHEDULE_PERSISTENT, 1, 0, 1};
     if (opt) {
         o = *opt;
@@ -807,12 +861,12 @@ int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_pla
             if (top >= 0 && share[top] > 0 && resident_workgroups(&resident) == ICBC_OK) {
                 o.workgroups_per_cu = tune.workgroups_per_cu[top];
                 if (tune.schedule[top] == ICBC_SCHEDULE_PERSISTENT ||
-                    tiles.size() <= kGridScheduleMaxWaves * (uint64_t)resident)
+                    ntiles <= kGridScheduleMaxWaves * (uint64_t)resident)
                     o.schedule = tune.schedule[top];
             }
         }
     }
-    if (o.order_by_cost) order_tiles(tiles, dev);
+    const std::vector<uint2> tiles = make_tiles(dev, o.order_by_cost != 0);
     icbc_bc1_plan* pl = new icbc_bc1_plan();
     pl->opt = o;
     pl->ntiles = (uint32_t)tiles.size();
@@ -1102,7 +1156,11 @@ int icbc_gpu_device_name(char* buf, size_t len) {
     HIP_TRY(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, dev));
-    snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    if (!buf || len == 0) return fail(ICBC_ERR_INVALID_ARGUMENT, "empty name buffer");
+    // the marketing name can be empty (no product table for the device in the
+    // runtime); the architecture name and CU count always identify the part
+    snprintf(buf, len, "%s (%s, %d CUs)", prop.name[0] ? prop.name : "AMD GPU", prop.gcnArchName,
+             prop.multiProcessorCount);
     return ICBC_OK;
 }
 
