@@ -47,8 +47,8 @@ struct DecoderWeights {
 extern __constant__ float k_u8_to_unit[256];   // i / 255.0f
 extern __constant__ float k_mid5[32];          // 565 rounding midpoints
 extern __constant__ float k_mid6[64];
-extern __constant__ uint32_t k_four[976];      // c0 | c1 << 8 | c2 << 16
-extern __constant__ uint32_t k_three[160];     // c0 | c1 << 8
+extern __constant__ uint32_t k_four[976];      // c0 | c1 << 8 | c2 << 16 | pair(c0, c1) << 24
+extern __constant__ uint32_t k_three[160];     // c0 | c1 << 8 | (pair p's c0 | c1 << 8) << 16
 extern __constant__ int k_four_total[16];
 extern __constant__ int k_three_total[16];
 extern __constant__ uint16_t k_match5[4][256]; // max | min << 8
@@ -684,12 +684,17 @@ __device__ __forceinline__ float candidate4(float4 s0, float4 s1, float4 s2, flo
 // of the candidate's three SAT rows S0, S1, S2 (X = w1_3 S2 + (w2_3 - w1_3) S1
 // + (1 - w2_3) S0, likewise Wa on the weights, A with the squared palette
 // weights), so X - (Wa / W) t is the same combination of the centred rows
-// R = (S.xyz - (S.w / W) t, S.w), staged once per block.  The bound is
-// computed with fused multiply-adds and the hardware reciprocal (it only
-// decides which candidates are evaluated, never a result); the rounding of
-// this form and of candidate4's own arithmetic is covered by the block's
-// margin (bound_row).  Candidates with a poorly conditioned system
-// (A W - Wa^2 < A W / 16) get -inf: never pruned.
+// R = (S.xyz - (S.w / W) t, S.w), staged once per block with their colour
+// part scaled by the channel weights (then sum_c m_c d_c^2 = |d|^2, m = cw^2).
+// The terms of the first two rows depend only on the pair (c0, c1), of which
+// a block has at most 153: they are summed once per block (pair_sums), and a
+// candidate adds its third row (candidate4_bound_q).  The search prunes on
+// q = |d|^2 / (A W - Wa^2), the bound being -K0 - W q.  Fused multiply-adds
+// and the hardware reciprocal are used (the bound only decides which
+// candidates are evaluated, never a result); the rounding of this form and
+// of candidate4's own arithmetic is covered by the block's margin
+// (bound_row).  Candidates with a poorly conditioned system
+// (A W - Wa^2 < A W / 16) get q = NaN: never pruned.
 struct BoundCoefs {
     float c0, c1, c2;  // rows' weights in X and Wa
     float a0, a1, a2;  // rows' weights in A
@@ -699,24 +704,34 @@ __device__ __forceinline__ BoundCoefs bound_coefs(const DecoderWeights& dw) {
     return BoundCoefs{1.0f - dw.w2_3, dw.w2_3 - dw.w1_3, dw.w1_3, 1.0f - dw.w4_9, dw.w4_9 - dw.w1_9, dw.w1_9};
 }
 
-// Centred SAT row (`invW` = 1 / total weight, `t` = colour totals).
-__device__ __forceinline__ float4 centred_row(float4 s, float invW, f3 t) {
+// Centred SAT row (`invW` = 1 / total weight, `t` = colour totals), colour
+// part scaled by the channel weights `cw`.
+__device__ __forceinline__ float4 centred_row(float4 s, float invW, f3 t, f3 cw) {
     const float f = s.w * invW;
-    return make_float4(__builtin_fmaf(-f, t.x, s.x), __builtin_fmaf(-f, t.y, s.y), __builtin_fmaf(-f, t.z, s.z), s.w);
+    return make_float4(__builtin_fmaf(-f, t.x, s.x) * cw.x, __builtin_fmaf(-f, t.y, s.y) * cw.y,
+                       __builtin_fmaf(-f, t.z, s.z) * cw.z, s.w);
 }
 
-__device__ __forceinline__ float candidate4_bound(float4 r0, float4 r1, float4 r2, float W, float K0, f3 m,
-                                                  const BoundCoefs& k) {
-    const float A = __builtin_fmaf(k.a2, r2.w, __builtin_fmaf(k.a1, r1.w, k.a0 * r0.w));
-    const float Wa = __builtin_fmaf(k.c2, r2.w, __builtin_fmaf(k.c1, r1.w, k.c0 * r0.w));
+// Sums of the bound's terms over the first two rows of the candidates with
+// boundaries (c0, c1, *): `c` for d (xyz) and Wa (w), `a` for A.
+__device__ __forceinline__ void pair_sums(float4 r0, float4 r1, const BoundCoefs& k, float4& c, float& a) {
+    c = make_float4(__builtin_fmaf(k.c1, r1.x, k.c0 * r0.x), __builtin_fmaf(k.c1, r1.y, k.c0 * r0.y),
+                    __builtin_fmaf(k.c1, r1.z, k.c0 * r0.z), __builtin_fmaf(k.c1, r1.w, k.c0 * r0.w));
+    a = __builtin_fmaf(k.a1, r1.w, k.a0 * r0.w);
+}
+
+// q of the candidate with pair sums (pc, pa) and third centred row r2.
+__device__ __forceinline__ float candidate4_bound_q(float4 pc, float pa, float4 r2, float W, const BoundCoefs& k) {
+    const float A = __builtin_fmaf(k.a2, r2.w, pa);
+    const float Wa = __builtin_fmaf(k.c2, r2.w, pc.w);
     const float AW = A * W;
     const float D = __builtin_fmaf(-Wa, Wa, AW);
-    const float dx = __builtin_fmaf(k.c2, r2.x, __builtin_fmaf(k.c1, r1.x, k.c0 * r0.x));
-    const float dy = __builtin_fmaf(k.c2, r2.y, __builtin_fmaf(k.c1, r1.y, k.c0 * r0.y));
-    const float dz = __builtin_fmaf(k.c2, r2.z, __builtin_fmaf(k.c1, r1.z, k.c0 * r0.z));
-    const float sd = __builtin_fmaf(m.z * dz, dz, __builtin_fmaf(m.y * dy, dy, (m.x * dx) * dx));
-    const float lb = __builtin_fmaf(-W * sd, __builtin_amdgcn_rcpf(D), -K0);
-    return (D >= AW * 0.0625f) ? lb : -INFINITY;
+    const float dx = __builtin_fmaf(k.c2, r2.x, pc.x);
+    const float dy = __builtin_fmaf(k.c2, r2.y, pc.y);
+    const float dz = __builtin_fmaf(k.c2, r2.z, pc.z);
+    const float sd = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    const float q = sd * __builtin_amdgcn_rcpf(D);
+    return (D >= AW * 0.0625f) ? q : __builtin_nanf("");
 }
 
 // Margin of a block's four-cluster pruning: 2^-13 sum_c m_c P_c (P = sum of

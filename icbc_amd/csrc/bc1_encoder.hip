@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -66,10 +67,9 @@ __device__ __forceinline__ void argmin_step(float& e, int& i) {
     const float oe = __int_as_float(
         __builtin_amdgcn_update_dpp(__float_as_int(e), __float_as_int(e), kCtrl, kRowMask, 0xf, false));
     const int oi = __builtin_amdgcn_update_dpp(i, i, kCtrl, kRowMask, 0xf, false);
-    if (oe < e || (oe == e && oi < i)) {
-        e = oe;
-        i = oi;
-    }
+    const bool take = (oe < e) | ((oe == e) & (oi < i));  // no short circuit: no branches
+    e = take ? oe : e;
+    i = take ? oi : i;
 }
 
 __device__ __forceinline__ int wave_argmin(float e, int i) {
@@ -85,14 +85,38 @@ __device__ __forceinline__ int wave_argmin(float e, int i) {
 constexpr int kNoWinner = 0x7fffffff;
 
 // Per-workgroup LDS: candidate tables and two SAT buffers (the block being
-// searched and the next one).
+// searched and the next one).  9,984 bytes: at most 10 KiB keeps 16
+// workgroups (4 waves per SIMD) resident per CU.
 struct Shared {
-    uint32_t four[976];
+    uint32_t four[1024];          // 976 entries, padded so 16 rounds of 64 lanes stay inside
     uint32_t three[160];
-    float unit[256];         // k / 255.0f
+    float unit[256];              // k / 255.0f
     float4 sat[2][kSatRows];
     float4 centred[2][kSatRows];  // four-cluster search: the rows centred for the bound pass
-    uint16_t survivors[976];      // four-cluster candidates left by the bound pass
+    union {
+        struct {
+            float4 c[kPairs];     // bound pass: pair sums of d (xyz) and Wa (w)
+            float a[kPairs];      //             pair sums of A
+        } pairs;
+        uint16_t survivors[976];  // after it: the candidates left
+    };
+};
+
+// LDS of the level 0-1 kernel: the tile's texel colours, RGB, one row of 49
+// floats per block (48 + 1 pad: an odd stride keeps the 64 lanes' reads of
+// texel i in distinct banks); 12,544 bytes.
+constexpr int kBoxRow = 49;
+struct SharedBox {
+    float tex[kTile * kBoxRow];
+};
+
+// Texel source over a block's LDS row and its weights in HBM.
+template <bool kWeighted>
+struct TexelsLds {
+    const float* c;  // 48 floats: texel i at 3 i
+    const float* w;  // 16 weights (kWeighted)
+    __device__ __forceinline__ f3 color(int i) const { return mk(c[3 * i], c[3 * i + 1], c[3 * i + 2]); }
+    __device__ __forceinline__ float weight(int i) const { return kWeighted ? w[i] : 1.0f; }
 };
 
 // Four-cluster searches with at least this many colours run the bound pass
@@ -100,14 +124,20 @@ struct Shared {
 constexpr int kPruneMinColors = 9;
 
 // Wave-wide minimum (all lanes get it); NaN inputs must be mapped out first.
+// Which of -0 and +0 survives is unspecified (the value serves as a bound).
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ float min_step(float e) {
+    return fminf(e, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(e), __float_as_int(e), kCtrl, kRowMask,
+                                                               0xf, false)));
+}
+
 __device__ __forceinline__ float wave_min(float e) {
-    int dummy = 0;
-    argmin_step<0xB1, 0xf>(e, dummy);
-    argmin_step<0x4E, 0xf>(e, dummy);
-    argmin_step<0x141, 0xf>(e, dummy);
-    argmin_step<0x140, 0xf>(e, dummy);
-    argmin_step<0x142, 0xa>(e, dummy);
-    argmin_step<0x143, 0xc>(e, dummy);
+    e = min_step<0xB1, 0xf>(e);
+    e = min_step<0x4E, 0xf>(e);
+    e = min_step<0x141, 0xf>(e);
+    e = min_step<0x140, 0xf>(e);
+    e = min_step<0x142, 0xa>(e);
+    e = min_step<0x143, 0xc>(e);
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e), 63));
 }
 
@@ -128,7 +158,7 @@ __device__ __forceinline__ int next_block(int my_n, int tile_blocks, int from, i
 // passed my_alt != 0.
 template <int kClusters>
 __device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offset, int my_alt, int tile_blocks,
-                                      int my_n, f3 metric, const DecoderWeights& dw) {
+                                      int my_n, f3 cw, f3 metric, const DecoderWeights& dw) {
     constexpr int kMinN = kClusters == 4 ? 2 : 1;
     const int lane = threadIdx.x;
     int my_win = -1;
@@ -145,7 +175,7 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offs
         if (kClusters == 4 && nb >= kPruneMinColors) {
             const f3 t = mk(__shfl(row.x, nb, 64), __shfl(row.y, nb, 64), __shfl(row.z, nb, 64));
             const float invW = __builtin_amdgcn_rcpf(__shfl(row.w, nb, 64));
-            if (lane <= nb) sh.centred[buf][lane] = centred_row(row, invW, t);
+            if (lane <= nb) sh.centred[buf][lane] = centred_row(row, invW, t, cw);
         }
         __syncthreads();
         // prefetch the next block's rows while this one is searched
@@ -160,50 +190,66 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offs
         const float4 brow = sat[kBoundRow];
         if (kClusters == 4 && nb >= kPruneMinColors && brow.y != 0.0f) {
             const int total = (nb + 3) * (nb + 2) * (nb + 1) / 6 - 1;  // >= 219: every lane has 3+
+            const int rounds = (total + kTile - 1) / kTile;
             const float margin = brow.x;
-            const f3 t = mk(tot.x, tot.y, tot.z);
-            const float invW = __builtin_amdgcn_rcpf(tot.w);
-            const float K0 = ((metric.x * t.x) * t.x + (metric.y * t.y) * t.y + (metric.z * t.z) * t.z) * invW;
+            const float W = tot.w;
+            const float K0 = ((metric.x * tot.x) * tot.x + (metric.y * tot.y) * tot.y + (metric.z * tot.z) * tot.z) *
+                             __builtin_amdgcn_rcpf(W);
             const BoundCoefs coefs = bound_coefs(dw);
             const float4* cen = sh.centred[buf];
-            // bound pass: lane l holds the bounds of candidates l, l + 64, ...
-            float lb[16];
-            float key_min = INFINITY;
-            int k_min = 0;
+            // pair sums of every boundary pair (those with c1 > nb are not read)
+            for (int p = lane; p < kPairs; p += kTile) {
+                const uint32_t cc = sh.three[p] >> 16;
+                pair_sums(cen[cc & 0xff], cen[cc >> 8], coefs, sh.pairs.c[p], sh.pairs.a[p]);
+            }
+            __syncthreads();
+            // bound pass: lane l holds q of candidates l, l + 64, ... (NaN: keep);
+            // its key is the one with the lowest bound (largest q)
+            float q[16];
+            float qkey = -INFINITY;
+            uint32_t pkey = sh.four[lane];
+            uint32_t pk_next = pkey;  // the next round's table entry is read one round ahead
 #pragma unroll
             for (int k = 0; k < 16; k++) {
-                const int i = lane + kTile * k;
-                lb[k] = INFINITY;
-                if (i < total) {
-                    const uint32_t pk = sh.four[i];
-                    lb[k] = candidate4_bound(cen[pk & 0xff], cen[(pk >> 8) & 0xff], cen[pk >> 16], tot.w, K0, metric,
-                                             coefs);
-                    if (lb[k] > -INFINITY && lb[k] < key_min) {
-                        key_min = lb[k];
-                        k_min = k;
-                    }
+                q[k] = __builtin_nanf("");
+                if (k < rounds) {
+                    const int i = lane + kTile * k;
+                    const uint32_t pk = pk_next;
+                    if (k + 1 < 16) pk_next = sh.four[i + kTile];
+                    const uint32_t pr = pk >> 24;
+                    const float v = candidate4_bound_q(sh.pairs.c[pr], sh.pairs.a[pr], cen[(pk >> 16) & 0xff], W, coefs);
+                    const bool valid = i < total;
+                    q[k] = valid ? v : q[k];
+                    const bool better = valid & (v > qkey);
+                    qkey = better ? v : qkey;
+                    pkey = better ? pk : pkey;
                 }
             }
-            // threshold: the best true error among the lanes' lowest-bound candidates
+            // threshold: the best true error among the lanes' key candidates;
+            // a candidate is pruned when -K0 - W q - margin > threshold
             float thr;
             {
-                const uint32_t pk = sh.four[lane + kTile * k_min];
                 f3 a, e;
-                float err = candidate4(sat[pk & 0xff], sat[(pk >> 8) & 0xff], sat[pk >> 16], tot, metric, dw, a, e);
+                float err = candidate4(sat[pkey & 0xff], sat[(pkey >> 8) & 0xff], sat[(pkey >> 16) & 0xff], tot, metric,
+                                       dw, a, e);
                 thr = wave_min(err == err ? err : FLT_MAX);
             }
-            // survivors (bound - margin <= threshold), in candidate order, to LDS
+            const float qthr = (-K0 - margin - thr) / W;
+            __syncthreads();  // the pair sums are dead: their space takes the survivor list
+            // survivors, in candidate order, to LDS
             int count = 0;
 #pragma unroll
             for (int k = 0; k < 16; k++) {
-                const int i = lane + kTile * k;
-                const bool keep = i < total && !(lb[k] - margin > thr);
-                const uint64_t mask = __ballot(keep);
-                if (keep)
-                    sh.survivors[count + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0))] =
-                        (uint16_t)i;
-                count += __popcll(mask);
+                if (k < rounds) {
+                    const int i = lane + kTile * k;
+                    const bool keep = i < total && !(q[k] < qthr);
+                    const uint64_t mask = __ballot(keep);
+                    if (keep)
+                        sh.survivors[count + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0))] =
+                            (uint16_t)i;
+                    count += __popcll(mask);
+                }
             }
             __syncthreads();
             // the survivors get the exact evaluation; every candidate that could
@@ -212,8 +258,8 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offs
                 const int i = sh.survivors[j];
                 const uint32_t pk = sh.four[i];
                 f3 a, e;
-                const float err = candidate4(sat[pk & 0xff], sat[(pk >> 8) & 0xff], sat[pk >> 16], tot, metric,
-                                             dw, a, e);
+                const float err = candidate4(sat[pk & 0xff], sat[(pk >> 8) & 0xff], sat[(pk >> 16) & 0xff], tot,
+                                             metric, dw, a, e);
                 if (err < best) {
                     best = err;
                     besti = i;
@@ -227,8 +273,8 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offs
                 const uint32_t pk = pk_next;
                 pk_next = sh.four[min(i + kTile, 975)];
                 f3 a, e;
-                const float err = candidate4(sat[pk & 0xff], sat[(pk >> 8) & 0xff], sat[pk >> 16], tot, metric,
-                                             dw, a, e);
+                const float err = candidate4(sat[pk & 0xff], sat[(pk >> 8) & 0xff], sat[(pk >> 16) & 0xff], tot,
+                                             metric, dw, a, e);
                 if (err < best) {
                     best = err;
                     besti = i;
@@ -262,21 +308,19 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offs
 // One tile: the reference's compress_bc1 driver (icbc.h:3734-3808) per lane.
 // `ws` is this workgroup's workspace: 64 blocks x 17 SAT rows.
 
-// Levels 0-1 from register-held texels: box fit, output block, and at level 1
-// least-squares endpoints and a second output block, all reading the block
-// once (icbc.h:3769-3786 without reduce_colors, as at these levels).
-template <bool kWeighted>
-__device__ __forceinline__ float encode_box(const Texels& tx, bool lsq, f3 cw, int dec, const float* unit,
-                                            Code& out) {
-    const TexelRegs<kWeighted> r(tx);
+// Levels 0-1: box fit, output block, and at level 1 least-squares endpoints
+// and a second output block (icbc.h:3769-3786 without reduce_colors, as at
+// these levels).
+template <typename Src>
+__device__ __forceinline__ float encode_box(const Src& t, bool lsq, f3 cw, int dec, const float* unit, Code& out) {
     f3 c0, c1;
-    box_fit([&](int i) { return r.color(i); }, 16, c0, c1);
-    float err = emit4(r, cw, dec, unit, c0, c1, out);
+    box_fit([&](int i) { return t.color(i); }, 16, c0, c1);
+    float err = emit4(t, cw, dec, unit, c0, c1, out);
     if (lsq) {
         f3 a, b;
-        if (lsq_endpoints(out.idx, r, a, b)) {
+        if (lsq_endpoints(out.idx, t, a, b)) {
             Code o2;
-            const float e2 = emit4(r, cw, dec, unit, a, b, o2);
+            const float e2 = emit4(t, cw, dec, unit, a, b, o2);
             if (e2 < err) {
                 err = e2;
                 out = o2;
@@ -284,6 +328,62 @@ __device__ __forceinline__ float encode_box(const Texels& tx, bool lsq, f3 cw, i
         }
     }
     return err;
+}
+
+// Copies the colours of the tile's blocks [first, first + nblk) to LDS: the
+// tile's texels are contiguous in HBM (16 float4 per block), read with
+// coalesced 16-byte loads (lane l takes float4 l, l + 64, ...), all issued
+// before the first is stored.
+__device__ __forceinline__ void stage_tile(const JobDev& job, uint64_t first, int nblk, SharedBox& sh) {
+    const int lane = threadIdx.x;
+    const float4* src = job.colors + first * 16;
+    const int nf = nblk * 16;
+    float4 v[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int f = j * kTile + lane;
+        v[j] = f < nf ? src[f] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int f = j * kTile + lane;
+        float* d = sh.tex + (f >> 4) * kBoxRow + (f & 15) * 3;
+        d[0] = v[j].x;
+        d[1] = v[j].y;
+        d[2] = v[j].z;
+    }
+    __syncthreads();
+}
+
+// A tile of a level 0-1 (or invalid-level) job.
+__device__ __forceinline__ void encode_box_tile(const JobDev& job, uint64_t first, SharedBox& sh) {
+    const int lane = threadIdx.x;
+    const int nblk = (int)min<uint64_t>((uint64_t)job.tile_blocks, job.block_count - first);
+    const bool active = lane < nblk;
+    const uint64_t bi = first + lane;
+    const Plan p = make_plan(job.level, job.tcm != 0, job.tcb != 0);
+    float err = FLT_MAX;  // invalid level: FLT_MAX, output untouched (icbc.h:3766)
+    if (p.box) {
+        stage_tile(job, first, nblk, sh);
+        if (active) {
+            const f3 cw = mk(job.cw0, job.cw1, job.cw2);
+            const float* row = sh.tex + lane * kBoxRow;
+            Code out{0u, 0u};
+            if (job.weights) {
+                const TexelsLds<true> t{row, reinterpret_cast<const float*>(job.weights + bi * 4)};
+                err = encode_box(t, p.lsq, cw, job.decoder, k_u8_to_unit, out);
+            } else {
+                const TexelsLds<false> t{row, nullptr};
+                err = encode_box(t, p.lsq, cw, job.decoder, k_u8_to_unit, out);
+            }
+            job.codes[bi] = make_uint2(out.cols, out.idx);
+        }
+        __syncthreads();  // the next tile restages the rows
+    }
+    if (active) {
+        if (job.errors) job.errors[bi] = err;
+        if (job.stats) job.stats[bi] = make_int4(16, -1, 0, 0);
+    }
 }
 
 // Kernel instances: kModeBox serves launches whose jobs are all at levels
@@ -307,17 +407,6 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
     Code out{0u, 0u};
     bool written = false;
     float err = FLT_MAX;
-    if constexpr (kMode == kModeBox) {
-        if (!active) return;
-        if (p.box) {
-            err = tx.w ? encode_box<true>(tx, p.lsq, cw, dec, sh.unit, out)
-                       : encode_box<false>(tx, p.lsq, cw, dec, sh.unit, out);
-            job.codes[bi] = make_uint2(out.cols, out.idx);
-        }
-        if (job.errors) job.errors[bi] = err;
-        if (job.stats) job.stats[bi] = make_int4(16, -1, 0, 0);
-        return;
-    }
     int n = 16, n3 = -1, steps = 0;
     f3 rc[16];
     float rw[16];
@@ -391,14 +480,15 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
         if (use3) n3 = m;
         __syncthreads();
         // 3-cluster SATs: the skip_blacks table where one was built, else the 4-cluster one
-        const int win4 = search<4>(sh, ws, 0, 0, tile_blocks, fit ? n : 0, metric, dw);
-        const int win3 = search<3>(sh, ws, kTile * kSatRows, skip ? 1 : 0, tile_blocks, use3 ? m : 0, metric, dw);
+        const int win4 = search<4>(sh, ws, 0, 0, tile_blocks, fit ? n : 0, cw, metric, dw);
+        const int win3 = search<3>(sh, ws, kTile * kSatRows, skip ? 1 : 0, tile_blocks, use3 ? m : 0, cw, metric, dw);
 
         if (fit) {
             f3 a4 = mk(0.0f, 0.0f, 0.0f), b4 = mk(0.0f, 0.0f, 0.0f);
             if (win4 >= 0) {
                 const uint32_t pk = sh.four[win4];
-                candidate4(sat4[pk & 0xff], sat4[(pk >> 8) & 0xff], sat4[pk >> 16], sat4[n], metric, dw, a4, b4);
+                candidate4(sat4[pk & 0xff], sat4[(pk >> 8) & 0xff], sat4[(pk >> 16) & 0xff], sat4[n], metric, dw, a4,
+                           b4);
             }
             f3 a3 = mk(0.0f, 0.0f, 0.0f), b3 = mk(0.0f, 0.0f, 0.0f);
             if (use3 && win3 >= 0) {
@@ -449,14 +539,14 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
 // ---------------------------------------------------------------------------
 // kernels
 
-template <int kMode>
-__device__ __forceinline__ void load_tables(Shared& sh) {
-    if (kMode != kModeBox) {
-        for (int i = threadIdx.x; i < 976; i += kTile) sh.four[i] = k_four[i];
+template <int kMode, typename S>
+__device__ __forceinline__ void load_tables(S& sh) {
+    if constexpr (kMode != kModeBox) {
+        for (int i = threadIdx.x; i < 1024; i += kTile) sh.four[i] = k_four[i < 976 ? i : 975];
         for (int i = threadIdx.x; i < 160; i += kTile) sh.three[i] = k_three[i];
+        for (int i = threadIdx.x; i < 256; i += kTile) sh.unit[i] = k_u8_to_unit[i];
+        __syncthreads();
     }
-    for (int i = threadIdx.x; i < 256; i += kTile) sh.unit[i] = k_u8_to_unit[i];
-    __syncthreads();
 }
 
 // Resident workgroups encode tiles until none are left.  With `counter` the
@@ -467,7 +557,7 @@ template <int kMode>
 __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) void bc1_encode_kernel(
     const JobDev* __restrict__ jobs, const uint2* __restrict__ tiles, uint32_t ntiles,
     uint32_t* __restrict__ counter, float4* __restrict__ workspace, RefineList refine) {
-    __shared__ Shared sh;
+    __shared__ std::conditional_t<kMode == kModeBox, SharedBox, Shared> sh;
     load_tables<kMode>(sh);
     float4* ws = workspace + (size_t)blockIdx.x * (kWorkspaceBytesPerWorkgroup / sizeof(float4));
     // queue of this workgroup's XCD: its share is tiles x, x + 8, ...; the
@@ -491,7 +581,8 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) v
             job = tile.x;
             first = tile.y;
         }
-        encode_tile<kMode>(jobs[job], job, first, sh, ws, refine);
+        if constexpr (kMode == kModeBox) encode_box_tile(jobs[job], first, sh);
+        else encode_tile<kMode>(jobs[job], job, first, sh, ws, refine);
         if (queue) {
             uint32_t v = 0;
             if (threadIdx.x == 0) v = atomicAdd(queue, 1u);

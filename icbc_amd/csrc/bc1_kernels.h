@@ -9,6 +9,7 @@ namespace icbc_gpu {
 constexpr int kTile = 64;     // blocks per tile = lanes per wavefront
 constexpr int kSatRows = 18;  // zero row + up to 16 prefix rows + the bound row per block
 constexpr int kBoundRow = 17; // {search margin, prune allowed, 0, 0} of the four-cluster search
+constexpr int kPairs = 153;    // boundary pairs c0 <= c1 <= 16 of the four-cluster partitions
 
 // One job as the kernels see it (device memory, built by the runtime).
 struct JobDev {
