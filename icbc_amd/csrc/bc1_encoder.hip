@@ -208,21 +208,38 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offs
             float q[16];
             float qkey = -INFINITY;
             uint32_t pkey = sh.four[lane];
-            uint32_t pk_next = pkey;  // the next round's table entry is read one round ahead
+            // software pipeline: round k computes while round k + 1's pair
+            // sums and row are read, and round k + 2's table entry
+            uint32_t pk = pkey;
+            float4 pc = sh.pairs.c[pk >> 24];
+            float pa = sh.pairs.a[pk >> 24];
+            float4 r2 = cen[(pk >> 16) & 0xff];
+            uint32_t pk_next = sh.four[lane + kTile];
 #pragma unroll
             for (int k = 0; k < 16; k++) {
                 q[k] = __builtin_nanf("");
                 if (k < rounds) {
                     const int i = lane + kTile * k;
-                    const uint32_t pk = pk_next;
-                    if (k + 1 < 16) pk_next = sh.four[i + kTile];
-                    const uint32_t pr = pk >> 24;
-                    const float v = candidate4_bound_q(sh.pairs.c[pr], sh.pairs.a[pr], cen[(pk >> 16) & 0xff], W, coefs);
+                    float4 pc_n = pc, r2_n = r2;
+                    float pa_n = pa;
+                    uint32_t pk_nn = pk_next;
+                    if (k + 1 < 16) {
+                        pc_n = sh.pairs.c[pk_next >> 24];
+                        pa_n = sh.pairs.a[pk_next >> 24];
+                        r2_n = cen[(pk_next >> 16) & 0xff];
+                    }
+                    if (k + 2 < 16) pk_nn = sh.four[i + 2 * kTile];
+                    const float v = candidate4_bound_q(pc, pa, r2, W, coefs);
                     const bool valid = i < total;
                     q[k] = valid ? v : q[k];
                     const bool better = valid & (v > qkey);
                     qkey = better ? v : qkey;
                     pkey = better ? pk : pkey;
+                    pk = pk_next;
+                    pc = pc_n;
+                    pa = pa_n;
+                    r2 = r2_n;
+                    pk_next = pk_nn;
                 }
             }
             // threshold: the best true error among the lanes' key candidates;
@@ -825,16 +842,35 @@ __global__ void bc1_evaluate_kernel(const uint32_t* __restrict__ rgba, const uin
 // launch glue
 
 int encode_workgroups_per_cu() {
-    int box = 0, cluster = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&box, bc1_encode_kernel<kModeBox>, kTile, 0) != hipSuccess)
-        box = 1;
+    int cluster = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&cluster, bc1_encode_kernel<kModeCluster>, kTile, 0) !=
         hipSuccess)
         cluster = 1;
-    return std::max(1, std::min(box, cluster));
+    return std::max(1, cluster);
 }
 
 namespace {
+
+// Resident workgroups of the level 0-1 instance on the current device (its
+// LDS staging area allows fewer per CU than the cluster instance, whose
+// residency sizes the launches and workspaces).
+uint32_t box_grid() {
+    static std::mutex m;
+    static std::vector<std::pair<int, uint32_t>> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    std::lock_guard<std::mutex> lock(m);
+    for (auto& c : cache)
+        if (c.first == dev) return c.second;
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bc1_encode_kernel<kModeBox>, kTile, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const uint32_t g = (uint32_t)(cus * per_cu);
+    cache.push_back({dev, g});
+    return g;
+}
 
 // Resident workgroups of the refinement kernel on the current device.
 uint32_t refine_grid() {
@@ -872,9 +908,11 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
+    // every schedule covers all tiles with any grid (queue claims or a
+    // grid-stride walk), so the level 0-1 instance runs at its own residency
     if (mode == kModeBox)
-        hipLaunchKernelGGL(bc1_encode_kernel<kModeBox>, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles,
-                           counter, workspace, refine);
+        hipLaunchKernelGGL(bc1_encode_kernel<kModeBox>, dim3(std::min(grid, box_grid())), dim3(kTile), 0, stream, jobs,
+                           tiles, ntiles, counter, workspace, refine);
     else
         hipLaunchKernelGGL(bc1_encode_kernel<kModeCluster>, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles,
                            counter, workspace, refine);

@@ -40,7 +40,9 @@ struct HostTables {
 
 hipError_t upload_tables(const HostTables& t, hipStream_t stream);
 
-// Workgroups of the encode kernel that fit on one compute unit at once.
+// Workgroups of the cluster-fit encode instance that fit on one compute unit
+// at once (sizes launches and workspaces; the level 0-1 instance, which fits
+// fewer, is launched at its own residency).
 int encode_workgroups_per_cu();
 
 // Encodes `ntiles` tiles with `grid` resident workgroups (clamped to ntiles).
