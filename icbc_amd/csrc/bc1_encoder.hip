@@ -119,6 +119,13 @@ struct TexelsLds {
     __device__ __forceinline__ float weight(int i) const { return kWeighted ? w[i] : 1.0f; }
 };
 
+// Texel loops over LDS rows unroll by 8: the reads of 8 texels are issued
+// together (a full unroll spills).
+template <bool kWeighted>
+struct TexelUnroll<TexelsLds<kWeighted>> {
+    static constexpr int value = 8;
+};
+
 // Four-cluster searches with at least this many colours run the bound pass
 // first (smaller ones have too few candidates to gain from it).
 constexpr int kPruneMinColors = 9;
@@ -141,6 +148,20 @@ __device__ __forceinline__ float wave_min(float e) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e), 63));
 }
 
+// Lane l's value of x (l wave-uniform), as a scalar.
+__device__ __forceinline__ float lane_value(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+// A wave-uniform pointer held in scalar registers.
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return reinterpret_cast<T*>(lo | (hi << 32));
+}
+
 // First block b >= from (wave-uniform) whose owner lane holds my_n >= min_n.
 __device__ __forceinline__ int next_block(int my_n, int tile_blocks, int from, int min_n) {
     int b = from;
@@ -157,10 +178,11 @@ __device__ __forceinline__ int next_block(int my_n, int tile_blocks, int from, i
 // Block b's SAT is at ws + b * kSatRows, or alt_offset further when its owner
 // passed my_alt != 0.
 template <int kClusters>
-__device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offset, int my_alt, int tile_blocks,
+__device__ __forceinline__ int search(Shared& sh, const float4* ws_in, int alt_offset, int my_alt, int tile_blocks,
                                       int my_n, f3 cw, f3 metric, const DecoderWeights& dw) {
     constexpr int kMinN = kClusters == 4 ? 2 : 1;
     const int lane = threadIdx.x;
+    const float4* ws = uniform_ptr(ws_in);  // the workgroup's workspace: scalar base
     int my_win = -1;
     int b = next_block(my_n, tile_blocks, 0, kMinN);
     int nb = b < tile_blocks ? __builtin_amdgcn_readlane(my_n, b) : 0;
@@ -173,8 +195,8 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws, int alt_offs
     while (b < tile_blocks) {
         if (lane <= nb || lane == kBoundRow) sh.sat[buf][lane] = row;
         if (kClusters == 4 && nb >= kPruneMinColors) {
-            const f3 t = mk(__shfl(row.x, nb, 64), __shfl(row.y, nb, 64), __shfl(row.z, nb, 64));
-            const float invW = __builtin_amdgcn_rcpf(__shfl(row.w, nb, 64));
+            const f3 t = mk(lane_value(row.x, nb), lane_value(row.y, nb), lane_value(row.z, nb));
+            const float invW = __builtin_amdgcn_rcpf(lane_value(row.w, nb));
             if (lane <= nb) sh.centred[buf][lane] = centred_row(row, invW, t, cw);
         }
         __syncthreads();
@@ -538,7 +560,7 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
     if (hand_mask) {
         uint32_t at = 0;
         if (lane == 0) at = atomicAdd(rl.counters, (uint32_t)__popcll(hand_mask));
-        at = __builtin_amdgcn_readfirstlane(__shfl(at, 0, 64));
+        at = __builtin_amdgcn_readlane(at, 0);
         if (hand_off) {
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(hand_mask >> 32),
                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)hand_mask, 0));
@@ -603,7 +625,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         if (queue) {
             uint32_t v = 0;
             if (threadIdx.x == 0) v = atomicAdd(queue, 1u);
-            j = static_claims + __builtin_amdgcn_readfirstlane(__shfl(v, 0, 64));
+            j = static_claims + __builtin_amdgcn_readlane(v, 0);
         } else {
             t += gridDim.x;
         }
@@ -669,7 +691,7 @@ __global__ __launch_bounds__(kRefineThreads) void bc1_refine_kernel(const JobDev
         if (free_mask && lo == hi && !drained) {
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(claims, chunk);
-            base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
+            base = __builtin_amdgcn_readlane(base, 0);
             lo = min(base, total);
             hi = min(base + chunk, total);
             if (base + chunk >= total) drained = true;
