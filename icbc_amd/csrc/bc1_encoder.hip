@@ -169,32 +169,28 @@ __device__ __forceinline__ int next_block(int my_n, int tile_blocks, int from, i
     return b;
 }
 
-// Wave-cooperative cluster search over every block of the tile whose owner
-// passed my_n >= kMinN: for each such block the 64 lanes split its candidates
-// (table `tab`, `total_of(n)` entries) and the DPP argmin picks the winner,
+// Wave-cooperative four-cluster search (icbc.h:2348-2743) over every block of
+// the tile whose owner passed my_n >= 2: for each such block the 64 lanes
+// split its C(n+3,3)-1 candidates and the DPP argmin picks the winner,
 // returned in the owning lane (-1 if none).  SAT rows stream from the
 // workgroup's workspace through a double-buffered LDS slot: the next block's
 // rows are loaded into registers while the current block is searched.
-// Block b's SAT is at ws + b * kSatRows, or alt_offset further when its owner
-// passed my_alt != 0.
-template <int kClusters>
-__device__ __forceinline__ int search(Shared& sh, const float4* ws_in, int alt_offset, int my_alt, int tile_blocks,
-                                      int my_n, f3 cw, f3 metric, const DecoderWeights& dw) {
-    constexpr int kMinN = kClusters == 4 ? 2 : 1;
+// Block b's SAT is at ws + b * kSatRows.
+__device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile_blocks, int my_n, f3 cw, f3 metric,
+                                       const DecoderWeights& dw) {
+    constexpr int kMinN = 2;
     const int lane = threadIdx.x;
     const float4* ws = uniform_ptr(ws_in);  // the workgroup's workspace: scalar base
     int my_win = -1;
     int b = next_block(my_n, tile_blocks, 0, kMinN);
     int nb = b < tile_blocks ? __builtin_amdgcn_readlane(my_n, b) : 0;
     float4 row = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    auto base = [&](int blk) {
-        return ws + blk * kSatRows + (__builtin_amdgcn_readlane(my_alt, blk) ? alt_offset : 0);
-    };
+    auto base = [&](int blk) { return ws + blk * kSatRows; };
     if (b < tile_blocks && (lane <= nb || lane == kBoundRow)) row = base(b)[lane];
     int buf = 0;
     while (b < tile_blocks) {
         if (lane <= nb || lane == kBoundRow) sh.sat[buf][lane] = row;
-        if (kClusters == 4 && nb >= kPruneMinColors) {
+        if (nb >= kPruneMinColors) {
             const f3 t = mk(lane_value(row.x, nb), lane_value(row.y, nb), lane_value(row.z, nb));
             const float invW = __builtin_amdgcn_rcpf(lane_value(row.w, nb));
             if (lane <= nb) sh.centred[buf][lane] = centred_row(row, invW, t, cw);
@@ -210,7 +206,7 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws_in, int alt_o
         float best = FLT_MAX;
         int besti = kNoWinner;
         const float4 brow = sat[kBoundRow];
-        if (kClusters == 4 && nb >= kPruneMinColors && brow.y != 0.0f) {
+        if (nb >= kPruneMinColors && brow.y != 0.0f) {
             const int total = (nb + 3) * (nb + 2) * (nb + 1) / 6 - 1;  // >= 219: every lane has 3+
             const int rounds = (total + kTile - 1) / kTile;
             const float margin = brow.x;
@@ -304,7 +300,7 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws_in, int alt_o
                     besti = i;
                 }
             }
-        } else if (kClusters == 4) {
+        } else {
             const int total = (nb + 3) * (nb + 2) * (nb + 1) / 6 - 1;  // k_four_total[nb - 1]
             // the next candidate's table entry is read while this one is evaluated
             uint32_t pk_next = sh.four[lane];
@@ -319,25 +315,107 @@ __device__ __forceinline__ int search(Shared& sh, const float4* ws_in, int alt_o
                     besti = i;
                 }
             }
-        } else {
-            const int total = (nb + 2) * (nb + 1) / 2 - 1;  // k_three_total[nb - 1]
-            uint32_t pk_next = sh.three[lane];
-            for (int i = lane; i < total; i += kTile) {
-                const uint32_t pk = pk_next;
-                pk_next = sh.three[min(i + kTile, 159)];
-                f3 a, e;
-                const float err = candidate3(sat[pk & 0xff], sat[(pk >> 8) & 0xff], tot, metric, a, e);
-                if (err < best) {
-                    best = err;
-                    besti = i;
-                }
-            }
         }
         const int w = wave_argmin(best, besti);
         if (lane == b && w != kNoWinner) my_win = w;
         b = b2;
         nb = nb2;
         buf ^= 1;
+    }
+    __syncthreads();
+    return my_win;
+}
+
+// Wave-cooperative three-cluster search (icbc.h:2075-2345) over every block
+// of the tile whose owner passed my_n >= 1, with consecutive blocks' candidate
+// lists packed into shared rounds.  The window holds two blocks, A and B,
+// whose SAT rows sit in the two LDS slots: a round evaluates the next 64
+// positions of the stream "A's remaining candidates, then B's", so a block
+// whose last round is partly empty shares it with the next block.  Each lane
+// keeps its first minimum per block (a block's candidate indices reach a lane
+// in increasing order); once A's last candidate is evaluated the DPP argmin
+// picks its lowest-index minimum for the owning lane (-1 if none), B moves
+// up, and the freed slot takes the rows of the next block, prefetched into
+// registers one block ahead.
+__device__ __forceinline__ int search3(Shared& sh, const float4* ws_in, int alt_offset, int my_alt, int tile_blocks,
+                                       int my_n, f3 metric) {
+    const int lane = threadIdx.x;
+    const float4* ws = uniform_ptr(ws_in);
+    auto base = [&](int blk) {
+        return ws + blk * kSatRows + (__builtin_amdgcn_readlane(my_alt, blk) ? alt_offset : 0);
+    };
+    auto total_of = [](int n) { return (n + 2) * (n + 1) / 2 - 1; };  // k_three_total[n - 1]
+    auto count_of = [&](int blk) { return blk < tile_blocks ? __builtin_amdgcn_readlane(my_n, blk) : 0; };
+    int my_win = -1;
+    int a = next_block(my_n, tile_blocks, 0, 1);
+    int b = a < tile_blocks ? next_block(my_n, tile_blocks, a + 1, 1) : tile_blocks;
+    int c = b < tile_blocks ? next_block(my_n, tile_blocks, b + 1, 1) : tile_blocks;
+    int na = count_of(a), nb = count_of(b), nc = count_of(c);
+    int sa = 0;  // A's slot; B's is sa ^ 1
+    if (a < tile_blocks && lane <= na) sh.sat[0][lane] = base(a)[lane];
+    if (b < tile_blocks && lane <= nb) sh.sat[1][lane] = base(b)[lane];
+    float4 row = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // rows of block c
+    if (c < tile_blocks && lane <= nc) row = base(c)[lane];
+    __syncthreads();
+    int done = 0;  // A's candidates evaluated
+    float best_a = FLT_MAX, best_b = FLT_MAX;
+    int bi_a = kNoWinner, bi_b = kNoWinner;
+    while (a < tile_blocks) {
+        const int ta = total_of(na);
+        const int tb = b < tile_blocks ? total_of(nb) : 0;
+        {
+            const float4* sat_a = sh.sat[sa];
+            const float4* sat_b = sh.sat[sa ^ 1];
+            const int g = done + lane;
+            const bool in_a = g < ta;
+            const int ci = in_a ? g : g - ta;
+            const bool in_b = !in_a && ci < tb;
+            const float4* sat = in_a ? sat_a : sat_b;
+            const float4 tot = in_a ? sat_a[na] : sat_b[nb];
+            const uint32_t pk = sh.three[min(ci, 159)];
+            f3 p0, p1;
+            const float err = candidate3(sat[pk & 0xff], sat[(pk >> 8) & 0xff], tot, metric, p0, p1);
+            if (in_a && err < best_a) {
+                best_a = err;
+                bi_a = ci;
+            }
+            if (in_b && err < best_b) {
+                best_b = err;
+                bi_b = ci;
+            }
+        }
+        if (done + kTile < ta) {
+            done += kTile;
+            continue;
+        }
+        // A is complete; B has its first min(done + 64 - ta, tb) candidates
+        done = min(done + kTile - ta, tb);
+        for (;;) {
+            const int w = wave_argmin(best_a, bi_a);
+            if (lane == a && w != kNoWinner) my_win = w;
+            // B moves up; the freed slot takes the prefetched rows of c
+            best_a = best_b;
+            bi_a = bi_b;
+            best_b = FLT_MAX;
+            bi_b = kNoWinner;
+            const int freed = sa;
+            sa ^= 1;
+            a = b;
+            na = nb;
+            b = c;
+            nb = nc;
+            if (b < tile_blocks && lane <= nb) sh.sat[freed][lane] = row;
+            c = b < tile_blocks ? next_block(my_n, tile_blocks, b + 1, 1) : tile_blocks;
+            nc = count_of(c);
+            if (c < tile_blocks && lane <= nc) row = base(c)[lane];
+            // the new A may already be complete (the round also covered all of it)
+            if (a < tile_blocks && done >= total_of(na)) {
+                done = 0;
+                continue;
+            }
+            break;
+        }
+        __syncthreads();
     }
     __syncthreads();
     return my_win;
@@ -519,8 +597,8 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
         if (use3) n3 = m;
         __syncthreads();
         // 3-cluster SATs: the skip_blacks table where one was built, else the 4-cluster one
-        const int win4 = search<4>(sh, ws, 0, 0, tile_blocks, fit ? n : 0, cw, metric, dw);
-        const int win3 = search<3>(sh, ws, kTile * kSatRows, skip ? 1 : 0, tile_blocks, use3 ? m : 0, cw, metric, dw);
+        const int win4 = search4(sh, ws, tile_blocks, fit ? n : 0, cw, metric, dw);
+        const int win3 = search3(sh, ws, kTile * kSatRows, skip ? 1 : 0, tile_blocks, use3 ? m : 0, metric);
 
         if (fit) {
             f3 a4 = mk(0.0f, 0.0f, 0.0f), b4 = mk(0.0f, 0.0f, 0.0f);
