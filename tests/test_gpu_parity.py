@@ -66,6 +66,46 @@ def test_edge_blocks_every_level(gpu, oracle, level):
             assert np.array_equal(got[2][:, :3], want[2][:, :3]), "stats differ"
 
 
+def _few_colour_blocks(seed, n):
+    """Blocks of k distinct colours (k = 1..6, with near-black texels in some):
+    short three-cluster candidate lists (2..27 entries), so that one 64-lane
+    round of the packed three-cluster search completes several blocks, with
+    single-colour blocks (no search) between them."""
+    rng = np.random.default_rng(seed)
+    out = np.empty((n, 16, 4), np.float32)
+    for b in range(n):
+        k = int(rng.integers(1, 7))
+        pal = rng.random((k, 3), dtype=np.float32)
+        if rng.random() < 0.3:
+            pal[0] = rng.random(3, dtype=np.float32) * 0.02  # a black colour: skip_blacks path
+        pick = rng.integers(0, k, 16)
+        out[b, :, :3] = pal[pick]
+        out[b, :, 3] = 1.0
+    return out
+
+
+@pytest.mark.parametrize("tile", [8, 64])
+def test_few_colour_blocks_packed_three_cluster_rounds(gpu, oracle, tile):
+    colors = _few_colour_blocks(77 + tile, 3000)
+    n = len(colors)
+    dc = gpu.DeviceBuffer.from_array(colors)
+    dcode = gpu.DeviceBuffer(8 * n)
+    derr = gpu.DeviceBuffer(4 * n)
+    dst = gpu.DeviceBuffer(16 * n)
+    try:
+        for level, tcb in ((8, 0), (8, 1), (9, 1)):
+            want = _oracle_run(oracle, level, colors, None, (1, 1, 1), 1, tcb)
+            dcode.fill(0)
+            job = gpu.make_job(level, n, dc, dcode, None, derr, dst, (1, 1, 1), True, bool(tcb), tile_blocks=tile)
+            gpu.compress_batch([job])
+            assert_same(dcode.download(np.uint8, (n, 8)), derr.download(np.float32, n), want[0], want[1],
+                        f"few-colour L{level} tcb={tcb} tile={tile}")
+            assert np.array_equal(dst.download(np.int32, (n, 4))[:, :3], want[2][:, :3]), "stats differ"
+    finally:
+        for b in (dc, dcode, derr, dst):
+            b.close()
+
+
 def test_plain_image_level8(gpu, oracle):
     # 512x512, 50% gradients / 30% value noise / 20% flat, L8, three_color_mode off
     colors, _ = synth.make_image_blocks(1, 512, 512, synth.MIX_PLAIN)
