@@ -763,9 +763,16 @@ __global__ __launch_bounds__(kRefineThreads) void bc1_refine_kernel(const JobDev
     // list counter is touched once per kTile / K blocks, not per free group
     const uint32_t chunk = (uint32_t)(kTile / K);
     uint32_t lo = 0, hi = 0;
+    // starting a walk reads its block's texels from HBM, which stalls the
+    // whole wavefront: free groups start together, once at least a quarter of
+    // the wavefront is free (or all of it)
+    const int refill_groups = max(1, (kTile / 4) / K);
+    const int all_groups = kTile / K;
     for (;;) {
         // free groups (their leader lanes) start the wavefront's next blocks
-        const uint64_t free_mask = __ballot(!has) & leaders;
+        uint64_t free_mask = __ballot(!has) & leaders;
+        const int nfree = __popcll(free_mask);
+        if (nfree < refill_groups && nfree < all_groups) free_mask = 0;
         if (free_mask && lo == hi && !drained) {
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(claims, chunk);
