@@ -167,15 +167,25 @@ __device__ __forceinline__ void palette8(uint32_t c0, uint32_t c1, int decoder, 
     p.r[3] = q.r3; p.g[3] = q.g3; p.b[3] = q.b3;
 }
 
-// palette as floats: channel / 255.0f (icbc.h:1656, 2902-2909), from a table
-// holding exactly those quotients.
-// `unit` is the 256-entry table of k / 255.0f (an LDS copy in the encoder).
-__device__ __forceinline__ void palette_f(uint32_t c0, uint32_t c1, int decoder, const float* unit, f3 pal[4]) {
+// k / 255.0f for k in 0..255, correctly rounded (the value of the IEEE
+// division): the product with the rounded reciprocal, corrected by one
+// fused multiply-add residual step.  The residual k - q * 255 is exact, and
+// the corrected quotient equals the division for every k (checked for all 256
+// values in exact rational arithmetic by the CPU tests).
+__device__ __forceinline__ float u8_to_unit(uint32_t k) {
+    constexpr float kRecip = 1.0f / 255.0f;
+    const float x = (float)k;
+    const float q = x * kRecip;
+    return __builtin_fmaf(__builtin_fmaf(-q, 255.0f, x), kRecip, q);
+}
+
+// palette as floats: channel / 255.0f (icbc.h:1656, 2902-2909).
+__device__ __forceinline__ void palette_f(uint32_t c0, uint32_t c1, int decoder, f3 pal[4]) {
     const PalInts p = palette_ints(c0, c1, decoder);
-    pal[0] = mk(unit[p.r0], unit[p.g0], unit[p.b0]);
-    pal[1] = mk(unit[p.r1], unit[p.g1], unit[p.b1]);
-    pal[2] = mk(unit[p.r2], unit[p.g2], unit[p.b2]);
-    pal[3] = mk(unit[p.r3], unit[p.g3], unit[p.b3]);
+    pal[0] = mk(u8_to_unit(p.r0), u8_to_unit(p.g0), u8_to_unit(p.b0));
+    pal[1] = mk(u8_to_unit(p.r1), u8_to_unit(p.g1), u8_to_unit(p.b1));
+    pal[2] = mk(u8_to_unit(p.r2), u8_to_unit(p.g2), u8_to_unit(p.b2));
+    pal[3] = mk(u8_to_unit(p.r3), u8_to_unit(p.g3), u8_to_unit(p.b3));
 }
 
 // ---------------------------------------------------------------------------
@@ -247,9 +257,9 @@ __device__ __forceinline__ float palette_error(const Texels& t, f3 cw, const f3 
     return e;
 }
 
-__device__ __forceinline__ float code_error(const Texels& t, f3 cw, int decoder, const float* unit, Code c) {
+__device__ __forceinline__ float code_error(const Texels& t, f3 cw, int decoder, Code c) {
     f3 pal[4];
-    palette_f(col0(c), col1(c), decoder, unit, pal);
+    palette_f(col0(c), col1(c), decoder, pal);
     return palette_error(t, cw, pal, c.idx);
 }
 
@@ -349,7 +359,7 @@ __device__ __forceinline__ float indexed_error(const Src& t, f3 w, const f3 pal[
 // Block output (icbc.h:3238-3274)
 
 template <typename Src>
-__device__ __forceinline__ float emit4(const Src& t, f3 cw, int decoder, const float* unit, f3 a, f3 b, Code& out) {
+__device__ __forceinline__ float emit4(const Src& t, f3 cw, int decoder, f3 a, f3 b, Code& out) {
     uint32_t c0 = to565(a), c1 = to565(b);
     if (c0 < c1) {
         uint32_t tmp = c0;
@@ -357,14 +367,14 @@ __device__ __forceinline__ float emit4(const Src& t, f3 cw, int decoder, const f
         c1 = tmp;
     }
     f3 pal[4];
-    palette_f(c0, c1, decoder, unit, pal);
+    palette_f(c0, c1, decoder, pal);
     uint32_t idx;
     const float e = select4(t, cw, pal, idx);
     out = make_code(c0, c1, idx);
     return e;
 }
 
-__device__ __forceinline__ float emit3(const Texels& t, f3 cw, int decoder, const float* unit, bool allow_black,
+__device__ __forceinline__ float emit3(const Texels& t, f3 cw, int decoder, bool allow_black,
                                        f3 a, f3 b, Code& out) {
     uint32_t c0 = to565(a), c1 = to565(b);
     if (c0 > c1) {
@@ -373,7 +383,7 @@ __device__ __forceinline__ float emit3(const Texels& t, f3 cw, int decoder, cons
         c1 = tmp;
     }
     f3 pal[4];
-    palette_f(c0, c1, decoder, unit, pal);
+    palette_f(c0, c1, decoder, pal);
     uint32_t idx;
     const float e = select3(t, cw, allow_black, pal, idx);
     out = make_code(c0, c1, idx);
@@ -383,7 +393,7 @@ __device__ __forceinline__ float emit3(const Texels& t, f3 cw, int decoder, cons
 // output_block4 and, when `with3`, output_block3 (icbc.h:3238-3274) sharing
 // one pass over the texels: per texel both index rules run and each error sum
 // accumulates in texel order, as two separate passes would.
-__device__ __forceinline__ float emit43(const Texels& t, f3 w, int decoder, const float* unit, bool allow_black,
+__device__ __forceinline__ float emit43(const Texels& t, f3 w, int decoder, bool allow_black,
                                         bool with3, f3 a4, f3 b4, f3 a3, f3 b3, Code& out4, Code& out3, float& err3) {
     uint32_t c40 = to565(a4), c41 = to565(b4);
     if (c40 < c41) {
@@ -398,8 +408,8 @@ __device__ __forceinline__ float emit43(const Texels& t, f3 w, int decoder, cons
         c31 = tmp;
     }
     f3 pal4[4], pal3[4];
-    palette_f(c40, c41, decoder, unit, pal4);
-    palette_f(c30, c31, decoder, unit, pal3);
+    palette_f(c40, c41, decoder, pal4);
+    palette_f(c30, c31, decoder, pal3);
     const f3 p0 = pal4[0] * w, p1 = pal4[1] * w, p2 = pal4[2] * w, p3 = pal4[3] * w;
     const f3 q0 = pal3[0] * w, q1 = pal3[1] * w, q2 = pal3[2] * w;
     uint32_t idx4 = 0, idx3 = 0;

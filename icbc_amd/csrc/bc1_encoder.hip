@@ -85,12 +85,11 @@ __device__ __forceinline__ int wave_argmin(float e, int i) {
 constexpr int kNoWinner = 0x7fffffff;
 
 // Per-workgroup LDS: candidate tables and two SAT buffers (the block being
-// searched and the next one).  9,984 bytes: at most 10 KiB keeps 16
+// searched and the next one).  8,960 bytes: at most 10 KiB keeps 16
 // workgroups (4 waves per SIMD) resident per CU.
 struct Shared {
     uint32_t four[1024];          // 976 entries, padded so 16 rounds of 64 lanes stay inside
     uint32_t three[160];
-    float unit[256];              // k / 255.0f
     float4 sat[2][kSatRows];
     float4 centred[2][kSatRows];  // four-cluster search: the rows centred for the bound pass
     union {
@@ -429,15 +428,15 @@ __device__ __forceinline__ int search3(Shared& sh, const float4* ws_in, int alt_
 // and a second output block (icbc.h:3769-3786 without reduce_colors, as at
 // these levels).
 template <typename Src>
-__device__ __forceinline__ float encode_box(const Src& t, bool lsq, f3 cw, int dec, const float* unit, Code& out) {
+__device__ __forceinline__ float encode_box(const Src& t, bool lsq, f3 cw, int dec, Code& out) {
     f3 c0, c1;
     box_fit([&](int i) { return t.color(i); }, 16, c0, c1);
-    float err = emit4(t, cw, dec, unit, c0, c1, out);
+    float err = emit4(t, cw, dec, c0, c1, out);
     if (lsq) {
         f3 a, b;
         if (lsq_endpoints(out.idx, t, a, b)) {
             Code o2;
-            const float e2 = emit4(t, cw, dec, unit, a, b, o2);
+            const float e2 = emit4(t, cw, dec, a, b, o2);
             if (e2 < err) {
                 err = e2;
                 out = o2;
@@ -488,10 +487,10 @@ __device__ __forceinline__ void encode_box_tile(const JobDev& job, uint64_t firs
             Code out{0u, 0u};
             if (job.weights) {
                 const TexelsLds<true> t{row, reinterpret_cast<const float*>(job.weights + bi * 4)};
-                err = encode_box(t, p.lsq, cw, job.decoder, k_u8_to_unit, out);
+                err = encode_box(t, p.lsq, cw, job.decoder, out);
             } else {
                 const TexelsLds<false> t{row, nullptr};
-                err = encode_box(t, p.lsq, cw, job.decoder, k_u8_to_unit, out);
+                err = encode_box(t, p.lsq, cw, job.decoder, out);
             }
             job.codes[bi] = make_uint2(out.cols, out.idx);
         }
@@ -539,7 +538,7 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
     }
     if (!done && n == 1) {  // one colour: optimal single-colour tables (icbc.h:3761-3764)
         out = single_color(to_u8(rc[0].x), to_u8(rc[0].y), to_u8(rc[0].z), dec);
-        err = code_error(tx, cw, dec, sh.unit, out);
+        err = code_error(tx, cw, dec, out);
         written = true;
         done = true;
     }
@@ -548,13 +547,13 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
         f3 c0, c1;
         if (p.cluster) box_fit([&](int i) { return rc[i]; }, n, c0, c1);
         else box_fit([&](int i) { return tx.color(i); }, 16, c0, c1);
-        err = emit4(tx, cw, dec, sh.unit, c0, c1, out);
+        err = emit4(tx, cw, dec, c0, c1, out);
         written = true;
         if (p.lsq) {
             f3 a, b;
             if (lsq_endpoints(out.idx, tx, a, b)) {
                 Code o2;
-                const float e2 = emit4(tx, cw, dec, sh.unit, a, b, o2);
+                const float e2 = emit4(tx, cw, dec, a, b, o2);
                 if (e2 < err) {
                     err = e2;
                     out = o2;
@@ -616,7 +615,7 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
             // output_block4 and output_block3 in one pass over the texels
             Code oc, o3;
             float e3;
-            float ec = emit43(tx, cw, dec, sh.unit, job.tcb != 0, use3, a4, b4, a3, b3, oc, o3, e3);
+            float ec = emit43(tx, cw, dec, job.tcb != 0, use3, a4, b4, a3, b3, oc, o3, e3);
             if (use3 && e3 < ec) {
                 ec = e3;
                 oc = o3;
@@ -661,7 +660,6 @@ __device__ __forceinline__ void load_tables(S& sh) {
     if constexpr (kMode != kModeBox) {
         for (int i = threadIdx.x; i < 1024; i += kTile) sh.four[i] = k_four[i < 976 ? i : 975];
         for (int i = threadIdx.x; i < 160; i += kTile) sh.three[i] = k_three[i];
-        for (int i = threadIdx.x; i < 256; i += kTile) sh.unit[i] = k_u8_to_unit[i];
         __syncthreads();
     }
 }
@@ -735,9 +733,6 @@ constexpr int kRefineThreads = 256;
 
 __global__ __launch_bounds__(kRefineThreads) void bc1_refine_kernel(const JobDev* __restrict__ jobs,
                                                                      RefineList rl) {
-    __shared__ float unit[256];
-    for (int i = threadIdx.x; i < 256; i += kRefineThreads) unit[i] = k_u8_to_unit[i];
-    __syncthreads();
     const uint32_t total = rl.counters[0];
     uint32_t* claims = rl.counters + kQueueStride / sizeof(uint32_t);
     const int lane = threadIdx.x & (kTile - 1);
@@ -803,7 +798,7 @@ __global__ __launch_bounds__(kRefineThreads) void bc1_refine_kernel(const JobDev
                 last = 0;
                 i = 0;
                 f3 sel[4];
-                palette_f(col0(out), col1(out), decoder, unit, sel);
+                palette_f(col0(out), col1(out), decoder, sel);
                 idx = select_indices(r, cw, sel);
                 has = true;
             }
@@ -827,7 +822,7 @@ __global__ __launch_bounds__(kRefineThreads) void bc1_refine_kernel(const JobDev
                 }
             }
             f3 pal[4];
-            palette_f(c0, c1, decoder, unit, pal);
+            palette_f(c0, c1, decoder, pal);
             const float e = indexed_error(r, cw, pal, idx);
             const uint64_t acc = __ballot(valid && e < best) & group_bits;
             bool finished;
@@ -839,7 +834,7 @@ __global__ __launch_bounds__(kRefineThreads) void bc1_refine_kernel(const JobDev
                 best = ej;
                 out = Code{cols, idx};
                 last = i + j;
-                palette_f(cols & 0xffffu, cols >> 16, decoder, unit, pal);
+                palette_f(cols & 0xffffu, cols >> 16, decoder, pal);
                 idx = select_indices(r, cw, pal);
                 i = last + 1;
                 finished = last == 255;
@@ -883,8 +878,8 @@ __global__ void rgba8_to_float_kernel(const uint32_t* __restrict__ src, float4* 
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= texels) return;
     const uint32_t v = src[i];
-    dst[i] = make_float4(k_u8_to_unit[v & 0xff], k_u8_to_unit[(v >> 8) & 0xff], k_u8_to_unit[(v >> 16) & 0xff],
-                         k_u8_to_unit[v >> 24]);
+    dst[i] = make_float4(u8_to_unit(v & 0xff), u8_to_unit((v >> 8) & 0xff), u8_to_unit((v >> 16) & 0xff),
+                         u8_to_unit(v >> 24));
 }
 
 // decode_bc1 (icbc.h:2915-2927): one thread per block, RGBA8 out.
