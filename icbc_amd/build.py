@@ -8,6 +8,13 @@ Compiler flags that the encoder's bit-exactness depends on:
   -fno-gpu-flush-denormals-to-zero     fp32 denormals kept, as on the host
   (HIP's default correctly-rounded fp32 division is left on)
 Packed-fp32 instruction selection is disabled for device code.
+
+Device code is scheduled for instruction-level parallelism
+(-amdgpu-sched-strategy=max-ilp): the encode kernel runs at 4 waves per SIMD
+with long dependency chains, and interleaving independent work within a wave
+beats the default occupancy-first schedule (the kernels carry waves-per-EU
+bounds, so their register budgets, and thus occupancy, are unchanged).
+Scheduling reorders instructions only; it does not change any arithmetic.
 """
 from __future__ import annotations
 
@@ -28,6 +35,7 @@ HIPCC_FLAGS = [
     "-ffp-contract=off", "-fno-fast-math", "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
+    "-mllvm", "-amdgpu-sched-strategy=max-ilp",
 ]
 
 

@@ -731,7 +731,7 @@ __global__ void zero_words_kernel(uint32_t* p, uint32_t n, uint32_t stride) {
 // block from the list (claims are gathered per wavefront).
 constexpr int kRefineThreads = 256;
 
-__global__ __launch_bounds__(kRefineThreads) void bc1_refine_kernel(const JobDev* __restrict__ jobs,
+__global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void bc1_refine_kernel(const JobDev* __restrict__ jobs,
                                                                      RefineList rl) {
     const uint32_t total = rl.counters[0];
     uint32_t* claims = rl.counters + kQueueStride / sizeof(uint32_t);
