@@ -750,6 +750,9 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
     TexelRegs<true> r;
     f3 cw = mk(0.0f, 0.0f, 0.0f);
     bool tcm = false, has = false, drained = false;
+    // `stale`: the walk's best code changed (or the walk just started) and its
+    // indices, which every later step uses, are not recomputed yet
+    bool stale = false;
     Code out{0u, 0u};
     float best = 0.0f;
     int last = 0, i = 0;
@@ -797,14 +800,26 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
                 best = it.error;
                 last = 0;
                 i = 0;
+                has = true;
+                stale = true;
+            }
+        }
+        const uint64_t active = __ballot(has);
+        if (!active && lo == hi && drained) break;  // every lane idle, nothing left to start
+        // index pass: the indices of a new best code are needed before the
+        // walk's next step; lanes wait until a quarter of the active lanes (or
+        // all of them) need it, so one pass serves many walks
+        const int nstale = __popcll(__ballot(has && stale));
+        if (nstale > 0 && (4 * nstale >= __popcll(active) || nstale == __popcll(active))) {
+            if (has && stale) {
                 f3 sel[4];
                 palette_f(col0(out), col1(out), decoder, sel);
                 idx = select_indices(r, cw, sel);
-                has = true;
+                stale = false;
             }
+            continue;
         }
-        if (!__ballot(has) && lo == hi && drained) break;  // every lane idle, nothing left to start
-        if (has) {
+        if (has && !stale) {
             const int limit = min(last + 33, 255);  // the walk's last step unless one is accepted
             const int step = i + sub;
             const bool valid = step <= limit;
@@ -827,17 +842,17 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
             const uint64_t acc = __ballot(valid && e < best) & group_bits;
             bool finished;
             if (acc) {
-                // the first accepted step of the group becomes the best code
+                // the first accepted step of the group becomes the best code;
+                // its indices are the current ones, the next steps need new ones
                 const int j = __builtin_ctzll(acc >> leader);
                 const uint32_t cols = __shfl(c0 | (c1 << 16), leader + j, 64);
                 const float ej = __shfl(e, leader + j, 64);
                 best = ej;
                 out = Code{cols, idx};
                 last = i + j;
-                palette_f(cols & 0xffffu, cols >> 16, decoder, pal);
-                idx = select_indices(r, cw, pal);
                 i = last + 1;
                 finished = last == 255;
+                stale = true;
             } else {
                 finished = i + K - 1 >= limit;
                 i += K;
@@ -851,6 +866,7 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
                     if (jd.stats) reinterpret_cast<int32_t*>(jd.stats + block)[2] = steps;
                 }
                 has = false;
+                stale = false;
             }
         }
     }
