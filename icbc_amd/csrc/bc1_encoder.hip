@@ -147,6 +147,23 @@ __device__ __forceinline__ float wave_min(float e) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e), 63));
 }
 
+// Inclusive prefix sum of v over the wave's lanes (lane l gets the sum over
+// lanes 0..l): shifted adds within each row of 16, then row broadcasts.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ int scan_step(int v) {
+    return v + __builtin_amdgcn_update_dpp(0, v, kCtrl, kRowMask, 0xf, true);
+}
+
+__device__ __forceinline__ int wave_inclusive_sum(int v) {
+    v = scan_step<0x111, 0xf>(v);  // row_shr:1
+    v = scan_step<0x112, 0xf>(v);  // row_shr:2
+    v = scan_step<0x114, 0xf>(v);  // row_shr:4
+    v = scan_step<0x118, 0xf>(v);  // row_shr:8
+    v = scan_step<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v = scan_step<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 // Lane l's value of x (l wave-uniform), as a scalar.
 __device__ __forceinline__ float lane_value(float x, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
@@ -270,31 +287,32 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
             }
             const float qthr = (-K0 - margin - thr) / W;
             __syncthreads();  // the pair sums are dead: their space takes the survivor list
-            // survivors, in candidate order, to LDS
-            int count = 0;
+            // survivors to LDS, lane-major: each lane marks its kept rounds in a
+            // mask, a wave prefix sum of the counts gives its first slot
+            uint32_t keep = 0;
 #pragma unroll
-            for (int k = 0; k < 16; k++) {
-                if (k < rounds) {
-                    const int i = lane + kTile * k;
-                    const bool keep = i < total && !(q[k] < qthr);
-                    const uint64_t mask = __ballot(keep);
-                    if (keep)
-                        sh.survivors[count + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0))] =
-                            (uint16_t)i;
-                    count += __popcll(mask);
-                }
+            for (int k = 0; k < 16; k++)
+                if (k < rounds && lane + kTile * k < total && !(q[k] < qthr)) keep |= 1u << k;
+            const int mine = __popc(keep);
+            const int incl = wave_inclusive_sum(mine);
+            const int count = __builtin_amdgcn_readlane(incl, kTile - 1);
+            int slot = incl - mine;
+            while (keep) {
+                const int k = __builtin_ctz(keep);
+                keep &= keep - 1;
+                sh.survivors[slot++] = (uint16_t)(lane + kTile * k);
             }
             __syncthreads();
             // the survivors get the exact evaluation; every candidate that could
-            // reach the minimum error is among them
+            // reach the minimum error is among them (the list is not in candidate
+            // order: equal errors keep the lower index)
             for (int j = lane; j < count; j += kTile) {
                 const int i = sh.survivors[j];
                 const uint32_t pk = sh.four[i];
                 f3 a, e;
                 const float err = candidate4(sat[pk & 0xff], sat[(pk >> 8) & 0xff], sat[(pk >> 16) & 0xff], tot,
                                              metric, dw, a, e);
-                if (err < best) {
+                if ((err < best) | ((err == best) & (i < besti))) {
                     best = err;
                     besti = i;
                 }
