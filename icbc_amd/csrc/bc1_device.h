@@ -44,7 +44,6 @@ struct DecoderWeights {
     float w1_3, w2_3, w1_9, w4_9, w2_9;
 };
 
-extern __constant__ float k_u8_to_unit[256];   // i / 255.0f
 extern __constant__ float k_mid5[32];          // 565 rounding midpoints
 extern __constant__ float k_mid6[64];
 extern __constant__ uint32_t k_four[976];      // c0 | c1 << 8 | c2 << 16 | pair(c0, c1) << 24

@@ -25,7 +25,6 @@
 
 namespace icbc_gpu {
 
-__constant__ float k_u8_to_unit[256];
 __constant__ float k_mid5[32];
 __constant__ float k_mid6[64];
 __constant__ uint32_t k_four[976];
@@ -42,7 +41,6 @@ hipError_t upload_tables(const HostTables& t, hipStream_t stream) {
     e = hipMemcpyToSymbolAsync(HIP_SYMBOL(sym), &(src), sizeof(src), 0, hipMemcpyHostToDevice, \
                                stream);                                                       \
     if (e != hipSuccess) return e;
-    UP(k_u8_to_unit, t.u8_to_unit);
     UP(k_mid5, t.mid5);
     UP(k_mid6, t.mid6);
     UP(k_four, t.four);

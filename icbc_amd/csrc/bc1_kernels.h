@@ -26,7 +26,6 @@ struct JobDev {
 
 // Tables built by the host at init (all four decoders at once).
 struct HostTables {
-    float u8_to_unit[256];
     float mid5[32];
     float mid6[64];
     uint32_t four[976];

@@ -124,7 +124,6 @@ void build_match(uint16_t* table, int bits, int decoder) {
 }
 
 void build_tables(HostTables& t) {
-    for (int i = 0; i < 256; i++) t.u8_to_unit[i] = i / 255.0f;
     build_midpoints(t.mid5, 5);
     build_midpoints(t.mid6, 6);
     // Cluster partitions in the reference's total order (icbc.h:1965-2034):
