@@ -137,6 +137,7 @@ def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64
     pool is submitted at once to find the saturation rate; then each QPS
     fraction runs for `seconds`.  Every served result is compared with the
     same texture encoded alone."""
+    # (saturation is measured on a warm server: see the untimed passes below)
     import queue
     import threading
 
@@ -157,6 +158,11 @@ def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64
         p = pool[k]
         return srv.submit(p["level"], p["blocks"], None, (1, 1, 1), p["tcm"], p["tcm"], codes=codes)
 
+    # two untimed passes first: the first launches of a server grow its slab
+    # pool, staging and tables (one-time work that a running server has done)
+    for _ in range(2):
+        for t in [submit(k, np.zeros((pool[k]["n"], 8), np.uint8)) for k in range(pool_size)]:
+            srv.wait(t)
     t0 = time.perf_counter()
     tickets = [submit(k, np.zeros((pool[k]["n"], 8), np.uint8)) for k in range(pool_size)]
     for t in tickets:
