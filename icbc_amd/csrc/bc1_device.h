@@ -191,15 +191,26 @@ __device__ __forceinline__ void palette_f(uint32_t c0, uint32_t c1, int decoder,
 // The 16 texels and weights of one block, read from memory on demand (L1/L2
 // resident after the first pass) rather than held in registers.
 
-struct Texels {
-    const float4* c;  // 16 texels, RGBA (alpha unused)
-    const float* w;   // 16 weights, or null for all 1.0
+// A block's 16 texels in HBM: float RGBA (alpha unused), or with kRgba8 16
+// RGBA8 words (R in the low byte) read as c / 255.0f, as the reference's
+// harness converts 8-bit images (icbc_test.cpp:401-405); 16 weights or null
+// for all 1.0.
+template <bool kRgba8>
+struct TexelsOf {
+    const void* c;
+    const float* w;
     __device__ __forceinline__ f3 color(int i) const {
-        const float4 v = c[i];
-        return mk(v.x, v.y, v.z);
+        if constexpr (kRgba8) {
+            const uint32_t v = static_cast<const uint32_t*>(c)[i];
+            return mk(u8_to_unit(v & 0xffu), u8_to_unit((v >> 8) & 0xffu), u8_to_unit((v >> 16) & 0xffu));
+        } else {
+            const float4 v = static_cast<const float4*>(c)[i];
+            return mk(v.x, v.y, v.z);
+        }
     }
     __device__ __forceinline__ float weight(int i) const { return w ? w[i] : 1.0f; }
 };
+using Texels = TexelsOf<false>;
 
 // ---------------------------------------------------------------------------
 // Error of a block for a palette and index word (icbc.h:2955-2958, 3020-3028).
@@ -217,9 +228,13 @@ struct TexelRegs {
     f3 c[16];
     float w[kWeighted ? 16 : 1];
     __device__ __forceinline__ TexelRegs() {}
-    __device__ __forceinline__ explicit TexelRegs(const Texels& t) { load(t); }
+    template <typename Tx>
+    __device__ __forceinline__ explicit TexelRegs(const Tx& t) {
+        load(t);
+    }
     // With kWeighted, a block without weights holds 1.0 (w * x == x exactly).
-    __device__ __forceinline__ void load(const Texels& t) {
+    template <typename Tx>
+    __device__ __forceinline__ void load(const Tx& t) {
 #pragma unroll
         for (int i = 0; i < 16; i++) c[i] = t.color(i);
         if (kWeighted) {
@@ -248,7 +263,8 @@ __device__ __forceinline__ f3 pick(const f3 pal[4], uint32_t k) {
     return (k & 2) ? hi : lo;
 }
 
-__device__ __forceinline__ float palette_error(const Texels& t, f3 cw, const f3 pal[4], uint32_t indices) {
+template <typename Tx>
+__device__ __forceinline__ float palette_error(const Tx& t, f3 cw, const f3 pal[4], uint32_t indices) {
     float e = 0.0f;
 #pragma unroll 4
     for (int i = 0; i < 16; i++)
@@ -256,7 +272,8 @@ __device__ __forceinline__ float palette_error(const Texels& t, f3 cw, const f3 
     return e;
 }
 
-__device__ __forceinline__ float code_error(const Texels& t, f3 cw, int decoder, Code c) {
+template <typename Tx>
+__device__ __forceinline__ float code_error(const Tx& t, f3 cw, int decoder, Code c) {
     f3 pal[4];
     palette_f(col0(c), col1(c), decoder, pal);
     return palette_error(t, cw, pal, c.idx);
@@ -291,7 +308,8 @@ __device__ __forceinline__ float select4(const Src& t, f3 w, const f3 pal[4], ui
 }
 
 // compute_indices3 rule (icbc.h:3133-3176)
-__device__ __forceinline__ float select3(const Texels& t, f3 w, bool allow_black, const f3 pal[4],
+template <typename Tx>
+__device__ __forceinline__ float select3(const Tx& t, f3 w, bool allow_black, const f3 pal[4],
                                          uint32_t& indices) {
     const f3 p0 = pal[0] * w, p1 = pal[1] * w, p2 = pal[2] * w;
     uint32_t out = 0;
@@ -373,7 +391,8 @@ __device__ __forceinline__ float emit4(const Src& t, f3 cw, int decoder, f3 a, f
     return e;
 }
 
-__device__ __forceinline__ float emit3(const Texels& t, f3 cw, int decoder, bool allow_black,
+template <typename Tx>
+__device__ __forceinline__ float emit3(const Tx& t, f3 cw, int decoder, bool allow_black,
                                        f3 a, f3 b, Code& out) {
     uint32_t c0 = to565(a), c1 = to565(b);
     if (c0 > c1) {
@@ -392,7 +411,8 @@ __device__ __forceinline__ float emit3(const Texels& t, f3 cw, int decoder, bool
 // output_block4 and, when `with3`, output_block3 (icbc.h:3238-3274) sharing
 // one pass over the texels: per texel both index rules run and each error sum
 // accumulates in texel order, as two separate passes would.
-__device__ __forceinline__ float emit43(const Texels& t, f3 w, int decoder, bool allow_black,
+template <typename Tx>
+__device__ __forceinline__ float emit43(const Tx& t, f3 w, int decoder, bool allow_black,
                                         bool with3, f3 a4, f3 b4, f3 a3, f3 b3, Code& out4, Code& out3, float& err3) {
     uint32_t c40 = to565(a4), c41 = to565(b4);
     if (c40 < c41) {
@@ -474,7 +494,8 @@ __device__ __forceinline__ bool near3(f3 a, f3 b, float eps) {
 // Merge each weighted texel into the first earlier colour within `threshold`
 // (running weighted average), else append it; texels with weight <= 0 (or
 // NaN) are dropped (icbc.h:1683-1728).  Returns the colour count.
-__device__ __forceinline__ int reduce_colors(const Texels& t, float threshold, f3 (&c)[16], float (&w)[16],
+template <typename Tx>
+__device__ __forceinline__ int reduce_colors(const Tx& t, float threshold, f3 (&c)[16], float (&w)[16],
                                              bool& any_black) {
     int n = 0;
     any_black = false;

@@ -162,6 +162,17 @@ __device__ __forceinline__ int wave_inclusive_sum(int v) {
     return v;
 }
 
+// Texel source of block `bi` of a job: float colours, or RGBA8 colours when
+// kRgba8 (the job's rgba8 flag), and weights.
+template <bool kRgba8>
+__device__ __forceinline__ TexelsOf<kRgba8> block_texels(const JobDev& job, uint64_t bi) {
+    TexelsOf<kRgba8> t;
+    if constexpr (kRgba8) t.c = reinterpret_cast<const uint32_t*>(job.colors) + bi * 16;
+    else t.c = job.colors + bi * 16;
+    t.w = job.weights ? reinterpret_cast<const float*>(job.weights + bi * 4) : nullptr;
+    return t;
+}
+
 // Lane l's value of x (l wave-uniform), as a scalar.
 __device__ __forceinline__ float lane_value(float x, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
@@ -468,8 +479,33 @@ __device__ __forceinline__ float encode_box(const Src& t, bool lsq, f3 cw, int d
 // before the first is stored.
 __device__ __forceinline__ void stage_tile(const JobDev& job, uint64_t first, int nblk, SharedBox& sh) {
     const int lane = threadIdx.x;
-    const float4* src = job.colors + first * 16;
     const int nf = nblk * 16;
+    if (job.rgba8) {
+        // 64 B per block: lane l takes 4 texels (16 B) at l, l + 64, ...
+        const uint4* src8 = reinterpret_cast<const uint4*>(job.colors) + first * 4;
+        const int nq = nblk * 4;
+        uint4 u[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int g = j * kTile + lane;
+            u[j] = g < nq ? src8[g] : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int f = (j * kTile + lane) * 4;  // first texel of the word group
+            float* d = sh.tex + (f >> 4) * kBoxRow + (f & 15) * 3;
+            const uint32_t w4[4] = {u[j].x, u[j].y, u[j].z, u[j].w};
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                d[3 * m] = u8_to_unit(w4[m] & 0xffu);
+                d[3 * m + 1] = u8_to_unit((w4[m] >> 8) & 0xffu);
+                d[3 * m + 2] = u8_to_unit((w4[m] >> 16) & 0xffu);
+            }
+        }
+        __syncthreads();
+        return;
+    }
+    const float4* src = job.colors + first * 16;
     float4 v[16];
 #pragma unroll
     for (int j = 0; j < 16; j++) {
@@ -521,7 +557,9 @@ __device__ __forceinline__ void encode_box_tile(const JobDev& job, uint64_t firs
 // Kernel instances: kModeBox serves launches whose jobs are all at levels
 // 0-1 (or invalid levels), kModeCluster launches without level 9, kModeAll
 // everything (the refinement walk's registers weigh only on that one).
-template <int kMode>
+// kRgba8: the job's colours are RGBA8 words (one instance per texel format,
+// chosen per tile, so neither pays for the other's loads).
+template <int kMode, bool kRgba8>
 __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_index, uint64_t first, Shared& sh,
                                             float4* ws, const RefineList& rl) {
     const int lane = threadIdx.x;
@@ -533,8 +571,7 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
     const f3 cw = mk(job.cw0, job.cw1, job.cw2);
     const DecoderWeights dw = k_dweights[dec];
     const uint64_t safe = active ? bi : 0;
-    const Texels tx{job.colors + safe * 16,
-                    job.weights ? reinterpret_cast<const float*>(job.weights + safe * 4) : nullptr};
+    const TexelsOf<kRgba8> tx = block_texels<kRgba8>(job, safe);
 
     Code out{0u, 0u};
     bool written = false;
@@ -684,7 +721,7 @@ __device__ __forceinline__ void load_tables(S& sh) {
 // tiles are taken from a queue in list order (the batched path); without, a
 // workgroup takes tiles blockIdx.x, blockIdx.x + gridDim.x, ...  With `tiles`
 // null the launch covers one job, tile k = blocks [64 k, 64 k + 64).
-template <int kMode>
+template <int kMode, int kTex>
 __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) void bc1_encode_kernel(
     const JobDev* __restrict__ jobs, const uint2* __restrict__ tiles, uint32_t ntiles,
     uint32_t* __restrict__ counter, float4* __restrict__ workspace, RefineList refine) {
@@ -712,8 +749,13 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) v
             job = tile.x;
             first = tile.y;
         }
-        if constexpr (kMode == kModeBox) encode_box_tile(jobs[job], first, sh);
-        else encode_tile<kMode>(jobs[job], job, first, sh, ws, refine);
+        if constexpr (kMode == kModeBox) encode_box_tile(jobs[job], first, sh);  // either format, per tile
+        else if constexpr (kTex == kTexMixed) {
+            if (jobs[job].rgba8) encode_tile<kMode, true>(jobs[job], job, first, sh, ws, refine);
+            else encode_tile<kMode, false>(jobs[job], job, first, sh, ws, refine);
+        } else {
+            encode_tile<kMode, kTex == kTexRgba8>(jobs[job], job, first, sh, ws, refine);
+        }
         if (queue) {
             uint32_t v = 0;
             if (threadIdx.x == 0) v = atomicAdd(queue, 1u);
@@ -805,10 +847,8 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
                 const JobDev& jd = jobs[it.job];
                 job = it.job;
                 block = it.block;
-                const Texels tx{jd.colors + (uint64_t)block * 16,
-                                jd.weights ? reinterpret_cast<const float*>(jd.weights + (uint64_t)block * 4)
-                                           : nullptr};
-                r.load(tx);
+                if (jd.rgba8) r.load(block_texels<true>(jd, block));
+                else r.load(block_texels<false>(jd, block));
                 cw = mk(jd.cw0, jd.cw1, jd.cw2);
                 tcm = jd.tcm != 0;
                 const uint2 v = jd.codes[block];
@@ -902,18 +942,6 @@ __global__ void bc1_check_reciprocal_kernel(unsigned long long* __restrict__ mis
     if (bad) atomicAdd(mismatches, bad);
 }
 
-// RGBA8 texels -> float RGBA, each channel c / 255.0f as the reference's
-// harness converts 8-bit images (icbc_test.cpp:401-405); alpha is carried
-// along although the encoder never reads it.
-__global__ void rgba8_to_float_kernel(const uint32_t* __restrict__ src, float4* __restrict__ dst,
-                                      uint64_t texels) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= texels) return;
-    const uint32_t v = src[i];
-    dst[i] = make_float4(u8_to_unit(v & 0xff), u8_to_unit((v >> 8) & 0xff), u8_to_unit((v >> 16) & 0xff),
-                         u8_to_unit(v >> 24));
-}
-
 // decode_bc1 (icbc.h:2915-2927): one thread per block, RGBA8 out.
 __global__ void bc1_decode_kernel(const uint2* __restrict__ codes, uint64_t n,
                                   uint32_t* __restrict__ rgba, int decoder) {
@@ -977,7 +1005,7 @@ __global__ void bc1_evaluate_kernel(const uint32_t* __restrict__ rgba, const uin
 
 int encode_workgroups_per_cu() {
     int cluster = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&cluster, bc1_encode_kernel<kModeCluster>, kTile, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&cluster, bc1_encode_kernel<kModeCluster, kTexFloat>, kTile, 0) !=
         hipSuccess)
         cluster = 1;
     return std::max(1, cluster);
@@ -998,7 +1026,8 @@ uint32_t box_grid() {
         if (c.first == dev) return c.second;
     int cus = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bc1_encode_kernel<kModeBox>, kTile, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bc1_encode_kernel<kModeBox, kTexFloat>, kTile, 0) !=
+            hipSuccess ||
         per_cu < 1)
         per_cu = 1;
     const uint32_t g = (uint32_t)(cus * per_cu);
@@ -1037,6 +1066,8 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
+    const int tex = mode >> 4;
+    mode &= 15;
     if (mode == kModeAll) {
         hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, refine.counters, 2u, stride);
         hipError_t e = hipGetLastError();
@@ -1045,11 +1076,17 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
     // every schedule covers all tiles with any grid (queue claims or a
     // grid-stride walk), so the level 0-1 instance runs at its own residency
     if (mode == kModeBox)
-        hipLaunchKernelGGL(bc1_encode_kernel<kModeBox>, dim3(std::min(grid, box_grid())), dim3(kTile), 0, stream, jobs,
-                           tiles, ntiles, counter, workspace, refine);
+        hipLaunchKernelGGL((bc1_encode_kernel<kModeBox, kTexFloat>), dim3(std::min(grid, box_grid())), dim3(kTile), 0,
+                           stream, jobs, tiles, ntiles, counter, workspace, refine);
+    else if (tex == kTexRgba8)
+        hipLaunchKernelGGL((bc1_encode_kernel<kModeCluster, kTexRgba8>), dim3(grid), dim3(kTile), 0, stream, jobs, tiles,
+                           ntiles, counter, workspace, refine);
+    else if (tex == kTexMixed)
+        hipLaunchKernelGGL((bc1_encode_kernel<kModeCluster, kTexMixed>), dim3(grid), dim3(kTile), 0, stream, jobs, tiles,
+                           ntiles, counter, workspace, refine);
     else
-        hipLaunchKernelGGL(bc1_encode_kernel<kModeCluster>, dim3(grid), dim3(kTile), 0, stream, jobs, tiles, ntiles,
-                           counter, workspace, refine);
+        hipLaunchKernelGGL((bc1_encode_kernel<kModeCluster, kTexFloat>), dim3(grid), dim3(kTile), 0, stream, jobs, tiles,
+                           ntiles, counter, workspace, refine);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || mode != kModeAll) return e;
     hipLaunchKernelGGL(bc1_refine_kernel, dim3(refine_grid()), dim3(kRefineThreads), 0, stream, jobs, refine);
@@ -1059,15 +1096,6 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
 hipError_t launch_check_reciprocal(unsigned long long* mismatches, hipStream_t stream) {
     hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, (uint32_t*)mismatches, 2u, 1u);
     hipLaunchKernelGGL(bc1_check_reciprocal_kernel, dim3(4096), dim3(256), 0, stream, mismatches);
-    return hipGetLastError();
-}
-
-hipError_t launch_rgba8_to_float(const uint32_t* src, float4* dst, uint64_t blocks, hipStream_t stream) {
-    if (blocks == 0) return hipSuccess;
-    const uint64_t texels = blocks * 16;
-    const uint32_t threads = 256;
-    hipLaunchKernelGGL(rgba8_to_float_kernel, dim3((uint32_t)((texels + threads - 1) / threads)), dim3(threads), 0,
-                       stream, src, dst, texels);
     return hipGetLastError();
 }
 

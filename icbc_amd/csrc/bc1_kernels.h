@@ -22,6 +22,7 @@ struct JobDev {
     int32_t level, tcm, tcb, decoder;
     float cw0, cw1, cw2;
     int32_t tile_blocks;  // blocks per tile (8, 16, 32 or 64)
+    int32_t rgba8;        // colors holds 16 RGBA8 words per block, read as c / 255.0f
 };
 
 // Tables built by the host at init (all four decoders at once).
@@ -89,8 +90,16 @@ constexpr int kModeBox = 0;
 constexpr int kModeCluster = 1;
 constexpr int kModeAll = 2;
 
-// `mode` selects the kernel instance; encode_mode() gives the one a set of
-// jobs needs.
+// Texel formats of a launch's jobs, in bits 4-5 of a mode: the cluster
+// instance is compiled once per format (all float, all RGBA8, both mixed,
+// which picks per tile), so a launch of one format carries no code for the
+// other.
+constexpr int kTexFloat = 0;
+constexpr int kTexRgba8 = 1;
+constexpr int kTexMixed = 2;
+
+// `mode` selects the kernel instance (level mode | texel format << 4);
+// encode_mode() gives the one a set of jobs needs.
 // With kModeAll, `refine` holds room for the launch's level-9 blocks.
 hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, uint32_t* counter,
                          float4* workspace, uint32_t grid, int mode, RefineList refine, hipStream_t stream);
@@ -110,11 +119,15 @@ inline uint64_t level9_blocks(const JobDev* jobs, size_t n) {
 // The kernel instance a launch over these (host-side) jobs needs.
 inline int encode_mode(const JobDev* jobs, size_t n) {
     int mode = kModeBox;
+    bool any_float = false, any_rgba8 = false;
     for (size_t k = 0; k < n; k++) {
         const int m = encode_mode_of_level(jobs[k].level);
         if (m > mode) mode = m;
+        if (jobs[k].rgba8) any_rgba8 = true;
+        else any_float = true;
     }
-    return mode;
+    const int tex = any_rgba8 ? (any_float ? kTexMixed : kTexRgba8) : kTexFloat;
+    return mode | (tex << 4);
 }
 
 // The one-workgroup-per-tile schedule gives every tile its own workspace
@@ -125,9 +138,6 @@ constexpr uint64_t kGridScheduleMaxWaves = 4;
 // Counts (into *mismatches, zeroed first) the 2^32 float inputs where the
 // encoder's fast reciprocal differs from IEEE-754 division on this device.
 hipError_t launch_check_reciprocal(unsigned long long* mismatches, hipStream_t stream);
-
-// 16 x RGBA8 per block -> 16 x float4 per block (c / 255.0f).
-hipError_t launch_rgba8_to_float(const uint32_t* src, float4* dst, uint64_t blocks, hipStream_t stream);
 
 hipError_t launch_decode(const uint2* codes, uint64_t n, uint32_t* rgba, int decoder,
                          hipStream_t stream);

@@ -406,6 +406,7 @@ JobDev to_dev(const icbc_bc1_job& j, int decoder) {
     d.cw1 = j.color_weights[1];
     d.cw2 = j.color_weights[2];
     d.tile_blocks = j.tile_blocks ? (int32_t)j.tile_blocks : kTile;
+    d.rgba8 = (j.flags & ICBC_JOB_COLORS_RGBA8) ? 1 : 0;  // the kernels read RGBA8 texels directly
     return d;
 }
 
@@ -482,20 +483,16 @@ int stage_out(const icbc_bc1_job& j, hipStream_t s, Staged& st) {
     return ICBC_OK;
 }
 
-// A job whose texels are not already float blocks: RGBA8 blocks or an image,
-// converted into the encoder's layout before the encode launch.
+// An image job: its 4x4 blocks are gathered into the encoder's float layout
+// before the encode launch (RGBA8 blocks need no pass: the kernels read them).
 struct SourceConvert {
-    bool image = false;
-    const uint32_t* rgba8 = nullptr;  // RGBA8 blocks
-    ImageDev img{};                   // image source
-    float4* colors = nullptr;         // float blocks written here
-    float* weights = nullptr;         // alpha as weight (image jobs with ICBC_JOB_WEIGHTS_FROM_ALPHA)
+    ImageDev img{};            // image source
+    float4* colors = nullptr;  // float blocks written here
+    float* weights = nullptr;  // alpha as weight (ICBC_JOB_WEIGHTS_FROM_ALPHA)
     uint64_t blocks = 0;
 };
 
-bool needs_convert(const icbc_bc1_job& j) {
-    return j.block_count && (j.flags & (ICBC_JOB_COLORS_RGBA8 | ICBC_JOB_IMAGE));
-}
+bool needs_convert(const icbc_bc1_job& j) { return j.block_count && (j.flags & ICBC_JOB_IMAGE); }
 
 // Bytes of float blocks (and weights) a converted job needs.
 size_t convert_bytes(const icbc_bc1_job& j) {
@@ -508,23 +505,18 @@ SourceConvert describe_convert(const icbc_bc1_job& j, void* buf, JobDev& d) {
     SourceConvert c;
     c.blocks = j.block_count;
     c.colors = (float4*)buf;
-    if (j.flags & ICBC_JOB_IMAGE) {
-        c.image = true;
-        (void)image_desc(j.image, c.img, "job");  // validated before
-        if (j.flags & ICBC_JOB_WEIGHTS_FROM_ALPHA) {
-            c.weights = (float*)((char*)buf + (size_t)j.block_count * 256);
-            d.weights = (const float4*)c.weights;
-        }
-    } else {
-        c.rgba8 = (const uint32_t*)j.colors;
+    (void)image_desc(j.image, c.img, "job");  // validated before
+    if (j.flags & ICBC_JOB_WEIGHTS_FROM_ALPHA) {
+        c.weights = (float*)((char*)buf + (size_t)j.block_count * 256);
+        d.weights = (const float4*)c.weights;
     }
     d.colors = c.colors;
+    d.rgba8 = 0;
     return c;
 }
 
 hipError_t run_convert(const SourceConvert& c, hipStream_t s) {
-    if (c.image) return launch_image_gather(c.img, c.colors, c.weights, s);
-    return launch_rgba8_to_float(c.rgba8, c.colors, c.blocks, s);
+    return launch_image_gather(c.img, c.colors, c.weights, s);
 }
 
 // Shared front half of batch/plain: validate, stage host jobs, build JobDev list.

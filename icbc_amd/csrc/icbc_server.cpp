@@ -48,10 +48,10 @@ double now_s() {
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-// Device footprint of a request: colours, weights, codes, errors, stats.
+// Device footprint of a request: colours (float or RGBA8, as submitted),
+// weights, codes, errors, stats.
 struct Layout {
     size_t colors = 0, weights = 0, codes = 0, errors = 0, stats = 0, total = 0, color_bytes = 0;
-    size_t floats = 0;  // float texels converted from RGBA8 input (RGBA8 requests only)
 };
 
 Layout layout_of(const icbc_bc1_job& j) {
@@ -61,8 +61,6 @@ Layout layout_of(const icbc_bc1_job& j) {
     l.colors = off;
     l.color_bytes = n * ((j.flags & ICBC_JOB_COLORS_RGBA8) ? 64 : 256);
     off = align_up(off + l.color_bytes, 256);
-    l.floats = off;
-    if (j.flags & ICBC_JOB_COLORS_RGBA8) off = align_up(off + n * 256, 256);
     l.weights = off;
     if (j.weights) off = align_up(off + n * 64, 256);
     l.codes = off;
@@ -216,16 +214,8 @@ int icbc_server::launch(Slot& s, std::vector<Request*>& reqs) {
     for (Request* r : reqs) {
         icbc_bc1_job dj = r->job;
         char* base = (char*)r->dev;
-        dj.flags = 0;
-        if (r->job.flags & ICBC_JOB_COLORS_RGBA8) {
-            if (r->job.block_count &&
-                launch_rgba8_to_float((const uint32_t*)(base + r->lay.colors), (float4*)(base + r->lay.floats),
-                                      r->job.block_count, s.stream) != hipSuccess)
-                return ICBC_ERR_DEVICE;
-            dj.colors = (const float*)(base + r->lay.floats);
-        } else {
-            dj.colors = (const float*)(base + r->lay.colors);
-        }
+        dj.flags = r->job.flags & ICBC_JOB_COLORS_RGBA8;  // RGBA8 texels are read as they are
+        dj.colors = (const float*)(base + r->lay.colors);
         dj.weights = r->job.weights ? (const float*)(base + r->lay.weights) : nullptr;
         dj.codes = base + r->lay.codes;
         dj.errors = r->job.errors ? (float*)(base + r->lay.errors) : nullptr;

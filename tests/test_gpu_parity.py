@@ -268,6 +268,63 @@ def test_rgba8_input_matches_float(gpu, oracle):
         assert_same(gotf[0], gotf[1], want[0], want[1], f"float L{level}")
 
 
+@pytest.mark.parametrize("formats", ["mixed", "rgba8"])
+def test_rgba8_and_float_jobs_in_one_batch(gpu, oracle, formats):
+    # one launch holding 8-bit and float jobs reads each job in its own
+    # format; an all-8-bit launch takes the 8-bit instance of the kernels
+    specs = [(1, 300, True, False), (1, 200, False, True), (8, 500, True, True), (9, 257, False, False),
+             (9, 190, True, True), (0, 64, True, False), (5, 130, False, False), (8, 77, True, False)]
+    if formats == "rgba8":
+        specs = [(lv, n, True, w) for lv, n, _, w in specs]
+    host = []
+    for k, (level, n, is8, weighted) in enumerate(specs):
+        c, _ = synth.make_image_blocks(700 + k, 4 * n, 4, synth.MIX_DEFAULT)
+        c8 = np.clip(np.round(c * 255), 0, 255).astype(np.uint8)
+        cf = c8.astype(np.float32) / np.float32(255)
+        w = np.random.default_rng(k).random((n, 16)).astype(np.float32) if weighted else None
+        host.append((c8 if is8 else cf, cf, w))
+    bufs = []
+
+    def device_jobs():
+        jobs = []
+        for (level, n, is8, _), (c, _, w) in zip(specs, host):
+            dc = gpu.DeviceBuffer.from_array(c)
+            dw = gpu.DeviceBuffer.from_array(w) if w is not None else None
+            dcode = gpu.DeviceBuffer(8 * n)
+            dcode.fill(0x77)
+            derr = gpu.DeviceBuffer(4 * n)
+            bufs.append((dc, dw, dcode, derr))
+            jobs.append(gpu.make_job(level, n, dc, dcode, dw, derr, None, (1, 1, 1), True, True, rgba8=is8))
+        return jobs
+
+    def collect(first):
+        return [(dcode.download(np.uint8, (n, 8)), derr.download(np.float32, n))
+                for (_, n, *_), (_, _, dcode, derr) in zip(specs, bufs[first:first + len(specs)])]
+
+    results = {}
+    for name in ("batch", "plain", "plan"):
+        first = len(bufs)
+        jobs = device_jobs()
+        _say(f"{formats} {name}")
+        if name == "plan":
+            plan = gpu.Plan(jobs)
+            stream = gpu.Stream()
+            for _ in range(2):
+                plan.execute(stream)
+                stream.synchronize()
+                results[name] = collect(first)
+            plan.close()
+            stream.close()
+        else:
+            gpu.compress_batch(jobs, plain=name == "plain")
+            results[name] = collect(first)
+    for k, ((level, n, is8, _), (_, cf, w)) in enumerate(zip(specs, host)):
+        want = _oracle_run(oracle, level, cf, w, (1, 1, 1), 1, 1, np.full((n, 8), 0x77, np.uint8))
+        for name, res in results.items():
+            assert_same(res[k][0], res[k][1], want[0], want[1],
+                        f"{formats} {name} job {k} L{level} {'rgba8' if is8 else 'float'}")
+
+
 def test_decode_and_evaluate(gpu, golden, oracle):
     codes = np.ascontiguousarray(golden["dec_codes"])
     n = len(codes)
