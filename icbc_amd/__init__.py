@@ -697,6 +697,8 @@ class Server:
         r = lib().icbc_server_wait(self.handle, ticket, float(timeout), C.byref(info))
         if r == ERR_TIMEOUT:
             raise TimeoutError(f"request {ticket} still running")
+        if r == ERR_INVALID_ARGUMENT:   # unknown or already collected ticket
+            raise IcbcError(r, last_error())
         colors, w, codes, errors = self._keep.pop(ticket)
         if r != OK:
             raise IcbcError(r, last_error() if r != info.status else f"request failed ({r})")

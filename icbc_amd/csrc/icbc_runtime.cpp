@@ -524,12 +524,12 @@ struct Prepared {
     std::vector<icbc_bc1_job> jobs;  // device-pointer view
     std::vector<Staged> staged;      // per job (empty for device jobs)
     std::vector<JobDev> dev;
-    std::vector<void*> temps;        // float copies of RGBA8 / image inputs, freed after the launch
+    std::vector<void*> temps;        // float blocks gathered from image inputs, freed after the launch
     bool any_host = false;
 };
 
-// RGBA8 and image jobs: convert the texels to float RGBA blocks in
-// stream-ordered scratch and point the device job at them.
+// Image jobs: gather their texels into float RGBA blocks in stream-ordered
+// scratch and point the device job at them.
 int convert_sources(Prepared& p, hipStream_t s) {
     for (size_t k = 0; k < p.jobs.size(); k++) {
         const icbc_bc1_job& j = p.jobs[k];
@@ -801,7 +801,7 @@ struct icbc_bc1_plan {
     JobDev* djobs = nullptr;
     uint2* dtiles = nullptr;
     void* dwork = nullptr;  // tile-queue counters (kQueueBytes) + per-workgroup slices
-    std::vector<SourceConvert> converts;  // RGBA8 / image jobs, re-read on every execute
+    std::vector<SourceConvert> converts;  // image jobs, gathered again on every execute
     std::vector<void*> convert_bufs;
     uint32_t ntiles = 0;
     uint32_t grid = 0;

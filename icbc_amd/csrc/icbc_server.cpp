@@ -293,6 +293,26 @@ int icbc_server::launch(Slot& s, std::vector<Request*>& reqs) {
 void icbc_server::run_scheduler() {
     std::unique_lock<std::mutex> lk(m);
     for (;;) {
+        if (fatal != ICBC_OK && (!admitted.empty() || !pending.empty())) {
+            // after a device failure nothing more is launched: requests not yet
+            // launched end with the failure's status and give back their memory
+            const double t = now_s();
+            for (Request* r : admitted) {
+                pool.give(r->cls, r->dev);
+                r->dev = nullptr;
+                used -= r->cls;
+            }
+            auto finish = [&](Request* r) {
+                r->status = fatal;
+                r->state = kDone;
+                r->t_done = t;
+            };
+            for (Request* r : admitted) finish(r);
+            for (Request* r : pending) finish(r);
+            admitted.clear();
+            pending.clear();
+            cv_done.notify_all();
+        }
         // admit in arrival order while the budget allows
         while (!pending.empty()) {
             Request* r = pending.front();
@@ -506,11 +526,19 @@ int icbc_server_wait(icbc_server* sv, uint64_t ticket, double timeout_s, icbc_re
     auto it = sv->all.find(ticket);
     if (it == sv->all.end()) return icbc_internal_fail(ICBC_ERR_INVALID_ARGUMENT, "unknown ticket %llu",
                                                         (unsigned long long)ticket);
-    Request* q = it->second;
-    const auto pred = [&] { return q->state == kDone; };
+    // the ticket is looked up again after every wake-up: another thread may
+    // have collected it meanwhile
+    const auto pred = [&] {
+        it = sv->all.find(ticket);
+        return it == sv->all.end() || it->second->state == kDone;
+    };
     if (timeout_s < 0) sv->cv_done.wait(lk, pred);
     else if (!sv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred))
         return ICBC_ERR_TIMEOUT;
+    if (it == sv->all.end())
+        return icbc_internal_fail(ICBC_ERR_INVALID_ARGUMENT, "ticket %llu was collected by another wait",
+                                  (unsigned long long)ticket);
+    Request* q = it->second;
     if (info) {
         info->status = q->status;
         info->batch_size = q->batch_size;

@@ -161,7 +161,10 @@ int icbc_server_create(const icbc_server_options* opt, icbc_server** server);
  * never fit the memory budget. */
 int icbc_server_submit(icbc_server* server, const icbc_bc1_job* job, uint64_t* ticket);
 /* Waits for a request (timeout < 0: forever); returns its status (or
- * ICBC_ERR_TIMEOUT) and releases the ticket unless it timed out. */
+ * ICBC_ERR_TIMEOUT) and releases the ticket unless it timed out.  A ticket
+ * is collected once: other waits on it, also ones already blocked, return
+ * ICBC_ERR_INVALID_ARGUMENT.  After a device failure, requests not yet
+ * launched complete with that failure's status. */
 int icbc_server_wait(icbc_server* server, uint64_t ticket, double timeout_s, icbc_request_info* info);
 /* Cancels a request that has not been admitted yet. */
 int icbc_server_cancel(icbc_server* server, uint64_t ticket);

@@ -194,3 +194,28 @@ def test_identity_under_load(gpu):
         srv.close()
         for p in pinned:
             p.close()
+
+
+def test_two_waits_on_one_ticket(gpu):
+    # one wait collects the request; the other, blocked on the same ticket,
+    # gets an argument error instead of touching the released request
+    blocks, _ = synth.make_image_blocks(6, 512, 512, synth.MIX_DEFAULT)
+    srv = gpu.Server(memory_budget=64 << 20)
+    outcome = []
+
+    def waiter(t):
+        try:
+            outcome.append(("ok", srv.wait(t)[2].status))
+        except gpu.IcbcError as e:
+            outcome.append(("error", e.code))
+
+    try:
+        t = srv.submit(9, blocks, None, (1, 1, 1), True, True)
+        ths = [threading.Thread(target=waiter, args=(t,)) for _ in range(2)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+    finally:
+        srv.close()
+    assert sorted(outcome) == [("error", gpu.ERR_INVALID_ARGUMENT), ("ok", 0)], outcome
