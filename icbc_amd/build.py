@@ -15,6 +15,8 @@ with long dependency chains, and interleaving independent work within a wave
 beats the default occupancy-first schedule (the kernels carry waves-per-EU
 bounds, so their register budgets, and thus occupancy, are unchanged).
 Scheduling reorders instructions only; it does not change any arithmetic.
+
+The library exports only the public API (csrc/exports.map).
 """
 from __future__ import annotations
 
@@ -49,7 +51,8 @@ def hipcc():
 def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, *srcs]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"),
+           f"-Wl,--version-script={os.path.join(CSRC, 'exports.map')}", "-o", tmp, *srcs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)

@@ -174,6 +174,49 @@ def test_single_block_c_abi(gpu, oracle):
             assert np.array_equal(out2, want_c) and (np.float32(e2) == np.float32(want_e) or np.isnan(e2))
 
 
+def test_cpp_namespace_api_against_golden(gpu, golden):
+    # the exported icbc:: functions themselves (include/icbc.h), called by
+    # their C++ symbols as a program linked against the library would
+    import ctypes as C
+    lib = C.CDLL(gpu.LIB_PATH)
+    init = lib._ZN4icbc4initENS_7DecoderE
+    init.argtypes, init.restype = [C.c_int], None
+    comp = lib._ZN4icbc12compress_bc1ENS_7QualityEPKfS2_S2_bbPv
+    comp.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_bool, C.c_bool, C.c_void_p]
+    comp.restype = C.c_float
+    dec = lib._ZN4icbc10decode_bc1EPKvPhNS_7DecoderE
+    dec.argtypes, dec.restype = [C.c_void_p, C.c_void_p, C.c_int], None
+    ev = lib._ZN4icbc18evaluate_bc1_errorEPKhPKvNS_7DecoderE
+    ev.argtypes, ev.restype = [C.c_void_p, C.c_void_p, C.c_int], C.c_float
+    colors, weights, prior = golden["colors"], golden["weights"], golden["prior"]
+    try:
+        for k, (decoder, level, tcm, tcb, cwi) in enumerate(golden["cfg"]):
+            if k % 3:
+                continue   # every third configuration, block by block
+            init(int(decoder))
+            cw = np.ascontiguousarray(golden["cw"][cwi], np.float32)
+            for b in range(0, len(colors), 8):
+                c = np.ascontiguousarray(colors[b].reshape(64))
+                w = np.ascontiguousarray(weights[b])
+                out = np.array(prior[b], np.uint8, copy=True)
+                e = comp(int(level), c.ctypes.data, w.ctypes.data, cw.ctypes.data, bool(tcm), bool(tcb),
+                         out.ctypes.data)
+                assert np.array_equal(out, golden["codes"][k][b]), f"config {k} block {b}"
+                want = golden["errors"][k][b]
+                assert np.float32(e).view(np.uint32) == want.view(np.uint32) or (np.isnan(e) and np.isnan(want))
+        for d in range(4):
+            for i in range(0, len(golden["dec_codes"]), 16):
+                code = np.ascontiguousarray(golden["dec_codes"][i])
+                rgba = np.zeros(64, np.uint8)
+                dec(code.ctypes.data, rgba.ctypes.data, d)
+                assert np.array_equal(rgba, golden["dec_rgba"][d][i])
+                src = np.ascontiguousarray(golden["eval_rgba"][i])
+                e = ev(src.ctypes.data, code.ctypes.data, d)
+                assert np.float32(e) == golden["eval_err"][d][i]
+    finally:
+        gpu.init(0)
+
+
 def test_batch_plain_and_plans_agree(gpu, oracle):
     # several jobs with different levels, sizes and options, on device memory
     specs = [(8, 700, (1, 1, 1), 1, 1), (1, 64, (1, 1, 1), 0, 0), (9, 333, (0.3, 0.59, 0.11), 1, 1),
