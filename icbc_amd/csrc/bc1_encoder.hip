@@ -251,38 +251,55 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
             float q[16];
             float qkey = -INFINITY;
             uint32_t pkey = sh.four[lane];
-            // software pipeline: round k computes while round k + 1's pair
-            // sums and row are read, and round k + 2's table entry
-            uint32_t pk = pkey;
-            float4 pc = sh.pairs.c[pk >> 24];
-            float pa = sh.pairs.a[pk >> 24];
-            float4 r2 = cen[(pk >> 16) & 0xff];
-            uint32_t pk_next = sh.four[lane + kTile];
+            if (nb == 16) {
+                // 968 candidates: 15 full rounds and 8 lanes of a 16th, as
+                // straight-line code whose reads the compiler issues early
 #pragma unroll
-            for (int k = 0; k < 16; k++) {
-                q[k] = __builtin_nanf("");
-                if (k < rounds) {
+                for (int k = 0; k < 16; k++) {
                     const int i = lane + kTile * k;
-                    float4 pc_n = pc, r2_n = r2;
-                    float pa_n = pa;
-                    uint32_t pk_nn = pk_next;
-                    if (k + 1 < 16) {
-                        pc_n = sh.pairs.c[pk_next >> 24];
-                        pa_n = sh.pairs.a[pk_next >> 24];
-                        r2_n = cen[(pk_next >> 16) & 0xff];
-                    }
-                    if (k + 2 < 16) pk_nn = sh.four[i + 2 * kTile];
-                    const float v = candidate4_bound_q(pc, pa, r2, W, coefs);
-                    const bool valid = i < total;
-                    q[k] = valid ? v : q[k];
+                    const uint32_t pk = sh.four[i];
+                    const float v = candidate4_bound_q(sh.pairs.c[pk >> 24], sh.pairs.a[pk >> 24],
+                                                       cen[(pk >> 16) & 0xff], W, coefs);
+                    const bool valid = k < 15 || i < total;
+                    q[k] = valid ? v : __builtin_nanf("");
                     const bool better = valid & (v > qkey);
                     qkey = better ? v : qkey;
                     pkey = better ? pk : pkey;
-                    pk = pk_next;
-                    pc = pc_n;
-                    pa = pa_n;
-                    r2 = r2_n;
-                    pk_next = pk_nn;
+                }
+            } else {
+                // software pipeline: round k computes while round k + 1's pair
+                // sums and row are read, and round k + 2's table entry
+                uint32_t pk = pkey;
+                float4 pc = sh.pairs.c[pk >> 24];
+                float pa = sh.pairs.a[pk >> 24];
+                float4 r2 = cen[(pk >> 16) & 0xff];
+                uint32_t pk_next = sh.four[lane + kTile];
+#pragma unroll
+                for (int k = 0; k < 16; k++) {
+                    q[k] = __builtin_nanf("");
+                    if (k < rounds) {
+                        const int i = lane + kTile * k;
+                        float4 pc_n = pc, r2_n = r2;
+                        float pa_n = pa;
+                        uint32_t pk_nn = pk_next;
+                        if (k + 1 < 16) {
+                            pc_n = sh.pairs.c[pk_next >> 24];
+                            pa_n = sh.pairs.a[pk_next >> 24];
+                            r2_n = cen[(pk_next >> 16) & 0xff];
+                        }
+                        if (k + 2 < 16) pk_nn = sh.four[i + 2 * kTile];
+                        const float v = candidate4_bound_q(pc, pa, r2, W, coefs);
+                        const bool valid = i < total;
+                        q[k] = valid ? v : q[k];
+                        const bool better = valid & (v > qkey);
+                        qkey = better ? v : qkey;
+                        pkey = better ? pk : pkey;
+                        pk = pk_next;
+                        pc = pc_n;
+                        pa = pa_n;
+                        r2 = r2_n;
+                        pk_next = pk_nn;
+                    }
                 }
             }
             // threshold: the best true error among the lanes' key candidates;
