@@ -185,6 +185,43 @@ def make_edge_blocks(seed, n):
     return colors.astype(np.float32), weights.astype(np.float32)
 
 
+
+def make_nonfinite_blocks(seed, n):
+    """Blocks with non-finite and extreme inputs: NaN or infinite colour
+    channels and weights, colours of magnitude 1e30, denormal colours, and
+    NaN texels whose weight is zero (dropped before any arithmetic)."""
+    rng = np.random.default_rng(seed)
+    colors = rng.random((n, 16, 4), dtype=np.float32)
+    colors[..., 3] = 1.0
+    weights = np.ones((n, 16), dtype=np.float32)
+    for b in range(n):
+        k = b % 10
+        t = rng.integers(0, 16)
+        ch = rng.integers(0, 3)
+        if k == 0:
+            colors[b, t, ch] = np.nan
+        elif k == 1:
+            colors[b, t, ch] = np.inf
+        elif k == 2:
+            colors[b, t, ch] = -np.inf
+        elif k == 3:
+            colors[b, :, :3] = np.nan
+        elif k == 4:
+            weights[b, t] = np.nan
+        elif k == 5:
+            weights[b, t] = np.inf
+        elif k == 6:
+            colors[b, rng.random(16) < 0.5, :3] *= np.float32(1e30)
+        elif k == 7:
+            colors[b, :, :3] *= np.float32(1e-40)
+        elif k == 8:
+            colors[b, t, :3] = np.nan
+            weights[b, t] = 0.0
+        else:
+            colors[b, t, ch] = np.nan
+            colors[b, (t + 5) % 16, (ch + 1) % 3] = np.inf
+    return colors, weights
+
 # ---------------------------------------------------------------------------
 # textures with mip chains (the batched workload)
 

@@ -450,6 +450,20 @@ def _adversarial_blocks(seed, n):
     return out, wts
 
 
+def test_nonfinite_blocks_every_level(gpu, oracle):
+    # NaN / infinite colours and weights, huge and denormal colours (the
+    # oracle follows the reference build on these, see test_oracle.py)
+    colors, weights = synth.make_nonfinite_blocks(41, 2000)
+    prior = np.random.default_rng(42).integers(0, 256, size=(len(colors), 8), dtype=np.uint8)
+    with np.errstate(all="ignore"):
+        for level in range(10):
+            for tcm in (False, True):
+                _say(f"non-finite L{level} tcm={tcm}")
+                want = _oracle_run(oracle, level, colors, weights, (1, 1, 1), tcm, tcm, prior)
+                got = gpu.compress_many(level, colors, weights, (1, 1, 1), tcm, tcm, prior=prior)
+                assert_same(got[0], got[1], want[0], want[1], f"non-finite L{level} tcm={tcm}")
+
+
 @pytest.mark.parametrize("cw", [(1, 1, 1), (0.3, 0.59, 0.11), (0, 1, 0), (5, 0.001, 1)])
 def test_pruned_search_adversarial_blocks(gpu, oracle, cw):
     blocks, weights = _adversarial_blocks(31, 6000)

@@ -279,3 +279,19 @@ def test_whole_blocks_against_reference_all_decoders(oracle, reference):
                     assert np.array_equal(_bits(eo), _bits(er)), (dec, level, tcm, tcb, cw)
     oracle.init(0)
     reference.init(0)
+
+
+def test_nonfinite_blocks_against_reference(oracle, reference):
+    # NaN / infinite colours and weights, huge and denormal colours: the
+    # restatement follows the reference build through every level
+    colors, weights = synth.make_nonfinite_blocks(41, 200)
+    prior = np.random.default_rng(42).integers(0, 256, size=(len(colors), 8), dtype=np.uint8)
+    with np.errstate(all="ignore"):
+        for level in range(10):
+            for tcm, tcb in ((0, 0), (1, 1)):
+                co, eo, _, _ = oracle.compress_many(level, colors, weights, (1, 1, 1), tcm, tcb, prior=prior)
+                cr, er, _ = reference.compress_many(level, colors, weights, (1, 1, 1), tcm, tcb, prior=prior)
+                bad = np.nonzero(np.any(co != cr, axis=1))[0]
+                assert len(bad) == 0, (level, tcm, bad[:8])
+                same = (_bits(eo) == _bits(er)) | (np.isnan(eo) & np.isnan(er))
+                assert np.all(same), (level, tcm, np.nonzero(~same)[0][:8])
