@@ -806,8 +806,28 @@ __global__ void zero_words_kernel(uint32_t* p, uint32_t n, uint32_t stride) {
 // block from the list (claims are gathered per wavefront).
 constexpr int kRefineThreads = 256;
 
+// Weighted squared error of fixed `indices` against the palette of thread
+// `tid` in `pal` (entry-major: entry k of thread t at pal[k][t]).  Each texel
+// reads its entry with one LDS load instead of selecting it from registers;
+// the 64 lanes of a wavefront read 64 consecutive float4 of one row, or of
+// rows 4 KiB apart, so the reads are free of bank conflicts.  Same values and
+// order of operations as indexed_error.
+template <typename Src>
+__device__ __forceinline__ float indexed_error_lds(const Src& t, f3 w, const float4 (*pal)[kRefineThreads], int tid,
+                                                   uint32_t indices) {
+    float e = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const float4 p = pal[(indices >> (2 * i)) & 3][tid];
+        e += t.weight(i) * texel_error(mk(p.x, p.y, p.z), t.color(i), w);
+    }
+    return e;
+}
+
 __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void bc1_refine_kernel(const JobDev* __restrict__ jobs,
                                                                      RefineList rl) {
+    // each thread's candidate palette, written and read back by the same lane
+    __shared__ float4 s_pal[4][kRefineThreads];
     const uint32_t total = rl.counters[0];
     uint32_t* claims = rl.counters + kQueueStride / sizeof(uint32_t);
     const int lane = threadIdx.x & (kTile - 1);
@@ -911,7 +931,9 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
             }
             f3 pal[4];
             palette_f(c0, c1, decoder, pal);
-            const float e = indexed_error(r, cw, pal, idx);
+#pragma unroll
+            for (int k = 0; k < 4; k++) s_pal[k][threadIdx.x] = make_float4(pal[k].x, pal[k].y, pal[k].z, 0.0f);
+            const float e = indexed_error_lds(r, cw, s_pal, threadIdx.x, idx);
             const uint64_t acc = __ballot(valid && e < best) & group_bits;
             bool finished;
             if (acc) {
