@@ -48,6 +48,27 @@ def test_no_internal_symbols_leak():
     assert sum(s.startswith("_ZN4icbc") for s in exported) == 5
 
 
+def test_programs_link_against_library(tmp_path):
+    # a C caller of the batch ABI and a C++ caller of the namespace icbc API,
+    # written as the reference's users write them, link against the library
+    # (linked only: running them needs a GPU)
+    libdir = os.path.dirname(icbc_amd.LIB_PATH)
+    c = tmp_path / "caller.c"
+    c.write_text('#include "icbc_gpu.h"\n#include <stddef.h>\n'
+                 'int main(void){icbc_bc1_job j = {0}; if (icbc_gpu_init(ICBC_DECODER_D3D10)) return 1;\n'
+                 'return icbc_gpu_compress_bc1_batch(&j, 1, NULL);}\n')
+    subprocess.check_call(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(c),
+                           "-L", libdir, "-licbc_amd", "-o", str(tmp_path / "caller_c")])
+    cpp = tmp_path / "caller.cpp"
+    cpp.write_text('#include "icbc.h"\nint main(){icbc::init();\n'
+                   'float c[64]={0},w[16]={1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1},cw[3]={1,1,1}; unsigned char o[8];\n'
+                   'float e = icbc::compress_bc1(icbc::Quality_Default, c, w, cw, false, false, o);\n'
+                   'unsigned char rgba[64]; icbc::decode_bc1(o, rgba);\n'
+                   'return (int)(e + icbc::evaluate_bc1_error(rgba, o));}\n')
+    subprocess.check_call(["g++", "-std=c++11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(cpp),
+                           "-L", libdir, "-licbc_amd", "-o", str(tmp_path / "caller_cpp")])
+
+
 def test_library_links_hip_runtime_and_has_gfx950_code():
     out = subprocess.check_output(["ldd", icbc_amd.LIB_PATH], text=True)
     assert "libamdhip64" in out
