@@ -665,14 +665,20 @@ __device__ __forceinline__ void build_sat(f3 (&c)[16], float (&w)[16], int n, fl
 // instructions where b and 1/b are both normal (biased exponent 1..252): the
 // hardware reciprocal estimate (within 1 ulp) and one Newton step with fused
 // multiply-adds, e = 1 - b*y0 (exact) and y0 + e*y0.  Zero, denormal, huge,
-// infinite and NaN b take the division itself.  bc1_check_reciprocal_kernel
-// compares the two for all 2^32 bit patterns on the device.
+// infinite and NaN b take the division itself, behind a wavefront-uniform
+// branch: left to itself the compiler computes the division on every call and
+// selects it, about ten instructions per candidate evaluation for a case
+// that almost never occurs.  bc1_check_reciprocal_kernel compares the two for
+// all 2^32 bit patterns on the device.
 __device__ __forceinline__ float recip_rn(float b) {
     const float y0 = __builtin_amdgcn_rcpf(b);
     const float e = __builtin_fmaf(-b, y0, 1.0f);
     float y1 = __builtin_fmaf(e, y0, y0);
     const uint32_t ex = (__float_as_uint(b) >> 23) & 0xffu;
-    if (ex - 1u >= 252u) y1 = 1.0f / b;
+    const bool slow = ex - 1u >= 252u;
+    if (__builtin_expect(__ballot(slow) != 0, 0)) {
+        if (slow) y1 = 1.0f / b;
+    }
     return y1;
 }
 
