@@ -315,10 +315,13 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
             __syncthreads();  // the pair sums are dead: their space takes the survivor list
             // survivors to LDS, lane-major: each lane marks its kept rounds in a
             // mask, a wave prefix sum of the counts gives its first slot
-            uint32_t keep = 0;
+            // (the lane's candidates are rounds 0 .. ceil((total - lane) / 64) - 1,
+            // which also bounds them by `rounds`; total >= 219 > lane)
+            uint32_t pruned = 0;
 #pragma unroll
-            for (int k = 0; k < 16; k++)
-                if (k < rounds && lane + kTile * k < total && !(q[k] < qthr)) keep |= 1u << k;
+            for (int k = 0; k < 16; k++) pruned |= (q[k] < qthr ? 1u : 0u) << k;
+            const int mine_rounds = (total - lane + kTile - 1) / kTile;
+            uint32_t keep = ((1u << mine_rounds) - 1u) & ~pruned;
             const int mine = __popc(keep);
             const int incl = wave_inclusive_sum(mine);
             const int count = __builtin_amdgcn_readlane(incl, kTile - 1);
