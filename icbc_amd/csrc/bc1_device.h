@@ -830,8 +830,10 @@ __device__ __forceinline__ bool lsq_endpoints(uint32_t indices, const Src& t, f3
 #pragma unroll TexelUnroll<Src>::value
     for (int i = 0; i < 16; i++) {
         const uint32_t bits = indices >> (2 * i);
+        // beta = bits & 1, or (1 + beta) / 3 when bit 1 is set: the quotients
+        // are the correctly rounded constants 1/3 and 2/3 (no division per texel)
         float beta = (float)(bits & 1);
-        if (bits & 2) beta = (1 + beta) / 3.0f;
+        if (bits & 2) beta = (bits & 1) ? 2.0f / 3.0f : 1.0f / 3.0f;
         const float alpha = 1.0f - beta;
         a2 += alpha * alpha;
         b2 += beta * beta;
