@@ -55,29 +55,46 @@ hipError_t upload_tables(const HostTables& t, hipStream_t stream) {
 }
 
 // ---------------------------------------------------------------------------
-// Wave-wide argmin over (error, candidate index) with DPP lane moves: smaller
-// error wins, equal errors (including -0 == +0) resolve to the smaller index.
-// Quad swaps, half-row and row mirrors, then the row broadcasts carry the
-// minimum to lane 63, which is read back as a wave-uniform value.
+// Wave-wide argmin over (error, candidate index): smaller error wins, equal
+// errors (including -0 == +0) resolve to the smaller index.  Each lane's pair
+// becomes one 64-bit key, the error's bits mapped to an unsigned order
+// (after -0 + 0 = +0, so that equal errors tie) above the index, and the
+// minimum key is carried to lane 63 by DPP moves: quad swaps, half-row and
+// row mirrors (every lane has a source in its row), then the row broadcasts.
+// Errors here are never NaN (a lane's best only takes values below FLT_MAX).
+
+__device__ __forceinline__ uint64_t argmin_key(float e, int i) {
+    const uint32_t u = __float_as_uint(e + 0.0f);
+    const uint32_t k = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    return ((uint64_t)k << 32) | (uint32_t)i;
+}
+
+template <int kCtrl>
+__device__ __forceinline__ uint64_t key_min_in_row(uint64_t k) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)k, kCtrl, 0xf, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(k >> 32), kCtrl, 0xf, 0xf, false);
+    const uint64_t o = ((uint64_t)hi << 32) | lo;
+    return o < k ? o : k;
+}
 
 template <int kCtrl, int kRowMask>
-__device__ __forceinline__ void argmin_step(float& e, int& i) {
-    const float oe = __int_as_float(
-        __builtin_amdgcn_update_dpp(__float_as_int(e), __float_as_int(e), kCtrl, kRowMask, 0xf, false));
-    const int oi = __builtin_amdgcn_update_dpp(i, i, kCtrl, kRowMask, 0xf, false);
-    const bool take = (oe < e) | ((oe == e) & (oi < i));  // no short circuit: no branches
-    e = take ? oe : e;
-    i = take ? oi : i;
+__device__ __forceinline__ uint64_t key_min_bcast(uint64_t k) {  // rows outside kRowMask keep theirs
+    const int lo0 = (int)(uint32_t)k, hi0 = (int)(uint32_t)(k >> 32);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(lo0, lo0, kCtrl, kRowMask, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(hi0, hi0, kCtrl, kRowMask, 0xf, false);
+    const uint64_t o = ((uint64_t)hi << 32) | lo;
+    return o < k ? o : k;
 }
 
 __device__ __forceinline__ int wave_argmin(float e, int i) {
-    argmin_step<0xB1, 0xf>(e, i);   // quad_perm [1,0,3,2]
-    argmin_step<0x4E, 0xf>(e, i);   // quad_perm [2,3,0,1]
-    argmin_step<0x141, 0xf>(e, i);  // row_half_mirror
-    argmin_step<0x140, 0xf>(e, i);  // row_mirror
-    argmin_step<0x142, 0xa>(e, i);  // row_bcast:15 into rows 1 and 3
-    argmin_step<0x143, 0xc>(e, i);  // row_bcast:31 into rows 2 and 3
-    return __builtin_amdgcn_readlane(i, 63);
+    uint64_t k = argmin_key(e, i);
+    k = key_min_in_row<0xB1>(k);         // quad_perm [1,0,3,2]
+    k = key_min_in_row<0x4E>(k);         // quad_perm [2,3,0,1]
+    k = key_min_in_row<0x141>(k);        // row_half_mirror
+    k = key_min_in_row<0x140>(k);        // row_mirror
+    k = key_min_bcast<0x142, 0xa>(k);    // row_bcast:15 into rows 1 and 3
+    k = key_min_bcast<0x143, 0xc>(k);    // row_bcast:31 into rows 2 and 3
+    return __builtin_amdgcn_readlane((int)(uint32_t)k, 63);
 }
 
 constexpr int kNoWinner = 0x7fffffff;
