@@ -99,14 +99,20 @@ __device__ __forceinline__ int wave_argmin(float e, int i) {
 
 constexpr int kNoWinner = 0x7fffffff;
 
+// Fields of a four-cluster table entry (four_entry).
+__device__ __forceinline__ uint32_t four_c0(uint32_t e) { return (e >> 21) & 31u; }
+__device__ __forceinline__ uint32_t four_c1(uint32_t e) { return (e >> 26) & 31u; }
+__device__ __forceinline__ uint32_t four_c2(uint32_t e) { return (e >> 16) & 31u; }
+
 // Per-workgroup LDS: candidate tables and two SAT buffers (the block being
-// searched and the next one).  8,960 bytes: at most 10 KiB keeps 16
+// searched and the next one).  8,672 bytes: at most 10 KiB keeps 16
 // workgroups (4 waves per SIMD) resident per CU.
 struct Shared {
     uint32_t four[1024];          // 976 entries, padded so 16 rounds of 64 lanes stay inside
     uint32_t three[160];
     float4 sat[2][kSatRows];
-    float4 centred[2][kSatRows];  // four-cluster search: the rows centred for the bound pass
+    float4 centred[kSatRows];     // four-cluster search: the block's rows centred for the bound pass
+                                  // (one buffer: the wavefront's LDS accesses complete in order)
     union {
         struct {
             float4 c[kPairs];     // bound pass: pair sums of d (xyz) and Wa (w)
@@ -115,6 +121,20 @@ struct Shared {
         uint16_t survivors[976];  // after it: the candidates left
     };
 };
+
+// The pruning bound's operands of a four-cluster table entry `e`: its pair's
+// sums and the centred row of its third boundary, each addressed by one
+// bit-field of the entry (a byte offset; the fields' low four bits are zero,
+// so the masks are contiguous and each extract is one instruction).
+__device__ __forceinline__ float4 pair_c(const Shared& sh, uint32_t e) {
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sh.pairs.c) + (e & 0xfffu));
+}
+__device__ __forceinline__ float pair_a(const Shared& sh, uint32_t e) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(sh.pairs.a) + ((e >> 2) & 0x3ffu));
+}
+__device__ __forceinline__ float4 row_at(const float4* rows, uint32_t e) {
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(rows) + ((e >> 12) & 0x1ffu));
+}
 
 // LDS of the level 0-1 kernel: the tile's texel colours, RGB, one row of 49
 // floats per block (48 + 1 pad: an odd stride keeps the 64 lanes' reads of
@@ -235,7 +255,7 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
         if (nb >= kPruneMinColors) {
             const f3 t = mk(lane_value(row.x, nb), lane_value(row.y, nb), lane_value(row.z, nb));
             const float invW = __builtin_amdgcn_rcpf(lane_value(row.w, nb));
-            if (lane <= nb) sh.centred[buf][lane] = centred_row(row, invW, t, cw);
+            if (lane <= nb) sh.centred[lane] = centred_row(row, invW, t, cw);
         }
         __syncthreads();
         // prefetch the next block's rows while this one is searched
@@ -256,7 +276,7 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
             const float K0 = ((metric.x * tot.x) * tot.x + (metric.y * tot.y) * tot.y + (metric.z * tot.z) * tot.z) *
                              __builtin_amdgcn_rcpf(W);
             const BoundCoefs coefs = bound_coefs(dw);
-            const float4* cen = sh.centred[buf];
+            const float4* cen = sh.centred;
             // pair sums of every boundary pair (those with c1 > nb are not read)
             for (int p = lane; p < kPairs; p += kTile) {
                 const uint32_t cc = sh.three[p] >> 16;
@@ -275,8 +295,7 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
                 for (int k = 0; k < 16; k++) {
                     const int i = lane + kTile * k;
                     const uint32_t pk = sh.four[i];
-                    const float v = candidate4_bound_q(sh.pairs.c[pk >> 24], sh.pairs.a[pk >> 24],
-                                                       cen[(pk >> 16) & 0xff], W, coefs);
+                    const float v = candidate4_bound_q(pair_c(sh, pk), pair_a(sh, pk), row_at(cen, pk), W, coefs);
                     const bool valid = k < 15 || i < total;
                     q[k] = valid ? v : __builtin_nanf("");
                     const bool better = valid & (v > qkey);
@@ -287,9 +306,9 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
                 // software pipeline: round k computes while round k + 1's pair
                 // sums and row are read, and round k + 2's table entry
                 uint32_t pk = pkey;
-                float4 pc = sh.pairs.c[pk >> 24];
-                float pa = sh.pairs.a[pk >> 24];
-                float4 r2 = cen[(pk >> 16) & 0xff];
+                float4 pc = pair_c(sh, pk);
+                float pa = pair_a(sh, pk);
+                float4 r2 = row_at(cen, pk);
                 uint32_t pk_next = sh.four[lane + kTile];
 #pragma unroll
                 for (int k = 0; k < 16; k++) {
@@ -300,9 +319,9 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
                         float pa_n = pa;
                         uint32_t pk_nn = pk_next;
                         if (k + 1 < 16) {
-                            pc_n = sh.pairs.c[pk_next >> 24];
-                            pa_n = sh.pairs.a[pk_next >> 24];
-                            r2_n = cen[(pk_next >> 16) & 0xff];
+                            pc_n = pair_c(sh, pk_next);
+                            pa_n = pair_a(sh, pk_next);
+                            r2_n = row_at(cen, pk_next);
                         }
                         if (k + 2 < 16) pk_nn = sh.four[i + 2 * kTile];
                         const float v = candidate4_bound_q(pc, pa, r2, W, coefs);
@@ -324,7 +343,7 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
             float thr;
             {
                 f3 a, e;
-                float err = candidate4(sat[pkey & 0xff], sat[(pkey >> 8) & 0xff], sat[(pkey >> 16) & 0xff], tot, metric,
+                float err = candidate4(sat[four_c0(pkey)], sat[four_c1(pkey)], sat[four_c2(pkey)], tot, metric,
                                        dw, a, e);
                 thr = wave_min(err == err ? err : FLT_MAX);
             }
@@ -356,7 +375,7 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
                 const int i = sh.survivors[j];
                 const uint32_t pk = sh.four[i];
                 f3 a, e;
-                const float err = candidate4(sat[pk & 0xff], sat[(pk >> 8) & 0xff], sat[(pk >> 16) & 0xff], tot,
+                const float err = candidate4(sat[four_c0(pk)], sat[four_c1(pk)], sat[four_c2(pk)], tot,
                                              metric, dw, a, e);
                 if ((err < best) | ((err == best) & (i < besti))) {
                     best = err;
@@ -371,7 +390,7 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
                 const uint32_t pk = pk_next;
                 pk_next = sh.four[min(i + kTile, 975)];
                 f3 a, e;
-                const float err = candidate4(sat[pk & 0xff], sat[(pk >> 8) & 0xff], sat[(pk >> 16) & 0xff], tot,
+                const float err = candidate4(sat[four_c0(pk)], sat[four_c1(pk)], sat[four_c2(pk)], tot,
                                              metric, dw, a, e);
                 if (err < best) {
                     best = err;
@@ -693,7 +712,7 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
             f3 a4 = mk(0.0f, 0.0f, 0.0f), b4 = mk(0.0f, 0.0f, 0.0f);
             if (win4 >= 0) {
                 const uint32_t pk = sh.four[win4];
-                candidate4(sat4[pk & 0xff], sat4[(pk >> 8) & 0xff], sat4[(pk >> 16) & 0xff], sat4[n], metric, dw, a4,
+                candidate4(sat4[four_c0(pk)], sat4[four_c1(pk)], sat4[four_c2(pk)], sat4[n], metric, dw, a4,
                            b4);
             }
             f3 a3 = mk(0.0f, 0.0f, 0.0f), b3 = mk(0.0f, 0.0f, 0.0f);

@@ -25,6 +25,16 @@ struct JobDev {
     int32_t rgba8;        // colors holds 16 RGBA8 words per block, read as c / 255.0f
 };
 
+// A four-cluster table entry holds the partition's boundaries c0 <= c1 <= c2
+// and the index of its pair (c0, c1) among the 153 pairs, laid out so that
+// the pruning bound's LDS addresses are single bit-field extracts: bits 0-11
+// pair << 4 (byte offset of the pair's float4 sums), bits 12-20 c2 << 4 (byte
+// offset of the third boundary's row; c2 itself at bits 16-20), bits 21-25
+// c0, bits 26-30 c1.
+constexpr uint32_t four_entry(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t pair) {
+    return (pair << 4) | (c2 << 16) | (c0 << 21) | (c1 << 26);
+}
+
 // Tables built by the host at init (all four decoders at once).
 struct HostTables {
     float mid5[32];

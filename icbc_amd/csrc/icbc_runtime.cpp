@@ -129,9 +129,9 @@ void build_tables(HostTables& t) {
     // Cluster partitions in the reference's total order (icbc.h:1965-2034):
     // the partitions first reachable with t colours are (c0, c1, t) for
     // 0 <= c0 <= c1 <= t, c0 then c1 ascending; the last entry pads to 976.
-    // Four-cluster entries also carry, in bits 24-31, the index of their
-    // first two boundaries as a pair (0..152), which the pruning bound uses;
-    // bits 16-31 of three-cluster entry p hold that pair's c0 | c1 << 8.
+    // Four-cluster entries (four_entry) also carry the index of their first
+    // two boundaries as a pair (0..152), which the pruning bound uses; bits
+    // 16-31 of three-cluster entry p hold that pair's c0 | c1 << 8.
     // Pairs are numbered c0-major (c0 <= c1 <= 16): consecutive candidates
     // of the table, which differ in c1 only, read consecutive pair entries.
     auto pair_index = [](int c0, int c1) { return (uint32_t)(17 * c0 - c0 * (c0 - 1) / 2 + (c1 - c0)); };
@@ -139,8 +139,7 @@ void build_tables(HostTables& t) {
     for (int tt = 1; tt <= 16; tt++) {
         for (int c0 = 0; c0 <= tt; c0++)
             for (int c1 = c0; c1 <= tt; c1++)
-                t.four[i++] =
-                    (uint32_t)c0 | ((uint32_t)c1 << 8) | ((uint32_t)tt << 16) | (pair_index(c0, c1) << 24);
+                t.four[i++] = four_entry((uint32_t)c0, (uint32_t)c1, (uint32_t)tt, pair_index(c0, c1));
         t.four_total[tt - 1] = i;
     }
     for (; i < 976; i++) t.four[i] = t.four[967];
