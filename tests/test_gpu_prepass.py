@@ -191,3 +191,47 @@ def test_golden_dds_regression(gpu, tmp_path):
     out = tmp_path / "c1.dds"
     gpu.write_container(str(out), gpu.CONTAINER_DDS, 512, 512, codes)
     assert out.read_bytes() == open(golden, "rb").read()
+
+
+def test_plan_with_image_rgba8_and_float_jobs(gpu, oracle):
+    # one plan: an 8-bit device image gathered on every execute, 8-bit blocks
+    # read by the kernels directly, and float blocks, at levels 8 and 9
+    w, h = 150, 90
+    img = (synth.make_image(31, w, h, synth.MIX_DEFAULT) * 255 + 0.5).astype(np.uint8)
+    img_blocks = prepass.gather_fast(img)
+    c8 = np.clip(np.round(synth.make_image_blocks(32, 256, 64, synth.MIX_DEFAULT)[0] * 255), 0, 255).astype(np.uint8)
+    cf = synth.make_image_blocks(33, 192, 64, synth.MIX_DEFAULT)[0]
+    di = gpu.DeviceImage(img)
+    specs = [(9, len(img_blocks)), (8, len(c8)), (9, len(cf))]
+    bufs = [gpu.DeviceBuffer.from_array(c8), gpu.DeviceBuffer.from_array(cf)]
+    outs = [(gpu.DeviceBuffer(8 * n), gpu.DeviceBuffer(4 * n)) for _, n in specs]
+    try:
+        for dcode, _ in outs:
+            dcode.fill(0)
+        jobs = [gpu.make_job(9, len(img_blocks), None, outs[0][0], None, outs[0][1], None, (1, 1, 1), True, True,
+                             image=di.desc),
+                gpu.make_job(8, len(c8), bufs[0], outs[1][0], None, outs[1][1], None, (1, 1, 1), True, True,
+                             rgba8=True),
+                gpu.make_job(9, len(cf), bufs[1], outs[2][0], None, outs[2][1], None, (1, 1, 1), True, True)]
+        plan = gpu.Plan(jobs)
+        stream = gpu.Stream()
+        for _ in range(2):
+            plan.execute(stream)
+            stream.synchronize()
+        plan.close()
+        stream.close()
+        sources = [img_blocks, c8.astype(np.float32) / np.float32(255), cf]
+        for k, ((level, n), src) in enumerate(zip(specs, sources)):
+            want = oracle.compress_many(level, src, None, (1, 1, 1), True, True, nthreads=8,
+                                        prior=np.zeros((n, 8), np.uint8))
+            codes = outs[k][0].download(np.uint8, (n, 8))
+            errs = outs[k][1].download(np.float32, n)
+            assert np.array_equal(codes, want[0]), f"job {k}"
+            assert np.array_equal(errs.view(np.uint32), want[1].view(np.uint32)), f"job {k}"
+    finally:
+        di.close()
+        for b in bufs:
+            b.close()
+        for a, b in outs:
+            a.close()
+            b.close()
