@@ -172,11 +172,17 @@ __device__ __forceinline__ float min_step(float e) {
                                                                0xf, false)));
 }
 
+// Steps whose every lane has a source in its row need no old value.
+template <int kCtrl>
+__device__ __forceinline__ float min_step_in_row(float e) {
+    return fminf(e, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(e), kCtrl, 0xf, 0xf, false)));
+}
+
 __device__ __forceinline__ float wave_min(float e) {
-    e = min_step<0xB1, 0xf>(e);
-    e = min_step<0x4E, 0xf>(e);
-    e = min_step<0x141, 0xf>(e);
-    e = min_step<0x140, 0xf>(e);
+    e = min_step_in_row<0xB1>(e);
+    e = min_step_in_row<0x4E>(e);
+    e = min_step_in_row<0x141>(e);
+    e = min_step_in_row<0x140>(e);
     e = min_step<0x142, 0xa>(e);
     e = min_step<0x143, 0xc>(e);
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e), 63));
