@@ -490,3 +490,33 @@ def test_refinement_group_size_does_not_change_results(gpu, oracle):
     want = oracle.compress_many(9, blocks[pick], None, (1, 1, 1), True, True, nthreads=8)
     assert np.array_equal(small[0], want[0])
     assert np.array_equal(small[1].view(np.uint32), want[1].view(np.uint32))
+
+
+def test_sweep_decoders_levels_options(gpu, oracle):
+    # every decoder x level x three-colour setting, random channel weights,
+    # on one mixed set: image content (some blocks weighted), edge cases and
+    # non-finite inputs, with random prior output bytes
+    rng = np.random.default_rng(101)
+    c1, _ = synth.make_image_blocks(101, 256, 256, synth.MIX_DEFAULT)
+    ec, ew = synth.make_edge_blocks(103, 1500)
+    nc, nw = synth.make_nonfinite_blocks(104, 500)
+    colors = np.concatenate([c1, ec, nc])
+    w1 = np.where(rng.random((len(c1), 1)) < 0.3, rng.random((len(c1), 16)), 1.0).astype(np.float32)
+    weights = np.concatenate([w1, ew, nw])
+    prior = rng.integers(0, 256, size=(len(colors), 8), dtype=np.uint8)
+    try:
+        for dec in range(4):
+            gpu.init(dec)
+            oracle.init(dec)
+            _say(f"sweep decoder {dec}")
+            for level in range(10):
+                for tcm, tcb in ((0, 0), (1, 0), (1, 1)):
+                    cw = tuple(float(x) for x in rng.random(3) * 2) if rng.random() < 0.5 else (1.0, 1.0, 1.0)
+                    with np.errstate(all="ignore"):
+                        want = _oracle_run(oracle, level, colors, weights, cw, tcm, tcb, prior)
+                        got = gpu.compress_many(level, colors, weights, cw, tcm, tcb, prior=prior)
+                        assert_same(got[0], got[1], want[0], want[1],
+                                    f"decoder {dec} L{level} tcm={tcm} tcb={tcb} cw={cw}")
+    finally:
+        gpu.init(0)
+        oracle.init(0)
