@@ -857,14 +857,17 @@ constexpr int kRefineThreads = 256;
 // the 64 lanes of a wavefront read 64 consecutive float4 of one row, or of
 // rows 4 KiB apart, so the reads are free of bank conflicts.  Same values and
 // order of operations as indexed_error.
-template <typename Src>
+// With kWeighted false the texels' weights are all 1.0 and the products
+// w * error, which equal the errors exactly, are not formed.
+template <bool kWeighted, typename Src>
 __device__ __forceinline__ float indexed_error_lds(const Src& t, f3 w, const float4 (*pal)[kRefineThreads], int tid,
                                                    uint32_t indices) {
     float e = 0.0f;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         const float4 p = pal[(indices >> (2 * i)) & 3][tid];
-        e += t.weight(i) * texel_error(mk(p.x, p.y, p.z), t.color(i), w);
+        const float te = texel_error(mk(p.x, p.y, p.z), t.color(i), w);
+        e += kWeighted ? t.weight(i) * te : te;
     }
     return e;
 }
@@ -890,6 +893,7 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
     TexelRegs<true> r;
     f3 cw = mk(0.0f, 0.0f, 0.0f);
     bool tcm = false, has = false, drained = false;
+    bool weighted = false;  // the walk's block has weights (else all 1.0)
     // `stale`: the walk's best code changed (or the walk just started) and its
     // indices, which every later step uses, are not recomputed yet
     bool stale = false;
@@ -933,6 +937,7 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
                 else r.load(block_texels<false>(jd, block));
                 cw = mk(jd.cw0, jd.cw1, jd.cw2);
                 tcm = jd.tcm != 0;
+                weighted = jd.weights != nullptr;
                 const uint2 v = jd.codes[block];
                 out = Code{v.x, v.y};
                 best = it.error;
@@ -978,7 +983,9 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
             palette_f(c0, c1, decoder, pal);
 #pragma unroll
             for (int k = 0; k < 4; k++) s_pal[k][threadIdx.x] = make_float4(pal[k].x, pal[k].y, pal[k].z, 0.0f);
-            const float e = indexed_error_lds(r, cw, s_pal, threadIdx.x, idx);
+            // the unweighted form when no lane taking a step walks a weighted block
+            const float e = __ballot(weighted) != 0 ? indexed_error_lds<true>(r, cw, s_pal, threadIdx.x, idx)
+                                                    : indexed_error_lds<false>(r, cw, s_pal, threadIdx.x, idx);
             const uint64_t acc = __ballot(valid && e < best) & group_bits;
             bool finished;
             if (acc) {
