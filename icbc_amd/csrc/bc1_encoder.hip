@@ -296,7 +296,8 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
             uint32_t pkey = sh.four[lane];
             if (nb == 16) {
                 // 968 candidates: 15 full rounds and 8 lanes of a 16th, as
-                // straight-line code whose reads the compiler issues early
+                // straight-line code: no per-round branch, so the rounds form
+                // one basic block without register copies between them
 #pragma unroll
                 for (int k = 0; k < 16; k++) {
                     const int i = lane + kTile * k;
