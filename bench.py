@@ -15,10 +15,17 @@ A step is one encode of the whole batch with inputs resident in HBM:
 
 Also reported: the dominant kernel's roofline (algorithmic FP32 FLOPs from the
 encoder's own per-block work counters over its average duration, measured
-with HIP events on the launch stream), and the reference encoder's own SIMD
-build run multithreaded on this host on a bounded sample as the CPU baseline.
+with HIP events on the launch stream), the reference encoder's own SIMD
+build run multithreaded on every host core on a bounded random sample as the
+CPU baseline, and, for every configuration, a parity check of a seeded sample
+of the GPU's codes and errors against the oracle (any mismatch makes the run
+exit non-zero) and the encoded quality as RMSE / PSNR, computed as the
+reference's harness does (icbc_test.cpp:309-324, 442) against the texels
+rounded to RGBA8.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+    python bench.py --workload image4096 --levels 1   # one configuration only
+    python bench.py --workload weighted2048           # (profiling runs)
 """
 from __future__ import annotations
 
@@ -59,6 +66,98 @@ def algorithmic_flops(stats):
     return int((FLOPS_FIT4 * fit4 + FLOPS_FIT3 * fit3 + FLOPS_REFINE_STEP * steps).sum())
 
 
+def host_threads():
+    """Host cores this process may run on (the CPU baseline and the oracle use
+    all of them)."""
+    return len(os.sched_getaffinity(0))
+
+
+class Parity:
+    """Seeded-sample checks of GPU results against the oracle (test
+    infrastructure: the oracle is the checker, never the thing measured)."""
+
+    def __init__(self):
+        from oracle import Oracle
+        self.o = Oracle()
+        self.o.init(0)
+        self.threads = host_threads()
+        self.checks = []
+
+    def check(self, what, level, colors, weights, cw, tcm, tcb, codes, errors, sample=8192, seed=0,
+              prior=None):
+        """Compares `codes`/`errors` (host arrays of all blocks) with the oracle
+        on `sample` blocks drawn with `seed` (all of them if fewer)."""
+        n = len(colors)
+        idx = (np.arange(n) if n <= sample
+               else np.sort(np.random.default_rng(seed).choice(n, size=sample, replace=False)))
+        w = None if weights is None else weights[idx]
+        pr = None if prior is None else prior[idx]
+        want_c, want_e, _, _ = self.o.compress_many(level, colors[idx], w, cw, tcm, tcb, nthreads=self.threads,
+                                                    prior=pr)
+        bad_c = int(np.any(codes[idx] != want_c, axis=1).sum())
+        ge, we = errors[idx], want_e
+        bad_e = int((~((ge.view(np.uint32) == we.view(np.uint32)) | (np.isnan(ge) & np.isnan(we)))).sum())
+        r = {"what": what, "level": level, "blocks_checked": int(len(idx)), "code_mismatches": bad_c,
+             "error_mismatches": bad_e}
+        self.checks.append(r)
+        log(f"parity {what} L{level}: {len(idx)} blocks, {bad_c} code / {bad_e} error mismatches")
+        return r
+
+    def summary(self):
+        return {"checker": "oracle (C restatement of the reference's scalar build)",
+                "blocks_checked": sum(c["blocks_checked"] for c in self.checks),
+                "mismatches": sum(c["code_mismatches"] + c["error_mismatches"] for c in self.checks),
+                "checks": self.checks}
+
+
+def check_levels(parity, what, colors, job_levels, counts, codes, errs, tcm, tcb, per_level=2048):
+    """Oracle checks of a batch whose jobs (runs of `counts` blocks) each have
+    their own level: a seeded sample of `per_level` blocks per level."""
+    offs = np.concatenate([[0], np.cumsum(counts)])
+    out = []
+    for lv in sorted(set(job_levels)):
+        idx = np.concatenate([np.arange(o, o + n) for n, o, l in zip(counts, offs[:-1], job_levels) if l == lv])
+        sub = np.sort(idx[np.random.default_rng(200 + lv).choice(len(idx), size=min(len(idx), per_level),
+                                                                 replace=False)])
+        out.append(parity.check(what, lv, colors[sub], None, (1, 1, 1), tcm, tcb, codes[sub], errs[sub],
+                                sample=1 << 30))
+    return out
+
+
+def rgba8_of(colors, chunk=1 << 20):
+    """Block-major float texels rounded to RGBA8 (the harness's 8-bit source
+    image from which it encodes c / 255.0f)."""
+    out = np.empty((len(colors), 64), np.uint8)
+    for a in range(0, len(colors), chunk):
+        c = colors[a:a + chunk].reshape(-1, 64)
+        out[a:a + chunk] = np.clip(np.rint(c * np.float32(255)), 0, 255).astype(np.uint8)
+    return out
+
+
+def quality(g, codes_ptr, colors, stream, rgba=None):
+    """RMSE / PSNR of the encoded blocks, as the reference's harness reports
+    them: mse = sum of evaluate_bc1_error over blocks / (16 * blocks)
+    (icbc_test.cpp:309-317), rmse = sqrt(mse), psnr = 20 log10(255 / rmse)
+    (319-324), the error taken on the device (bc1_evaluate_kernel) against
+    the texels rounded to RGBA8."""
+    n = len(colors)
+    rgba = rgba8_of(colors) if rgba is None else rgba
+    drgba = g.DeviceBuffer.from_array(rgba)
+    derr = g.DeviceBuffer(4 * n)
+    try:
+        g.evaluate_bc1_error_batch(drgba.ptr, codes_ptr, n, derr.ptr, 0, stream)
+        stream.synchronize()
+        errs = derr.download(np.float32, n)
+    finally:
+        drgba.close()
+        derr.close()
+    mse = float(np.sum(errs, dtype=np.float64) / (16 * n))
+    rms = math.sqrt(mse)
+    psnr = min(max(20 * math.log10(255.0 / rms), 0.0), 300.0) if rms else 1e10
+    return {"rmse": round(rms, 4), "psnr_db": round(psnr, 3), "blocks": n,
+            "against": "texels rounded to RGBA8 (icbc_test.cpp:309-324)"}
+
+
 TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                                "bc1_encode_kernel_texture_batch_L8_traffic.md")
 
@@ -89,10 +188,11 @@ def cpu_features():
     return set()
 
 
-def cpu_baseline(blocks, level, tcm, tcb, threads, budget_cpu_s=20.0):
-    """The reference encoder's SIMD build (oracle/_ref) run multithreaded on
-    a prefix of the workload sized to about `budget_cpu_s` of CPU time.  Falls
-    back to the oracle restatement when the reference build is absent."""
+def cpu_baseline(blocks, level, tcm, tcb, threads, budget_cpu_s=20.0, seed=12):
+    """The reference encoder's SIMD build (oracle/_ref) run multithreaded with
+    the harness's chunk-of-32 scheduling on a seeded random sample of the
+    workload, sized to about `budget_cpu_s` of CPU time.  Falls back to the
+    oracle restatement when the reference build is absent."""
     from oracle import Oracle, Reference, reference_available
     feats = cpu_features()
     variant = None
@@ -116,19 +216,22 @@ def cpu_baseline(blocks, level, tcm, tcb, threads, budget_cpu_s=20.0):
 
         def run(b):
             return enc.compress_many(level, b, None, (1, 1, 1), tcm, tcb, nthreads=threads)[3]
-    probe = blocks[: min(len(blocks), 32768)]
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(len(blocks))
+    probe = blocks[np.sort(order[: min(len(blocks), 32768)])]
     t = run(probe)
     per_block_cpu = t * threads / len(probe)
     n = int(min(len(blocks), max(len(probe), budget_cpu_s / max(per_block_cpu, 1e-9))))
-    sample = blocks[:n]
+    sample = blocks[np.sort(order[:n])]
     secs = run(sample)
     return {"value": round(n / secs / 1e6, 4), "unit": "Mblocks/s", "cores": threads, "kind": kind,
-            "sample": f"first {n} blocks of the workload, level {level}, {label}, "
-                      f"{threads} threads, chunks of 32 blocks; {secs:.2f} s wall"}
+            "sample": f"{n} blocks drawn at random (seed {seed}) from the {len(blocks)}-block workload, "
+                      f"level {level}, {label}, {threads} threads (every core this process may use), "
+                      f"chunks of 32 blocks; {secs:.2f} s wall"}
 
 
 def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64 << 30,
-                  max_inflight=2):
+                  max_inflight=2, parity=None):
     """Poisson request stream against the request server: textures with sides
     log-uniform in [16, 2048], level uniform in 1..9, three-colour mode on for
     half of them, block mix 40/30/20/10, stored as RGBA8 (read by the encoder
@@ -136,7 +239,8 @@ def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64
     texels cross PCIe inside its latency.  First the whole
     pool is submitted at once to find the saturation rate; then each QPS
     fraction runs for `seconds`.  Every served result is compared with the
-    same texture encoded alone."""
+    same texture encoded alone, and every pool texture encoded alone is
+    checked block for block against the oracle (with `parity`)."""
     # (saturation is measured on a warm server: see the untimed passes below)
     import queue
     import threading
@@ -151,7 +255,14 @@ def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64
         holder = g.pinned_copy(blocks8)
         pool.append({"blocks": holder.array, "holder": holder, "n": len(blocks),
                      "level": int(rng.integers(1, 10)), "tcm": bool(rng.integers(0, 2))})
-    alone = [g.compress_many(p["level"], p["blocks"], None, (1, 1, 1), p["tcm"], p["tcm"])[0] for p in pool]
+    alone_runs = [g.compress_many(p["level"], p["blocks"], None, (1, 1, 1), p["tcm"], p["tcm"]) for p in pool]
+    alone = [r[0] for r in alone_runs]
+    if parity is not None:
+        # the oracle reads RGBA8 texels as c / 255.0f, as the encoder does
+        for k, (p, r) in enumerate(zip(pool, alone_runs)):
+            f = np.asarray(p["blocks"], np.float32) / np.float32(255)
+            parity.check(f"serving pool texture {k}", p["level"], f, None, (1, 1, 1), p["tcm"], p["tcm"],
+                         r[0], r[1], sample=1 << 30)
     srv = g.Server(budget, max_inflight=max_inflight)
 
     def submit(k, codes):
@@ -242,7 +353,7 @@ def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64
             "input": "RGBA8 texels (64 B/block) in page-locked host memory, copied per request"}
 
 
-def single_image_levels(g, synth, levels=(1, 5, 9), steps=10, seed=7):
+def single_image_levels(g, synth, levels=(1, 5, 9), steps=10, seed=7, parity=None):
     """The 4096x4096 single-image configuration: one plan per level over its
     1,048,576 blocks (40/30/20/10 gradient / white noise / near-black / flat,
     three-colour mode and transparent black on), timed with HIP events; level
@@ -254,6 +365,7 @@ def single_image_levels(g, synth, levels=(1, 5, 9), steps=10, seed=7):
     dcode, derr, dstat = g.DeviceBuffer(8 * n), g.DeviceBuffer(4 * n), g.DeviceBuffer(16 * n)
     s = g.Stream()
     e0, e1 = g.Event(), g.Event()
+    rgba = rgba8_of(colors)
     out = {}
     try:
         for lv in levels:
@@ -269,10 +381,15 @@ def single_image_levels(g, synth, levels=(1, 5, 9), steps=10, seed=7):
             ms = e0.elapsed_ms(e1) / steps
             plan.close()
             r = {"ms": round(ms, 4), "mblocks_per_s": round(n / ms / 1e3, 1)}
+            codes, errs = dcode.download(np.uint8, (n, 8)), derr.download(np.float32, n)
+            if parity is not None:
+                r["parity"] = parity.check("4096x4096 image", lv, colors, None, (1, 1, 1), 1, 1, codes, errs,
+                                           seed=100 + lv)
+            r["quality"] = quality(g, dcode.ptr, colors, s, rgba)
             if lv <= 1:
                 gbps = n * (256 + 8 + 4) / (ms * 1e-3) / 1e9
                 r.update({"bound": "hbm", "achieved_gb_per_s": round(gbps, 1), "peak_gb_per_s": PEAK_HBM_GBPS,
-                          "frac": round(gbps / PEAK_HBM_GBPS, 4)})
+                          "frac": round(gbps / PEAK_HBM_GBPS, 4), "algorithmic_bytes_per_block": 268})
             else:
                 g.compress_batch([g.make_job(lv, n, dcol, dcode, None, derr, dstat, (1, 1, 1), True, True)],
                                  stream=s)
@@ -289,7 +406,8 @@ def single_image_levels(g, synth, levels=(1, 5, 9), steps=10, seed=7):
             "path": "plan (graph-captured launch) per level", "levels": out}
 
 
-def mixed_level_batch(g, jobs_for, counts, texs, stream, steps=3, seed=11):
+def mixed_level_batch(g, jobs_for, counts, texs, stream, steps=3, seed=11, parity=None, colors=None,
+                      codes_of=None):
     """The batching configuration with its per-texture quality drawn from 1-9:
     the same textures and mips, an autotuned persistent graph launch against
     one launch per image."""
@@ -319,6 +437,10 @@ def mixed_level_batch(g, jobs_for, counts, texs, stream, steps=3, seed=11):
 
     ms_b = timed(lambda: plan.execute(stream))
     plan.close()
+    checks = None
+    if parity is not None and codes_of is not None:
+        codes, errs = codes_of()
+        checks = check_levels(parity, "mixed-level batch", colors, levels, counts, codes, errs, True, True)
     ms_p = timed(lambda: g.compress_batch(jobs, stream=stream, plain=True))
     total = int(np.sum(counts))
     blocks_per_level = {str(lv): int(sum(int(n) for n, l in zip(counts, levels) if l == lv)) for lv in range(1, 10)}
@@ -327,25 +449,28 @@ def mixed_level_batch(g, jobs_for, counts, texs, stream, steps=3, seed=11):
            "batched_ms": round(ms_b, 3), "batched_mblocks_per_s": round(total / ms_b / 1e3, 2),
            "plain_ms": round(ms_p, 3), "plain_mblocks_per_s": round(total / ms_p / 1e3, 2),
            "batched_over_plain": round(ms_p / ms_b, 3),
-           "tuning": {str(k): v for k, v in tuned.as_dict().items()}}
+           "tuning": {str(k): v for k, v in tuned.as_dict().items()},
+           "parity": checks}
     log(f"mixed levels: {out}")
     return out
 
 
-def weighted_level9(g, synth, steps=5, seed=4):
+def weighted_level9(g, synth, steps=5, seed=4, parity=None):
     """The weighted configuration: a 2048x2048 RGBA float image whose alpha (30%
     of texels at 0) is each texel's weight, level 9, channel weights (0.3,
     0.59, 0.11), three-colour mode and transparent black on; blocks gathered
     on the device from the image (ICBC_JOB_IMAGE + weights from alpha)."""
     img = synth.make_image(seed, 2048, 2048, synth.MIX_DEFAULT)
     img[..., 3] = synth.make_weighted_alpha(seed, 2048, 2048)
+    blocks = synth.image_to_blocks(img)
     di = g.DeviceImage(img)
     n = di.block_count
-    dcode, derr = g.DeviceBuffer(8 * n), g.DeviceBuffer(4 * n)
+    cw = (0.3, 0.59, 0.11)
+    dcode, derr, dstat = g.DeviceBuffer(8 * n), g.DeviceBuffer(4 * n), g.DeviceBuffer(16 * n)
     s = g.Stream()
     e0, e1 = g.Event(), g.Event()
     try:
-        job = g.make_job(9, n, None, dcode, None, derr, None, (0.3, 0.59, 0.11), True, True, image=di.desc,
+        job = g.make_job(9, n, None, dcode, None, derr, None, cw, True, True, image=di.desc,
                          weights_from_alpha=True)
         plan = g.Plan([job], stream=s)
         plan.execute(s)
@@ -356,12 +481,30 @@ def weighted_level9(g, synth, steps=5, seed=4):
         e1.synchronize()
         ms = e0.elapsed_ms(e1) / steps
         plan.close()
+        codes, errs = dcode.download(np.uint8, (n, 8)), derr.download(np.float32, n)
+        check = None
+        if parity is not None:
+            check = parity.check("2048x2048 weighted image", 9, blocks, np.ascontiguousarray(blocks[:, :, 3]),
+                                 cw, 1, 1, codes, errs, seed=300)
+        q = quality(g, dcode.ptr, blocks, s)
+        g.compress_batch([g.make_job(9, n, None, dcode, None, derr, dstat, cw, True, True, image=di.desc,
+                                     weights_from_alpha=True)], stream=s)
+        s.synchronize()
+        stats = dstat.download(np.int32, (n, 4))
     finally:
-        for b in (dcode, derr, di):
+        for b in (dcode, derr, dstat, di):
             b.close()
+    flops = algorithmic_flops(stats)
+    tf = flops / (ms * 1e-3) / 1e12
     out = {"workload": "2048x2048 RGBA, alpha = weight (30% zero), level 9, cw (0.3, 0.59, 0.11), "
                        "gathered on the device", "blocks": n, "ms": round(ms, 4),
-           "mblocks_per_s": round(n / ms / 1e3, 2)}
+           "mblocks_per_s": round(n / ms / 1e3, 2),
+           "roofline": {"bound": "valu-fp32", "achieved": round(tf, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(tf / PEAK_FP32_TFLOPS, 4),
+                        "algorithmic_gflop_per_launch": round(flops / 1e9, 3),
+                        "refine_steps_per_block": round(float(stats[:, 2].mean()), 2),
+                        "kernels": "image_gather_kernel + bc1_encode_kernel + bc1_refine_kernel (one plan launch)"},
+           "parity": check, "quality": q}
     log(f"weighted level 9: {out}")
     return out
 
@@ -392,6 +535,12 @@ def main():
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM-counter calibration "
                          "under rocprofv3 --pmc)")
+    ap.add_argument("--workload", choices=("batch", "image4096", "weighted2048"), default="batch",
+                    help="batch: the full benchmark (default); image4096 / weighted2048: only that "
+                         "configuration (for per-configuration profiles)")
+    ap.add_argument("--levels", default="1,5,9", help="levels of the 4096x4096 image run")
+    ap.add_argument("--no-parity", action="store_true",
+                    help="skip the oracle checks of sampled outputs (for profiling runs)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -405,17 +554,55 @@ def main():
     g.init(g.Decoder.D3D10)
     dev = g.device_name()
     log(f"device: {dev}")
+    parity = None if args.no_parity else Parity()
+
+    def finish(line):
+        """Prints the JSON line; a parity mismatch fails the run."""
+        if parity is not None:
+            line["parity"] = parity.summary()
+        print(json.dumps(line), flush=True)
+        if parity is not None and line["parity"]["mismatches"]:
+            log(f"PARITY FAILURE: {line['parity']['mismatches']} mismatches against the oracle")
+            sys.exit(1)
+
+    if args.copy_calibration_mb:
+        nb = args.copy_calibration_mb << 20
+        src, dst = g.DeviceBuffer(nb), g.DeviceBuffer(nb)
+        src.fill(1)
+        g.check(g.lib().icbc_gpu_memcpy_dtod(dst.ptr, src.ptr, nb, None))
+        g.check(g.lib().icbc_gpu_stream_synchronize(None))
+        src.close()
+        dst.close()
+        log(f"calibration copy: {nb} bytes device to device")
 
     if args.serve:
-        res = serving_bench(g, synth, 60.0, (0.5, 0.8, 0.95))
+        res = serving_bench(g, synth, 60.0, (0.5, 0.8, 0.95), parity=parity)
         top = res["runs"][-1]
-        print(json.dumps({"metric": "BC1 serving: Mblocks/s and p50/p99 request latency under Poisson load",
-                          "value": top["mblocks_per_s"], "unit": "Mblocks/s", "n_gpus": 1,
-                          "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-                          "p99_ms": top["latency_ms_p99"], "p50_ms": top["latency_ms_p50"],
-                          "config": {"workload": "Poisson requests, sides log-uniform 16-2048, levels 1-9, "
-                                                 "three_color_mode 50/50, 64 GB budget", "device": dev},
-                          "serving": res}), flush=True)
+        finish({"metric": "BC1 serving: Mblocks/s and p50/p99 request latency under Poisson load",
+                "value": top["mblocks_per_s"], "unit": "Mblocks/s", "n_gpus": 1,
+                "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+                "p99_ms": top["latency_ms_p99"], "p50_ms": top["latency_ms_p50"],
+                "config": {"workload": "Poisson requests, sides log-uniform 16-2048, levels 1-9, "
+                                       "three_color_mode 50/50, 64 GB budget", "device": dev},
+                "serving": res})
+        return
+
+    if args.workload != "batch":
+        # one configuration alone (the per-configuration profiles run these)
+        if args.workload == "image4096":
+            lv = tuple(int(x) for x in args.levels.split(","))
+            res = single_image_levels(g, synth, levels=lv, steps=args.steps, parity=parity)
+            first = res["levels"][str(lv[0])]
+            line = {"metric": f"BC1 Mblocks/s, one 4096x4096 image, level {lv[0]}",
+                    "value": first["mblocks_per_s"], "ms_per_step": first["ms"]}
+        else:
+            res = weighted_level9(g, synth, steps=args.steps, parity=parity)
+            line = {"metric": "BC1 Mblocks/s, 2048x2048 weighted image, level 9",
+                    "value": res["mblocks_per_s"], "ms_per_step": res["ms"], "roofline": res["roofline"]}
+        line.update({"unit": "Mblocks/s", "n_gpus": 1, "steps": args.steps, "higher_is_better": True,
+                     "dtype": "f32", "data": "synthetic", "config": {"workload": args.workload, "device": dev},
+                     "result": res})
+        finish(line)
         return
 
     t0 = time.time()
@@ -444,15 +631,6 @@ def main():
                 for lv, n, o in zip(job_levels, counts, offsets[:-1])]
 
     stream = g.Stream()
-    if args.copy_calibration_mb:
-        nb = args.copy_calibration_mb << 20
-        src, dst = g.DeviceBuffer(nb), g.DeviceBuffer(nb)
-        src.fill(1)
-        g.check(g.lib().icbc_gpu_memcpy_dtod(dst.ptr, src.ptr, nb, stream.handle))
-        stream.synchronize()
-        src.close()
-        dst.close()
-        log(f"calibration copy: {nb} bytes device to device")
     tuning = None
     if not args.no_autotune:
         # untimed, like building the plan: tune the workload's levels on its
@@ -490,6 +668,10 @@ def main():
     ms_batched = float(np.mean(per_batched))
     codes_batched = dcode.download(np.uint8, (total, 8))
     log(f"batched: {ms_batched:.3f} ms/step ({total / ms_batched / 1e3:.2f} Mblocks/s)")
+    if parity is not None:
+        check_levels(parity, "texture batch (batched plan)", colors, job_levels, counts, codes_batched,
+                     derr.download(np.float32, total), tcm, tcb, per_level=8192)
+    batch_quality = None if args.mixed_levels else quality(g, dcode.ptr, colors, stream)
 
     result_plain = None
     if not args.no_plain:
@@ -523,7 +705,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         sample_lv = args.level if not args.mixed_levels else 8
         cpu = cpu_baseline(colors, sample_lv, tcm, tcb, threads)
         log(f"cpu baseline: {cpu}")
@@ -531,19 +713,22 @@ def main():
     levels_run = None
     weighted = None
     if not args.no_levels:
-        levels_run = single_image_levels(g, synth)
-        weighted = weighted_level9(g, synth)
+        levels_run = single_image_levels(g, synth, levels=tuple(int(x) for x in args.levels.split(",")),
+                                         parity=parity)
+        weighted = weighted_level9(g, synth, parity=parity)
     mixed = None
     if not args.no_mixed and not args.mixed_levels:
         def jobs_for(levels):
             return [g.make_job(lv, int(n), dcol.ptr + 256 * int(o), dcode.ptr + 8 * int(o), None,
                                derr.ptr + 4 * int(o), None, (1, 1, 1), tcm, tcb)
                     for lv, n, o in zip(levels, counts, offsets[:-1])]
-        mixed = mixed_level_batch(g, jobs_for, counts, texs, stream)
+        mixed = mixed_level_batch(g, jobs_for, counts, texs, stream, parity=parity, colors=colors,
+                                  codes_of=lambda: (dcode.download(np.uint8, (total, 8)),
+                                                    derr.download(np.float32, total)))
 
     serving = None
     if not args.no_serve:
-        serving = serving_bench(g, synth, args.serve_seconds, (0.8,))
+        serving = serving_bench(g, synth, args.serve_seconds, (0.8,), parity=parity)
 
     value = total / ms_batched / 1e3
     out = {
@@ -562,16 +747,17 @@ def main():
         "plain": result_plain,
         "batched_over_plain": (round(result_plain["ms_per_step"] / ms_batched, 4) if result_plain else None),
         "roofline": roofline,
+        "quality": batch_quality,
         "cpu_baseline": cpu,
         "serving": serving,
         "single_image_levels": levels_run,
         "weighted_level9": weighted,
         "mixed_level_batch": mixed,
     }
-    print(json.dumps(out), flush=True)
     plan.close()
     for b in (dcol, dcode, derr, dstat):
         b.close()
+    finish(out)
 
 
 if __name__ == "__main__":

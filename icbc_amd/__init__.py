@@ -138,6 +138,9 @@ _SIGNATURES = {
     "icbc_gpu_decode_bc1_batch": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, C.c_void_p]),
     "icbc_gpu_evaluate_bc1_error_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
                                                     C.c_int, C.c_void_p]),
+    "icbc_gpu_decode_bc3_batch": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, C.c_void_p]),
+    "icbc_gpu_evaluate_bc3_error_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                                    C.c_int, C.c_int, C.c_void_p]),
     "icbc_gpu_last_error": (C.c_char_p, []),
     "icbc_gpu_device_name": (C.c_int, [C.c_char_p, C.c_size_t]),
     "icbc_gpu_version": (C.c_int, []),
@@ -635,12 +638,42 @@ def evaluate_bc1_error_batch(rgba_dev, codes_dev, block_count, errors_dev, decod
                                                   stream.handle if stream else None))
 
 
+def decode_bc3_batch(blocks_dev, block_count, rgba_dev, decoder=Decoder.D3D10, stream=None):
+    """icbc::decode_bc3 (icbc.h:2929-2946) over 16-byte BC3 blocks in device memory."""
+    check(lib().icbc_gpu_decode_bc3_batch(int(blocks_dev), int(block_count), int(rgba_dev), int(decoder),
+                                          stream.handle if stream else None))
+
+
+def evaluate_bc3_error_batch(rgba_dev, blocks_dev, block_count, errors_dev, alpha_blend,
+                             decoder=Decoder.D3D10, stream=None):
+    """icbc::evaluate_bc3_error (icbc.h:3048-3071) over 16-byte BC3 blocks in device memory."""
+    check(lib().icbc_gpu_evaluate_bc3_error_batch(int(rgba_dev), int(blocks_dev), int(block_count),
+                                                  int(errors_dev), int(bool(alpha_blend)), int(decoder),
+                                                  stream.handle if stream else None))
+
+
+def _one_block(block, size):
+    b = np.frombuffer(bytes(block), np.uint8)
+    if len(b) < size:
+        raise ValueError(f"a block is {size} bytes")
+    return np.ascontiguousarray(b[:size])
+
+
 def decode_bc1(block, decoder=Decoder.D3D10):
     """icbc::decode_bc1 (icbc.h:38, 3820) for one block -> 64 RGBA8 bytes."""
-    b = np.ascontiguousarray(np.frombuffer(bytes(block), np.uint8)[:8])
     d = DeviceBuffer(128)
-    d.upload(b)
+    d.upload(_one_block(block, 8))
     decode_bc1_batch(d.ptr, 1, d.ptr + 64, decoder)
+    out = d.download(np.uint8, 64, offset=64)
+    d.close()
+    return out
+
+
+def decode_bc3(block, decoder=Decoder.D3D10):
+    """icbc::decode_bc3 (icbc.h:39, 3824) for one 16-byte block -> 64 RGBA8 bytes."""
+    d = DeviceBuffer(128)
+    d.upload(_one_block(block, 16))
+    decode_bc3_batch(d.ptr, 1, d.ptr + 64, decoder)
     out = d.download(np.uint8, 64, offset=64)
     d.close()
     return out
@@ -650,9 +683,20 @@ def evaluate_bc1_error(rgba_block, block, decoder=Decoder.D3D10):
     """icbc::evaluate_bc1_error (icbc.h:43, 3837) for one block."""
     d = DeviceBuffer(128)
     d.upload(np.ascontiguousarray(rgba_block, np.uint8).reshape(64))
-    d.upload(np.ascontiguousarray(np.frombuffer(bytes(block), np.uint8)[:8]), offset=64)
+    d.upload(_one_block(block, 8), offset=64)
     evaluate_bc1_error_batch(d.ptr, d.ptr + 64, 1, d.ptr + 80, decoder)
     e = float(d.download(np.float32, 1, offset=80)[0])
+    d.close()
+    return e
+
+
+def evaluate_bc3_error(rgba_block, block, alpha_blend, decoder=Decoder.D3D10):
+    """icbc::evaluate_bc3_error (icbc.h:44, 3841) for one 16-byte block."""
+    d = DeviceBuffer(128)
+    d.upload(np.ascontiguousarray(rgba_block, np.uint8).reshape(64))
+    d.upload(_one_block(block, 16), offset=64)
+    evaluate_bc3_error_batch(d.ptr, d.ptr + 64, 1, d.ptr + 96, alpha_blend, decoder)
+    e = float(d.download(np.float32, 1, offset=96)[0])
     d.close()
     return e
 

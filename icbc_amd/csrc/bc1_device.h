@@ -122,11 +122,13 @@ struct PalInts {
     int r0, g0, b0, r1, g1, b1, r2, g2, b2, r3, g3, b3;
 };
 
-__device__ __forceinline__ PalInts palette_ints(uint32_t c0, uint32_t c1, int decoder) {
+// With `force_four` the four-colour formulas apply whatever the endpoint order
+// (evaluate_palette4, icbc.h:2883-2891, which BC3's colour block uses).
+__device__ __forceinline__ PalInts palette_ints(uint32_t c0, uint32_t c1, int decoder, bool force_four = false) {
     PalInts p;
     expand565(c0, p.r0, p.g0, p.b0);
     expand565(c1, p.r1, p.g1, p.b1);
-    const bool four = c0 > c1;
+    const bool four = force_four || c0 > c1;
     p.r3 = p.g3 = p.b3 = 0;
     if (decoder == 1) {
         // NVIDIA: red and blue interpolate the 5-bit values, green the 8-bit ones.
@@ -158,8 +160,8 @@ __device__ __forceinline__ PalInts palette_ints(uint32_t c0, uint32_t c1, int de
 }
 
 // Array form for the decode / evaluate kernels, which index entries per texel.
-__device__ __forceinline__ void palette8(uint32_t c0, uint32_t c1, int decoder, Pal8& p) {
-    const PalInts q = palette_ints(c0, c1, decoder);
+__device__ __forceinline__ void palette8(uint32_t c0, uint32_t c1, int decoder, Pal8& p, bool force_four = false) {
+    const PalInts q = palette_ints(c0, c1, decoder, force_four);
     p.r[0] = q.r0; p.g[0] = q.g0; p.b[0] = q.b0;
     p.r[1] = q.r1; p.g[1] = q.g1; p.b[1] = q.b1;
     p.r[2] = q.r2; p.g[2] = q.g2; p.b[2] = q.b2;

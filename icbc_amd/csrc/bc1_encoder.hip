@@ -377,14 +377,18 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
             __syncthreads();
             // the survivors get the exact evaluation; every candidate that could
             // reach the minimum error is among them (the list is not in candidate
-            // order: equal errors keep the lower index)
+            // order: equal errors keep the lower index).  A tie only counts once
+            // the lane holds a winner, so, as in the reference (icbc.h:2725), an
+            // error must be below FLT_MAX to win.  (Blocks that reach this pass
+            // have a finite margin below 1e30, which bounds every candidate's
+            // error far below FLT_MAX; the guard keeps the rule explicit.)
             for (int j = lane; j < count; j += kTile) {
                 const int i = sh.survivors[j];
                 const uint32_t pk = sh.four[i];
                 f3 a, e;
                 const float err = candidate4(sat[four_c0(pk)], sat[four_c1(pk)], sat[four_c2(pk)], tot,
                                              metric, dw, a, e);
-                if ((err < best) | ((err == best) & (i < besti))) {
+                if ((err < best) | ((err == best) & (i < besti) & (besti != kNoWinner))) {
                     best = err;
                     besti = i;
                 }
@@ -1034,19 +1038,33 @@ __global__ void bc1_check_reciprocal_kernel(unsigned long long* __restrict__ mis
     if (bad) atomicAdd(mismatches, bad);
 }
 
-// decode_bc1 (icbc.h:2915-2927): one thread per block, RGBA8 out.
+// decode_bc1 (icbc.h:2915-2927), or with kBc3 decode_bc3 (icbc.h:2929-2946):
+// one thread per block, RGBA8 out.  A BC3 block is 16 bytes, the alpha block
+// {alpha0, alpha1, 6 index bytes} and then a BC1 block (icbc.h:260-269); the
+// reference decodes its colours with the four-colour palette whatever the
+// endpoint order, and gives every texel alpha0 (its alpha indices are
+// unimplemented: index 0 for every texel).
+template <bool kBc3>
 __global__ void bc1_decode_kernel(const uint2* __restrict__ codes, uint64_t n,
                                   uint32_t* __restrict__ rgba, int decoder) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint2 c = codes[i];
+    uint2 c;
+    uint32_t alpha0 = 255u;
+    if constexpr (kBc3) {
+        const uint4 b = reinterpret_cast<const uint4*>(codes)[i];
+        alpha0 = b.x & 0xffu;
+        c = make_uint2(b.z, b.w);
+    } else {
+        c = codes[i];
+    }
     Pal8 p;
-    palette8(c.x & 0xffff, c.x >> 16, decoder, p);
-    const bool four = (c.x & 0xffff) > (c.x >> 16);
+    palette8(c.x & 0xffff, c.x >> 16, decoder, p, kBc3);
+    const bool four = kBc3 || (c.x & 0xffff) > (c.x >> 16);
     uint32_t px[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        const uint32_t a = (k < 3 || four) ? 255u : 0u;
+        const uint32_t a = kBc3 ? alpha0 : ((k < 3 || four) ? 255u : 0u);
         px[k] = (uint32_t)p.r[k] | ((uint32_t)p.g[k] << 8) | ((uint32_t)p.b[k] << 16) | (a << 24);
     }
     uint4* dst = reinterpret_cast<uint4*>(rgba + i * 16);
@@ -1063,14 +1081,26 @@ __global__ void bc1_decode_kernel(const uint2* __restrict__ codes, uint64_t n,
 }
 
 // evaluate_bc1_error (icbc.h:3030-3046): integer squared error per texel,
-// summed in float in texel order.
+// summed in float in texel order.  With kBc3, evaluate_bc3_error
+// (icbc.h:3048-3071): per texel the colour error against the four-colour
+// palette, scaled by alpha / 255.0f when `alpha_blend`, then the squared
+// difference of alpha0 and the texel's alpha, each added in turn.
+template <bool kBc3>
 __global__ void bc1_evaluate_kernel(const uint32_t* __restrict__ rgba, const uint2* __restrict__ codes,
-                                    uint64_t n, float* __restrict__ errors, int decoder) {
+                                    uint64_t n, float* __restrict__ errors, int decoder, int alpha_blend) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint2 c = codes[i];
+    uint2 c;
+    int alpha0 = 0;
+    if constexpr (kBc3) {
+        const uint4 b = reinterpret_cast<const uint4*>(codes)[i];
+        alpha0 = (int)(b.x & 0xffu);
+        c = make_uint2(b.z, b.w);
+    } else {
+        c = codes[i];
+    }
     Pal8 p;
-    palette8(c.x & 0xffff, c.x >> 16, decoder, p);
+    palette8(c.x & 0xffff, c.x >> 16, decoder, p, kBc3);
     const uint4* src = reinterpret_cast<const uint4*>(rgba + i * 16);
     float e = 0.0f;
 #pragma unroll
@@ -1086,7 +1116,15 @@ __global__ void bc1_evaluate_kernel(const uint32_t* __restrict__ rgba, const uin
             const int dr = pr - (int)(v[k] & 0xff);
             const int dg = pg - (int)((v[k] >> 8) & 0xff);
             const int db = pb - (int)((v[k] >> 16) & 0xff);
-            e += (float)(dr * dr + dg * dg + db * db);
+            const float ce = (float)(dr * dr + dg * dg + db * db);
+            if constexpr (kBc3) {
+                const uint32_t a = v[k] >> 24;
+                e += (alpha_blend ? u8_to_unit(a) : 1.0f) * ce;
+                const int da = alpha0 - (int)a;
+                e += (float)(da * da);
+            } else {
+                e += ce;
+            }
         }
     }
     errors[i] = e;
@@ -1191,21 +1229,27 @@ hipError_t launch_check_reciprocal(unsigned long long* mismatches, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_decode(const uint2* codes, uint64_t n, uint32_t* rgba, int decoder,
+hipError_t launch_decode(const uint2* codes, uint64_t n, uint32_t* rgba, int decoder, bool bc3,
                          hipStream_t stream) {
     if (n == 0) return hipSuccess;
     const uint32_t threads = 256;
-    hipLaunchKernelGGL(bc1_decode_kernel, dim3((uint32_t)((n + threads - 1) / threads)),
-                       dim3(threads), 0, stream, codes, n, rgba, decoder);
+    const dim3 grid((uint32_t)((n + threads - 1) / threads));
+    if (bc3) hipLaunchKernelGGL(bc1_decode_kernel<true>, grid, dim3(threads), 0, stream, codes, n, rgba, decoder);
+    else hipLaunchKernelGGL(bc1_decode_kernel<false>, grid, dim3(threads), 0, stream, codes, n, rgba, decoder);
     return hipGetLastError();
 }
 
 hipError_t launch_evaluate(const uint32_t* rgba, const uint2* codes, uint64_t n, float* errors,
-                           int decoder, hipStream_t stream) {
+                           int decoder, bool bc3, bool alpha_blend, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     const uint32_t threads = 256;
-    hipLaunchKernelGGL(bc1_evaluate_kernel, dim3((uint32_t)((n + threads - 1) / threads)),
-                       dim3(threads), 0, stream, rgba, codes, n, errors, decoder);
+    const dim3 grid((uint32_t)((n + threads - 1) / threads));
+    if (bc3)
+        hipLaunchKernelGGL(bc1_evaluate_kernel<true>, grid, dim3(threads), 0, stream, rgba, codes, n, errors, decoder,
+                           alpha_blend ? 1 : 0);
+    else
+        hipLaunchKernelGGL(bc1_evaluate_kernel<false>, grid, dim3(threads), 0, stream, rgba, codes, n, errors,
+                           decoder, 0);
     return hipGetLastError();
 }
 

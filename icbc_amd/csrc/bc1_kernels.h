@@ -89,7 +89,7 @@ struct RefineList {
     uint32_t* counters;  // [0] appended, [kQueueStride / 4] claimed
 };
 constexpr size_t kRefineCounterBytes = 2 * kQueueStride;
-inline size_t refine_list_bytes(uint64_t level9_blocks) {
+constexpr size_t refine_list_bytes(uint64_t level9_blocks) {
     return kRefineCounterBytes + (size_t)level9_blocks * sizeof(RefineItem);
 }
 
@@ -149,9 +149,11 @@ constexpr uint64_t kGridScheduleMaxWaves = 4;
 // encoder's fast reciprocal differs from IEEE-754 division on this device.
 hipError_t launch_check_reciprocal(unsigned long long* mismatches, hipStream_t stream);
 
-hipError_t launch_decode(const uint2* codes, uint64_t n, uint32_t* rgba, int decoder,
+// decode_bc1 / evaluate_bc1_error over n blocks (8 bytes each), or with
+// `bc3` decode_bc3 / evaluate_bc3_error over n BC3 blocks (16 bytes each).
+hipError_t launch_decode(const uint2* codes, uint64_t n, uint32_t* rgba, int decoder, bool bc3,
                          hipStream_t stream);
 hipError_t launch_evaluate(const uint32_t* rgba, const uint2* codes, uint64_t n, float* errors,
-                           int decoder, hipStream_t stream);
+                           int decoder, bool bc3, bool alpha_blend, hipStream_t stream);
 
 }  // namespace icbc_gpu

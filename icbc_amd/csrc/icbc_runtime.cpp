@@ -25,6 +25,7 @@
 #include "../../include/icbc.h"
 #include "../../include/icbc_gpu.h"
 #include "bc1_kernels.h"
+#include "block_context.h"
 #include "icbc_internal.h"
 #include "texture_prepass.h"
 
@@ -966,30 +967,41 @@ uint64_t icbc_gpu_plan_tiles(const icbc_bc1_plan* pl) { return pl ? pl->ntiles :
 
 namespace {
 
-// Per-thread resources for single-block calls (the reference's compress_bc1
-// is reentrant; each calling thread gets its own stream and buffers).
-struct BlockContext {
-    int device = -1;
-    hipStream_t stream = nullptr;
-    void* dbuf = nullptr;    // colors 256 | weights 64 | code 8 | error 4
-    void* hbuf = nullptr;    // pinned mirror
-    ~BlockContext() {
-        // process teardown may already have destroyed the runtime; leak instead
-    }
-};
-thread_local BlockContext t_block;
+// Device buffer of a single-block context: colors 256 | weights 64 | code 8 |
+// error 4 (at 336) | job 128 (at 384) | workspace and level-9 handoff (at 768);
+// decode / evaluate use its first 128 bytes.  Host mirror: the first 512 bytes.
+constexpr size_t kBlockDeviceBytes = 768 + kWorkspaceBytesPerWorkgroup + 4096;
+constexpr size_t kBlockHostBytes = 512;
+static_assert(refine_list_bytes(1) <= 4096, "level-9 handoff of one block must fit");
 
-int block_context(BlockContext*& ctx) {
-    ctx = &t_block;
+struct HipBlockApi {
+    int stream_create(void** s) {
+        const hipError_t e = hipStreamCreateWithFlags((hipStream_t*)s, hipStreamNonBlocking);
+        return e == hipSuccess ? ICBC_OK : hip_fail(e, "hipStreamCreateWithFlags (single block)");
+    }
+    void stream_destroy(void* s) { (void)hipStreamDestroy((hipStream_t)s); }
+    int device_alloc(void** p) {
+        const hipError_t e = hipMalloc(p, kBlockDeviceBytes);
+        return e == hipSuccess ? ICBC_OK : hip_fail(e, "hipMalloc (single block)");
+    }
+    void device_free(void* p) { (void)hipFree(p); }
+    int host_alloc(void** p) {
+        const hipError_t e = hipHostMalloc(p, kBlockHostBytes, hipHostMallocDefault);
+        return e == hipSuccess ? ICBC_OK : hip_fail(e, "hipHostMalloc (single block)");
+    }
+    void host_free(void* p) { (void)hipHostFree(p); }
+};
+
+// The reference's compress_bc1 is reentrant; each calling thread gets its own
+// stream and buffers per device (never released: process teardown may already
+// have destroyed the runtime when thread-local destructors run).
+thread_local std::vector<BlockResources> t_block;
+
+int block_context(BlockResources*& ctx) {
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
-    if (ctx->device != dev) {
-        ctx->device = dev;
-        HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-        HIP_TRY(hipMalloc(&ctx->dbuf, 768 + kWorkspaceBytesPerWorkgroup + refine_list_bytes(1)));
-        HIP_TRY(hipHostMalloc(&ctx->hbuf, 512, hipHostMallocDefault));
-    }
-    return ICBC_OK;
+    HipBlockApi api;
+    return block_resources_for(api, t_block, dev, ctx);
 }
 
 // One block through the same kernel as the batch path; status code out.
@@ -1001,9 +1013,10 @@ int compress_one(int level, const float* input_colors, const float* input_weight
     int decoder;
     int r = ensure_device_ready(&decoder);
     if (r) return r;
-    BlockContext* ctx;
+    BlockResources* ctx;
     r = block_context(ctx);
     if (r) return r;
+    const hipStream_t stream = (hipStream_t)ctx->stream;
     char* h = (char*)ctx->hbuf;
     char* d = (char*)ctx->dbuf;
     memcpy(h, input_colors, 256);
@@ -1025,14 +1038,61 @@ int compress_one(int level, const float* input_colors, const float* input_weight
     JobDev jd = to_dev(j, decoder);
     static_assert(sizeof(JobDev) <= 128, "job descriptor must fit the block buffer");
     memcpy(h + 384, &jd, sizeof jd);
-    HIP_TRY(hipMemcpyAsync(d, h, 512, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d, h, 512, hipMemcpyHostToDevice, stream));
     HIP_TRY(launch_encode((const JobDev*)(d + 384), nullptr, 1, nullptr, (float4*)(d + 768), 1, encode_mode_of_level(level),
-                          refine_list_at(d + 768 + kWorkspaceBytesPerWorkgroup),
-                          ctx->stream));
-    HIP_TRY(hipMemcpyAsync(h + 320, d + 320, 32, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+                          refine_list_at(d + 768 + kWorkspaceBytesPerWorkgroup), stream));
+    HIP_TRY(hipMemcpyAsync(h + 320, d + 320, 32, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
     memcpy(output, h + 320, 8);
     memcpy(error, h + 336, 4);
+    return ICBC_OK;
+}
+
+// decode of one BC1 (8 B) or BC3 (16 B) block into 64 bytes of RGBA8, on the
+// calling thread's single-block buffers.
+int decode_one(const void* block, unsigned char* rgba, int decoder, bool bc3) {
+    if (!block || !rgba) return fail(ICBC_ERR_INVALID_ARGUMENT, "null argument");
+    if (decoder < 0 || decoder > 3) return fail(ICBC_ERR_INVALID_ARGUMENT, "decoder %d", decoder);
+    int d0;
+    int r = ensure_device_ready(&d0);
+    if (r) return r;
+    BlockResources* ctx;
+    r = block_context(ctx);
+    if (r) return r;
+    const hipStream_t stream = (hipStream_t)ctx->stream;
+    char* h = (char*)ctx->hbuf;
+    char* d = (char*)ctx->dbuf;
+    memcpy(h, block, bc3 ? 16 : 8);
+    HIP_TRY(hipMemcpyAsync(d, h, 16, hipMemcpyHostToDevice, stream));
+    HIP_TRY(launch_decode((const uint2*)d, 1, (uint32_t*)(d + 64), decoder, bc3, stream));
+    HIP_TRY(hipMemcpyAsync(h + 64, d + 64, 64, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    memcpy(rgba, h + 64, 64);
+    return ICBC_OK;
+}
+
+// evaluate_bc1_error / evaluate_bc3_error of one block.
+int evaluate_one(const unsigned char* rgba, const void* block, int decoder, bool bc3, bool alpha_blend,
+                 float* error) {
+    if (!block || !rgba) return fail(ICBC_ERR_INVALID_ARGUMENT, "null argument");
+    if (decoder < 0 || decoder > 3) return fail(ICBC_ERR_INVALID_ARGUMENT, "decoder %d", decoder);
+    int d0;
+    int r = ensure_device_ready(&d0);
+    if (r) return r;
+    BlockResources* ctx;
+    r = block_context(ctx);
+    if (r) return r;
+    const hipStream_t stream = (hipStream_t)ctx->stream;
+    char* h = (char*)ctx->hbuf;
+    char* d = (char*)ctx->dbuf;
+    memcpy(h, rgba, 64);
+    memcpy(h + 64, block, bc3 ? 16 : 8);
+    HIP_TRY(hipMemcpyAsync(d, h, 80, hipMemcpyHostToDevice, stream));
+    HIP_TRY(launch_evaluate((const uint32_t*)d, (const uint2*)(d + 64), 1, (float*)(d + 80), decoder, bc3,
+                            alpha_blend, stream));
+    HIP_TRY(hipMemcpyAsync(h + 80, d + 80, 4, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    memcpy(error, h + 80, 4);
     return ICBC_OK;
 }
 
@@ -1121,31 +1181,54 @@ int icbc_gpu_build_mip_chain(const icbc_image* source, float* dst, uint64_t dst_
     return ICBC_OK;
 }
 
-int icbc_gpu_decode_bc1_batch(const void* codes, uint64_t block_count, uint8_t* rgba, int decoder,
-                              void* stream) {
+namespace {
+
+int decode_batch(const void* codes, uint64_t block_count, uint8_t* rgba, int decoder, bool bc3, void* stream) {
     if (decoder < 0 || decoder > 3) return fail(ICBC_ERR_INVALID_ARGUMENT, "decoder %d", decoder);
     if (block_count && (!codes || !rgba)) return fail(ICBC_ERR_INVALID_ARGUMENT, "null buffer");
-    if (((uintptr_t)codes & 7) || ((uintptr_t)rgba & 15))
-        return fail(ICBC_ERR_INVALID_ARGUMENT, "codes must be 8-byte and rgba 16-byte aligned");
+    if (((uintptr_t)codes & (bc3 ? 15 : 7)) || ((uintptr_t)rgba & 15))
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "codes must be %d-byte and rgba 16-byte aligned", bc3 ? 16 : 8);
     int d;
     int r = ensure_device_ready(&d);
     if (r) return r;
-    HIP_TRY(launch_decode((const uint2*)codes, block_count, (uint32_t*)rgba, decoder, (hipStream_t)stream));
+    HIP_TRY(launch_decode((const uint2*)codes, block_count, (uint32_t*)rgba, decoder, bc3, (hipStream_t)stream));
     return ICBC_OK;
 }
 
-int icbc_gpu_evaluate_bc1_error_batch(const uint8_t* rgba, const void* codes, uint64_t block_count,
-                                      float* errors, int decoder, void* stream) {
+int evaluate_batch(const uint8_t* rgba, const void* codes, uint64_t block_count, float* errors, int decoder,
+                   bool bc3, bool alpha_blend, void* stream) {
     if (decoder < 0 || decoder > 3) return fail(ICBC_ERR_INVALID_ARGUMENT, "decoder %d", decoder);
     if (block_count && (!codes || !rgba || !errors)) return fail(ICBC_ERR_INVALID_ARGUMENT, "null buffer");
-    if (((uintptr_t)codes & 7) || ((uintptr_t)rgba & 15) || ((uintptr_t)errors & 3))
+    if (((uintptr_t)codes & (bc3 ? 15 : 7)) || ((uintptr_t)rgba & 15) || ((uintptr_t)errors & 3))
         return fail(ICBC_ERR_INVALID_ARGUMENT, "misaligned buffer");
     int d;
     int r = ensure_device_ready(&d);
     if (r) return r;
-    HIP_TRY(launch_evaluate((const uint32_t*)rgba, (const uint2*)codes, block_count, errors, decoder,
-                            (hipStream_t)stream));
+    HIP_TRY(launch_evaluate((const uint32_t*)rgba, (const uint2*)codes, block_count, errors, decoder, bc3,
+                            alpha_blend, (hipStream_t)stream));
     return ICBC_OK;
+}
+
+}  // namespace
+
+int icbc_gpu_decode_bc1_batch(const void* codes, uint64_t block_count, uint8_t* rgba, int decoder,
+                              void* stream) {
+    return decode_batch(codes, block_count, rgba, decoder, false, stream);
+}
+
+int icbc_gpu_evaluate_bc1_error_batch(const uint8_t* rgba, const void* codes, uint64_t block_count,
+                                      float* errors, int decoder, void* stream) {
+    return evaluate_batch(rgba, codes, block_count, errors, decoder, false, false, stream);
+}
+
+int icbc_gpu_decode_bc3_batch(const void* blocks, uint64_t block_count, uint8_t* rgba, int decoder,
+                              void* stream) {
+    return decode_batch(blocks, block_count, rgba, decoder, true, stream);
+}
+
+int icbc_gpu_evaluate_bc3_error_batch(const uint8_t* rgba, const void* blocks, uint64_t block_count,
+                                      float* errors, int alpha_blend, int decoder, void* stream) {
+    return evaluate_batch(rgba, blocks, block_count, errors, decoder, true, alpha_blend != 0, stream);
 }
 
 // ---- diagnostics / plumbing ----------------------------------------------------
@@ -1295,13 +1378,13 @@ float icbc_gpu_event_elapsed_ms(void* start, void* stop) {
 
 namespace icbc {
 
-static void die(const char* what) {
-    fprintf(stderr, "icbc (MI355X): %s failed: %s\n", what, icbc_gpu_last_error());
-    abort();
-}
+// The reference's API has no error channel.  On a device failure (or a call
+// before a successful init) the functions return a defined value -- NaN for
+// the float results, outputs untouched -- and icbc_gpu_last_error() holds the
+// reason.
 
 void init(Decoder decoder) {
-    if (icbc_gpu_init((int)decoder) != ICBC_OK) die("init");
+    (void)icbc_gpu_init((int)decoder);  // a failure shows in the calls that follow
 }
 
 float compress_bc1(Quality level, const float* input_colors, const float* input_weights,
@@ -1310,7 +1393,7 @@ float compress_bc1(Quality level, const float* input_colors, const float* input_
     float e = 0.0f;
     if (compress_one((int)level, input_colors, input_weights, color_weights, three_color_mode ? 1 : 0,
                      three_color_black ? 1 : 0, output, &e) != ICBC_OK)
-        die("compress_bc1");
+        return NAN;
     return e;
 }
 
@@ -1319,24 +1402,23 @@ float compress_bc3(Quality, const float*, const float*, const float[3], bool, vo
 }
 
 void decode_bc1(const void* block, unsigned char rgba_block[16 * 4], Decoder decoder) {
-    void* d = icbc_gpu_malloc(128);
-    if (!d) die("decode_bc1");
-    if (icbc_gpu_memcpy_htod(d, block, 8) || icbc_gpu_decode_bc1_batch(d, 1, (uint8_t*)d + 64, (int)decoder, nullptr) ||
-        icbc_gpu_memcpy_dtoh(rgba_block, (uint8_t*)d + 64, 64))
-        die("decode_bc1");
-    icbc_gpu_free(d);
+    (void)decode_one(block, rgba_block, (int)decoder, false);
+}
+
+void decode_bc3(const void* block, unsigned char rgba_block[16 * 4], Decoder decoder) {
+    (void)decode_one(block, rgba_block, (int)decoder, true);
 }
 
 float evaluate_bc1_error(const unsigned char rgba_block[16 * 4], const void* block, Decoder decoder) {
-    void* d = icbc_gpu_malloc(128);
-    if (!d) die("evaluate_bc1_error");
     float e = 0.0f;
-    if (icbc_gpu_memcpy_htod(d, rgba_block, 64) || icbc_gpu_memcpy_htod((uint8_t*)d + 64, block, 8) ||
-        icbc_gpu_evaluate_bc1_error_batch((const uint8_t*)d, (uint8_t*)d + 64, 1, (float*)((uint8_t*)d + 80),
-                                          (int)decoder, nullptr) ||
-        icbc_gpu_memcpy_dtoh(&e, (uint8_t*)d + 80, 4))
-        die("evaluate_bc1_error");
-    icbc_gpu_free(d);
+    if (evaluate_one(rgba_block, block, (int)decoder, false, false, &e) != ICBC_OK) return NAN;
+    return e;
+}
+
+float evaluate_bc3_error(const unsigned char rgba_block[16 * 4], const void* block, bool alpha_blend,
+                         Decoder decoder) {
+    float e = 0.0f;
+    if (evaluate_one(rgba_block, block, (int)decoder, true, alpha_blend, &e) != ICBC_OK) return NAN;
     return e;
 }
 

@@ -2,12 +2,16 @@
 // the MI355X (gfx950) BC1 encoder in libicbc_amd.so.
 //
 // Same namespace, enums, defaults and signatures as the reference header's
-// public section (icbc.h:10-52), so code written against the reference
-// compiles and links unchanged against libicbc_amd.so.  Unlike the reference
-// this header has no ICBC_IMPLEMENTATION section: the implementation is the
-// shared library.  Every call runs on the GPU; a device failure aborts with a
-// message (the reference API has no error channel).  For throughput, encode
-// many blocks per call through icbc_gpu.h instead of block by block.
+// public section (icbc.h:10-52): every function the reference declares there
+// (init, decode_bc1, decode_bc3, evaluate_bc1_error, evaluate_bc3_error,
+// compress_bc1, compress_bc3) is here, so code that calls them compiles and
+// links against libicbc_amd.so without change.  The reference's internal
+// functions (everything inside its ICBC_IMPLEMENTATION section) are not part
+// of this API.  Every call runs on the GPU.  The reference API has no error
+// channel: after a device failure, or before a successful init, the float
+// functions return NaN, outputs are left untouched, and icbc_gpu_last_error()
+// (icbc_gpu.h) gives the reason.  For throughput, encode many blocks per call
+// through icbc_gpu.h instead of block by block.
 #ifndef ICBC_H
 #define ICBC_H
 
@@ -44,8 +48,17 @@ namespace icbc {
     // Reference: icbc.h:38, 3820.
     void decode_bc1(const void * block, unsigned char rgba_block[16 * 4], Decoder decoder = Decoder_D3D10);
 
+    // Reference: icbc.h:39, 3824 (2929-2946).  16-byte BC3 block: colours
+    // from the four-colour palette of its BC1 part, every texel's alpha =
+    // alpha0 (the reference's alpha indices are unimplemented).
+    void decode_bc3(const void * block, unsigned char rgba_block[16 * 4], Decoder decoder = Decoder_D3D10);
+
     // Reference: icbc.h:43, 3837.
     float evaluate_bc1_error(const unsigned char rgba_block[16 * 4], const void * block, Decoder decoder = Decoder_D3D10);
+
+    // Reference: icbc.h:44, 3841 (3048-3071), with the same decoding as
+    // decode_bc3; the colour error is scaled by alpha / 255 when alpha_blend.
+    float evaluate_bc3_error(const unsigned char rgba_block[16 * 4], const void * block, bool alpha_blend, Decoder decoder = Decoder_D3D10);
 
     // Reference: icbc.h:48, 3846.  Returns the weighted squared error of the
     // encoded block; FLT_MAX with `output` untouched for an invalid level.

@@ -179,6 +179,16 @@ int icbc_gpu_decode_bc1_batch(const void* codes, uint64_t block_count, uint8_t* 
 int icbc_gpu_evaluate_bc1_error_batch(const uint8_t* rgba, const void* codes,
                                       uint64_t block_count, float* errors, int decoder,
                                       void* stream);
+/* decode_bc3 (icbc.h:2929-2946, 3824) and evaluate_bc3_error (icbc.h:3048-3071,
+ * 3841) over 16-byte BC3 blocks (16-byte aligned, device memory): the colour
+ * part decoded with the four-colour palette whatever its endpoint order, every
+ * texel's alpha = alpha0 (the reference leaves the alpha indices unimplemented);
+ * the error adds, per texel in order, the colour error (times alpha / 255 when
+ * alpha_blend) and (alpha0 - alpha)^2. */
+int icbc_gpu_decode_bc3_batch(const void* blocks, uint64_t block_count, uint8_t* rgba, int decoder,
+                              void* stream);
+int icbc_gpu_evaluate_bc3_error_batch(const uint8_t* rgba, const void* blocks, uint64_t block_count,
+                                      float* errors, int alpha_blend, int decoder, void* stream);
 
 /* ---- autotuning -----------------------------------------------------------
  * Per quality level: blocks per wavefront tile, resident workgroups per CU and

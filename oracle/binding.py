@@ -63,6 +63,11 @@ class _Encoder:
         self._eval = getattr(L, p + "evaluate_bc1_error")
         self._eval.restype = C.c_float
         self._eval.argtypes = [_u8, C.c_void_p, C.c_int]
+        self._decode3 = getattr(L, p + "decode_bc3")
+        self._decode3.argtypes = [C.c_void_p, _u8, C.c_int]
+        self._eval3 = getattr(L, p + "evaluate_bc3_error")
+        self._eval3.restype = C.c_float
+        self._eval3.argtypes = [_u8, C.c_void_p, C.c_int, C.c_int]
         for name, res in [("reduce_colors", C.c_int), ("compute_sat", C.c_int),
                           ("principal_component", None), ("cluster_fit_four", None),
                           ("cluster_fit_three", None), ("output_block4", C.c_float),
@@ -102,6 +107,17 @@ class _Encoder:
         r = np.ascontiguousarray(rgba64, np.uint8)
         b = np.ascontiguousarray(block8, np.uint8)
         return float(self._eval(_u8p(r), b.ctypes.data, decoder))
+
+    def decode_bc3(self, block16, decoder=0):
+        rgba = np.zeros(64, np.uint8)
+        b = np.ascontiguousarray(block16, np.uint8)
+        self._decode3(b.ctypes.data, _u8p(rgba), decoder)
+        return rgba
+
+    def evaluate_bc3_error(self, rgba64, block16, alpha_blend, decoder=0):
+        r = np.ascontiguousarray(rgba64, np.uint8)
+        b = np.ascontiguousarray(block16, np.uint8)
+        return float(self._eval3(_u8p(r), b.ctypes.data, int(bool(alpha_blend)), decoder))
 
     # ---- stages ------------------------------------------------------------
     def reduce_colors(self, colors64, weights16, threshold):

@@ -191,13 +191,15 @@ static inline uint8_t to_u8(float v) { return (uint8_t)(sat01(v) * 255 + 0.5f); 
 /* ------------------------------------------------------------------ */
 /* palettes (icbc.h:2751-2900)                                         */
 
-static void palette_rgba(uint16_t c0, uint16_t c1, int decoder, rgba_i pal[4]) {
+/* With force_four the four-colour formulas apply whatever the endpoint order
+   (evaluate_palette4, icbc.h:2883-2891). */
+static void palette_rgba_forced(uint16_t c0, uint16_t c1, int decoder, int force_four, rgba_i pal[4]) {
     pal[0] = expand565(c0);
     pal[1] = expand565(c1);
     int p0[3] = {pal[0].r, pal[0].g, pal[0].b};
     int p1[3] = {pal[1].r, pal[1].g, pal[1].b};
     int q2[3], q3[3];
-    int four = c0 > c1;
+    int four = force_four || c0 > c1;
     if (decoder == 1) { /* NVIDIA: red/blue from the 5-bit values, green from 8-bit */
         int r0 = (c0 >> 11) & 31, b0 = c0 & 31, r1 = (c1 >> 11) & 31, b1 = c1 & 31;
         int gd = p1[1] - p0[1];
@@ -239,6 +241,10 @@ static void palette_rgba(uint16_t c0, uint16_t c1, int decoder, rgba_i pal[4]) {
     } else {
         pal[3].r = pal[3].g = pal[3].b = pal[3].a = 0;
     }
+}
+
+static void palette_rgba(uint16_t c0, uint16_t c1, int decoder, rgba_i pal[4]) {
+    palette_rgba_forced(c0, c1, decoder, 0, pal);
 }
 
 /* float palette = colour / 255.0f per channel (icbc.h:1656, 2902-2909) */
@@ -876,6 +882,49 @@ float oracle_evaluate_bc1_error(const uint8_t* rgba64, const void* block8, int d
         rgba_i p = pal[(b.idx >> (2 * i)) & 3];
         int dr = p.r - rgba64[4 * i], dg = p.g - rgba64[4 * i + 1], db = p.b - rgba64[4 * i + 2];
         e += (float)(dr * dr + dg * dg + db * db);
+    }
+    return e;
+}
+
+/* decode_bc3 (icbc.h:2929-2946): the colour block (bytes 8-15) with the
+   four-colour palette whatever the endpoint order (evaluate_palette4,
+   icbc.h:2883-2891), alpha = alpha0 for every texel (alpha_palette[0],
+   icbc.h:2902-2911; the reference's alpha index is always 0). */
+static void palette4_rgba(uint16_t c0, uint16_t c1, int decoder, rgba_i pal[4]) {
+    palette_rgba_forced(c0, c1, decoder, 1, pal);
+}
+
+void oracle_decode_bc3(const void* block16, uint8_t* rgba64, int decoder) {
+    const uint8_t* b = (const uint8_t*)block16;
+    blk rgb;
+    memcpy(&rgb, b + 8, 8);
+    rgba_i pal[4];
+    palette4_rgba(rgb.c0, rgb.c1, decoder, pal);
+    for (int i = 0; i < 16; i++) {
+        rgba_i c = pal[(rgb.idx >> (2 * i)) & 3];
+        rgba64[4 * i + 0] = (uint8_t)c.r;
+        rgba64[4 * i + 1] = (uint8_t)c.g;
+        rgba64[4 * i + 2] = (uint8_t)c.b;
+        rgba64[4 * i + 3] = b[0];
+    }
+}
+
+/* evaluate_bc3_error (icbc.h:3048-3071): per texel, in order, the integer
+   colour error against the indexed four-colour entry, times alpha * 1.0f /
+   255.0f when alpha_blend, then the squared alpha0 difference. */
+float oracle_evaluate_bc3_error(const uint8_t* rgba64, const void* block16, int alpha_blend, int decoder) {
+    const uint8_t* b = (const uint8_t*)block16;
+    blk rgb;
+    memcpy(&rgb, b + 8, 8);
+    rgba_i pal[4];
+    palette4_rgba(rgb.c0, rgb.c1, decoder, pal);
+    float e = 0.0f;
+    for (int i = 0; i < 16; i++) {
+        rgba_i p = pal[(rgb.idx >> (2 * i)) & 3];
+        int dr = p.r - rgba64[4 * i], dg = p.g - rgba64[4 * i + 1], db = p.b - rgba64[4 * i + 2];
+        int a = rgba64[4 * i + 3];
+        e += (alpha_blend ? (a * 1.0f / 255.0f) : 1.0f) * (float)(dr * dr + dg * dg + db * db);
+        e += (float)((b[0] - a) * (b[0] - a));
     }
     return e;
 }

@@ -38,6 +38,8 @@ double oracle_compress_bc1_many(int level, const float* colors, const float* wei
 
 void oracle_decode_bc1(const void* block8, uint8_t* rgba64, int decoder);
 float oracle_evaluate_bc1_error(const uint8_t* rgba64, const void* block8, int decoder);
+void oracle_decode_bc3(const void* block16, uint8_t* rgba64, int decoder);
+float oracle_evaluate_bc3_error(const uint8_t* rgba64, const void* block16, int alpha_blend, int decoder);
 
 /* Stage entry points (for per-stage pinning against the reference build). */
 int oracle_reduce_colors(const float* colors64, const float* weights16, float threshold,

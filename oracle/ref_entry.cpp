@@ -101,6 +101,14 @@ float ref_evaluate_bc1_error(const uint8_t* rgba, const void* block, int decoder
     return icbc::evaluate_bc1_error(rgba, block, (Decoder)decoder);
 }
 
+void ref_decode_bc3(const void* block, uint8_t* rgba, int decoder) {
+    icbc::decode_bc3(block, rgba, (Decoder)decoder);
+}
+
+float ref_evaluate_bc3_error(const uint8_t* rgba, const void* block, int alpha_blend, int decoder) {
+    return icbc::evaluate_bc3_error(rgba, block, alpha_blend != 0, (Decoder)decoder);
+}
+
 // ---- internal stages (static functions of the header, reachable from this TU) ----
 
 // icbc.h:1683 reduce_colors over the 16 texels of one block.

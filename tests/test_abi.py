@@ -34,7 +34,9 @@ def test_library_exports_reference_cpp_api():
                 "_ZN4icbc12compress_bc1ENS_7QualityEPKfS2_S2_bbPv",
                 "_ZN4icbc12compress_bc3ENS_7QualityEPKfS2_S2_bPv",
                 "_ZN4icbc10decode_bc1EPKvPhNS_7DecoderE",
-                "_ZN4icbc18evaluate_bc1_errorEPKhPKvNS_7DecoderE"):
+                "_ZN4icbc10decode_bc3EPKvPhNS_7DecoderE",
+                "_ZN4icbc18evaluate_bc1_errorEPKhPKvNS_7DecoderE",
+                "_ZN4icbc18evaluate_bc3_errorEPKhPKvbNS_7DecoderE"):
         assert sym in out, sym
 
 
@@ -45,7 +47,7 @@ def test_no_internal_symbols_leak():
     exported = [l.split()[-1] for l in out.splitlines() if " T " in l or " W " in l]
     for s in exported:
         assert s.startswith(("icbc_gpu_", "icbc_server_", "icbc_container_", "_ZN4icbc")), s
-    assert sum(s.startswith("_ZN4icbc") for s in exported) == 5
+    assert sum(s.startswith("_ZN4icbc") for s in exported) == 7
 
 
 def test_programs_link_against_library(tmp_path):
@@ -64,7 +66,8 @@ def test_programs_link_against_library(tmp_path):
                    'float c[64]={0},w[16]={1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1},cw[3]={1,1,1}; unsigned char o[8];\n'
                    'float e = icbc::compress_bc1(icbc::Quality_Default, c, w, cw, false, false, o);\n'
                    'unsigned char rgba[64]; icbc::decode_bc1(o, rgba);\n'
-                   'return (int)(e + icbc::evaluate_bc1_error(rgba, o));}\n')
+                   'unsigned char b3[16] = {0}; icbc::decode_bc3(b3, rgba, icbc::Decoder_AMD);\n'
+                   'return (int)(e + icbc::evaluate_bc1_error(rgba, o) + icbc::evaluate_bc3_error(rgba, b3, true));}\n')
     subprocess.check_call(["g++", "-std=c++11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(cpp),
                            "-L", libdir, "-licbc_amd", "-o", str(tmp_path / "caller_cpp")])
 
@@ -82,6 +85,8 @@ def test_invalid_arguments_are_rejected_without_device():
     assert "decoder" in icbc_amd.last_error()
     assert lib.icbc_gpu_decode_bc1_batch(None, 0, None, 9, None) == icbc_amd.ERR_INVALID_ARGUMENT
     assert lib.icbc_gpu_evaluate_bc1_error_batch(None, None, 0, None, -1, None) == icbc_amd.ERR_INVALID_ARGUMENT
+    assert lib.icbc_gpu_decode_bc3_batch(None, 0, None, 4, None) == icbc_amd.ERR_INVALID_ARGUMENT
+    assert lib.icbc_gpu_evaluate_bc3_error_batch(None, None, 0, None, 1, -1, None) == icbc_amd.ERR_INVALID_ARGUMENT
     assert lib.icbc_gpu_version() == 10500
 
 

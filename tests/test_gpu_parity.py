@@ -188,6 +188,10 @@ def test_cpp_namespace_api_against_golden(gpu, golden):
     dec.argtypes, dec.restype = [C.c_void_p, C.c_void_p, C.c_int], None
     ev = lib._ZN4icbc18evaluate_bc1_errorEPKhPKvNS_7DecoderE
     ev.argtypes, ev.restype = [C.c_void_p, C.c_void_p, C.c_int], C.c_float
+    dec3 = lib._ZN4icbc10decode_bc3EPKvPhNS_7DecoderE
+    dec3.argtypes, dec3.restype = [C.c_void_p, C.c_void_p, C.c_int], None
+    ev3 = lib._ZN4icbc18evaluate_bc3_errorEPKhPKvbNS_7DecoderE
+    ev3.argtypes, ev3.restype = [C.c_void_p, C.c_void_p, C.c_bool, C.c_int], C.c_float
     colors, weights, prior = golden["colors"], golden["weights"], golden["prior"]
     try:
         for k, (decoder, level, tcm, tcb, cwi) in enumerate(golden["cfg"]):
@@ -213,6 +217,13 @@ def test_cpp_namespace_api_against_golden(gpu, golden):
                 src = np.ascontiguousarray(golden["eval_rgba"][i])
                 e = ev(src.ctypes.data, code.ctypes.data, d)
                 assert np.float32(e) == golden["eval_err"][d][i]
+                b3 = np.ascontiguousarray(golden["bc3_blocks"][i])
+                dec3(b3.ctypes.data, rgba.ctypes.data, d)
+                assert np.array_equal(rgba, golden["bc3_rgba"][d][i])
+                src3 = np.ascontiguousarray(golden["bc3_eval_rgba"][i])
+                for blend in (False, True):
+                    e = ev3(src3.ctypes.data, b3.ctypes.data, blend, d)
+                    assert np.float32(e) == golden["bc3_eval_err"][int(blend)][d][i]
     finally:
         gpu.init(0)
 
@@ -384,6 +395,34 @@ def test_decode_and_evaluate(gpu, golden, oracle):
     # single-block mirrors
     assert np.array_equal(gpu.decode_bc1(codes[3].tobytes(), 2), golden["dec_rgba"][2, 3])
     assert gpu.evaluate_bc1_error(rgba[5], codes[5].tobytes(), 1) == golden["eval_err"][1, 5]
+    for b in (dcodes, drgba, dsrc, derr):
+        b.close()
+
+
+def test_bc3_decode_and_evaluate(gpu, golden):
+    # decode_bc3 / evaluate_bc3_error (icbc.h:2929-2946, 3048-3071) against
+    # the reference-written vectors: 128 random BC3 blocks, every decoder,
+    # with and without alpha blending
+    blocks = np.ascontiguousarray(golden["bc3_blocks"])
+    n = len(blocks)
+    dblk = gpu.DeviceBuffer.from_array(blocks)
+    drgba = gpu.DeviceBuffer(64 * n)
+    dsrc = gpu.DeviceBuffer.from_array(np.ascontiguousarray(golden["bc3_eval_rgba"]))
+    derr = gpu.DeviceBuffer(4 * n)
+    try:
+        for dec in range(4):
+            gpu.decode_bc3_batch(dblk.ptr, n, drgba.ptr, dec)
+            assert np.array_equal(drgba.download(np.uint8, (n, 64)), golden["bc3_rgba"][dec])
+            for blend in range(2):
+                gpu.evaluate_bc3_error_batch(dsrc.ptr, dblk.ptr, n, derr.ptr, blend, dec)
+                assert np.array_equal(derr.download(np.float32, n).view(np.uint32),
+                                      golden["bc3_eval_err"][blend, dec].view(np.uint32)), (dec, blend)
+        assert np.array_equal(gpu.decode_bc3(blocks[7].tobytes(), 3), golden["bc3_rgba"][3, 7])
+        e = gpu.evaluate_bc3_error(golden["bc3_eval_rgba"][9], blocks[9].tobytes(), True, 1)
+        assert np.float32(e) == golden["bc3_eval_err"][1, 1, 9]
+    finally:
+        for b in (dblk, drgba, dsrc, derr):
+            b.close()
 
 
 def test_full_size_image_level8_properties(gpu, oracle):

@@ -63,6 +63,82 @@ def test_decode_and_evaluate_match_golden(oracle, golden):
             assert np.float32(e).view(np.uint32) == golden["eval_err"][dec, i].view(np.uint32)
 
 
+def test_bc3_decode_and_evaluate_match_golden(oracle, golden):
+    for dec in range(4):
+        for i in range(len(golden["bc3_blocks"])):
+            assert np.array_equal(oracle.decode_bc3(golden["bc3_blocks"][i], dec), golden["bc3_rgba"][dec, i])
+            for blend in range(2):
+                e = oracle.evaluate_bc3_error(golden["bc3_eval_rgba"][i], golden["bc3_blocks"][i], blend, dec)
+                assert np.float32(e).view(np.uint32) == golden["bc3_eval_err"][blend, dec, i].view(np.uint32)
+
+
+def test_bc3_palette_is_the_four_colour_palette(oracle):
+    # closed form: a BC3 colour block with col0 < col1 decodes like the BC1
+    # block with the endpoints swapped (four-colour, col0 > col1) and entries
+    # 0<->1, 2<->3 exchanged (indices ^ 0x55555555); col0 > col1 decodes like
+    # BC1 itself; alpha is alpha0 everywhere
+    rng = np.random.default_rng(31)
+    for dec in range(4):
+        for _ in range(300):
+            c0, c1 = (int(v) for v in rng.integers(0, 65536, size=2))
+            if c0 == c1:
+                continue
+            idx = int(rng.integers(0, 2**32, dtype=np.uint64))
+            a0 = int(rng.integers(0, 256))
+            b3 = np.zeros(16, np.uint8)
+            b3[0], b3[1] = a0, int(rng.integers(0, 256))
+            b3[8:16] = np.frombuffer(np.array([c0, c1], np.uint16).tobytes() + np.uint32(idx).tobytes(), np.uint8)
+            if c0 > c1:
+                bc1 = b3[8:16]
+            else:
+                bc1 = np.frombuffer(np.array([c1, c0], np.uint16).tobytes()
+                                    + np.uint32(idx ^ 0x55555555).tobytes(), np.uint8)
+            want = oracle.decode(bc1, dec).reshape(16, 4).copy()
+            want[:, 3] = a0
+            assert np.array_equal(oracle.decode_bc3(b3, dec).reshape(16, 4), want), (dec, c0, c1)
+
+
+def test_golden_inputs_regenerate():
+    # the fixture's recipe writes the committed inputs again (its generator is
+    # frozen in the script)
+    import importlib.util
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gen_golden.py")
+    spec = importlib.util.spec_from_file_location("gen_golden", path)
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    with np.load(path.replace("gen_golden.py", "bc1_golden.npz"), allow_pickle=False) as z:
+        colors, weights = gen.inputs()
+        assert np.array_equal(colors.view(np.uint32), z["colors"].view(np.uint32))
+        assert np.array_equal(weights.view(np.uint32), z["weights"].view(np.uint32))
+        blocks, rgba = gen.bc3_inputs(106, 128)
+        assert np.array_equal(blocks, z["bc3_blocks"]) and np.array_equal(rgba, z["bc3_eval_rgba"])
+
+
+def test_golden_outputs_equal_reference_build(reference, golden):
+    # every output in the fixture is what the reference build gives on the
+    # fixture's inputs
+    colors, weights, prior = golden["colors"], golden["weights"], golden["prior"]
+    try:
+        for k, (dec, level, tcm, tcb, cwi) in enumerate(golden["cfg"]):
+            reference.init(int(dec))
+            codes, errs, _ = reference.compress_many(int(level), colors, weights, golden["cw"][cwi], int(tcm),
+                                                     int(tcb), prior=prior)
+            assert np.array_equal(codes, golden["codes"][k]), k
+            assert np.array_equal(errs.view(np.uint32), golden["errors"][k].view(np.uint32)), k
+        for dec in range(4):
+            for i in range(len(golden["dec_codes"])):
+                assert np.array_equal(reference.decode(golden["dec_codes"][i], dec), golden["dec_rgba"][dec, i])
+                e = reference.evaluate_error(golden["eval_rgba"][i], golden["dec_codes"][i], dec)
+                assert np.float32(e).view(np.uint32) == golden["eval_err"][dec, i].view(np.uint32)
+                assert np.array_equal(reference.decode_bc3(golden["bc3_blocks"][i], dec), golden["bc3_rgba"][dec, i])
+                for blend in range(2):
+                    e = reference.evaluate_bc3_error(golden["bc3_eval_rgba"][i], golden["bc3_blocks"][i], blend, dec)
+                    assert np.float32(e).view(np.uint32) == golden["bc3_eval_err"][blend, dec, i].view(np.uint32)
+    finally:
+        reference.init(0)
+
+
 def test_invalid_level_leaves_output(oracle, golden):
     colors, weights, prior = golden["colors"][:32], golden["weights"][:32], golden["prior"][:32]
     for level in (-1, 10, 100):
