@@ -128,6 +128,8 @@ _SIGNATURES = {
     "icbc_gpu_init": (C.c_int, [C.c_int]),
     "icbc_gpu_compress_bc1_batch": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p]),
     "icbc_gpu_compress_bc1_plain": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p]),
+    "icbc_gpu_compress_bc1_batch_noalloc": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p]),
+    "icbc_gpu_noalloc_scratch_bytes": (C.c_uint64, []),
     "icbc_gpu_compress_bc1": (C.c_float, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                           C.c_int, C.c_void_p]),
     "icbc_gpu_plan_create": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,

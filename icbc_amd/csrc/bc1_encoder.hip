@@ -1223,6 +1223,67 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
     return hipGetLastError();
 }
 
+hipError_t warm_launch_caches() {
+    (void)box_grid();
+    (void)refine_grid();
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation entry point.  Host code never uses the array's
+// name as an address (on the host it is only a stand-in): the device address
+// is resolved per device with hipGetSymbolAddress at set-up and kept below.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices];  // per device ordinal; written at set-up under the runtime's lock
+}  // namespace
+
+hipError_t noalloc_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+void* noalloc_scratch(int device) {
+    return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
+
+// Thread g copies job g into the table and writes tile g of the list: tile g
+// belongs to the job at launch position p with first_tile[p] <= g <
+// first_tile[p + 1] and covers blocks from (g - first_tile[p]) * its tile size.
+__global__ void noalloc_setup_kernel(const NoAllocJobs a, JobDev* __restrict__ jobs, uint2* __restrict__ tiles,
+                                     uint32_t write_tiles) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < a.n) jobs[g] = a.jobs[g];
+    const uint32_t ntiles = a.first_tile[a.n];
+    if (!write_tiles || g >= ntiles) return;
+    uint32_t lo = 0, hi = a.n;  // first_tile[lo] <= g < first_tile[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (a.first_tile[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t k = a.order[lo];
+    tiles[g] = make_uint2(k, (g - a.first_tile[lo]) * (uint32_t)a.jobs[k].tile_blocks);
+}
+
+hipError_t launch_noalloc_setup(const NoAllocJobs& a, void* scratch_ptr, hipStream_t stream) {
+    char* base = static_cast<char*>(scratch_ptr);
+    const uint32_t ntiles = a.first_tile[a.n];
+    const uint32_t work = std::max(a.n, a.n > 1 ? ntiles : 0u);
+    if (work == 0) return hipSuccess;
+    const uint32_t threads = 256;
+    hipLaunchKernelGGL(noalloc_setup_kernel, dim3((work + threads - 1) / threads), dim3(threads), 0, stream, a,
+                       reinterpret_cast<JobDev*>(base + kNoAllocJobsOffset),
+                       reinterpret_cast<uint2*>(base + kNoAllocTilesOffset), a.n > 1 ? 1u : 0u);
+    return hipGetLastError();
+}
+
 hipError_t launch_check_reciprocal(unsigned long long* mismatches, hipStream_t stream) {
     hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, (uint32_t*)mismatches, 2u, 1u);
     hipLaunchKernelGGL(bc1_check_reciprocal_kernel, dim3(4096), dim3(256), 0, stream, mismatches);

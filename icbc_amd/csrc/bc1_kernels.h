@@ -147,6 +147,47 @@ constexpr uint64_t kGridScheduleMaxWaves = 4;
 
 // Counts (into *mismatches, zeroed first) the 2^32 float inputs where the
 // encoder's fast reciprocal differs from IEEE-754 division on this device.
+// ---------------------------------------------------------------------------
+// The no-allocation batch entry point (icbc_gpu_compress_bc1_batch_noalloc):
+// everything a launch needs besides the caller's inputs and outputs lives in
+// one file-scope device array of bc1_encoder.hip, laid out as
+//   [queue counters][job table][tile list][level-9 handoff][workgroup workspaces]
+// with these maxima per call.
+constexpr uint32_t kNoAllocMaxJobs = 32;
+constexpr uint32_t kNoAllocMaxTiles = 65536;
+constexpr uint64_t kNoAllocMaxLevel9Blocks = 262144;
+constexpr uint32_t kNoAllocMaxGrid = 1024;  // resident workgroups of its launches
+constexpr size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kNoAllocJobsOffset = round256(kQueueBytes);
+constexpr size_t kNoAllocTilesOffset = kNoAllocJobsOffset + round256(kNoAllocMaxJobs * sizeof(JobDev));
+constexpr size_t kNoAllocRefineOffset = kNoAllocTilesOffset + round256(kNoAllocMaxTiles * sizeof(uint2));
+constexpr size_t kNoAllocWorkspaceOffset = kNoAllocRefineOffset + round256(refine_list_bytes(kNoAllocMaxLevel9Blocks));
+constexpr size_t kNoAllocScratchBytes = kNoAllocWorkspaceOffset + (size_t)kNoAllocMaxGrid * kWorkspaceBytesPerWorkgroup;
+
+// Resolves the scratch array's address on the current device (`device`) with
+// hipGetSymbolAddress; called once per device at its set-up.
+hipError_t noalloc_scratch_setup(int device);
+// The address resolved for `device`, or null before its set-up.
+void* noalloc_scratch(int device);
+
+// A no-allocation launch's jobs, passed to the device as a kernel argument:
+// the job table, the launch order of the jobs and the first tile of each in
+// that order (first_tile[n] = the tile count).
+struct NoAllocJobs {
+    JobDev jobs[kNoAllocMaxJobs];
+    uint32_t order[kNoAllocMaxJobs];
+    uint32_t first_tile[kNoAllocMaxJobs + 1];
+    uint32_t n;
+};
+// Writes the job table and, for more than one job, the tile list into the
+// scratch at `scratch`.
+hipError_t launch_noalloc_setup(const NoAllocJobs& a, void* scratch, hipStream_t stream);
+
+// Fills the per-device caches launch_encode reads (resident workgroups of the
+// level 0-1 and refinement kernels), so later launches on the device take no
+// first-use path.
+hipError_t warm_launch_caches();
+
 hipError_t launch_check_reciprocal(unsigned long long* mismatches, hipStream_t stream);
 
 // decode_bc1 / evaluate_bc1_error over n blocks (8 bytes each), or with

@@ -36,12 +36,11 @@ namespace {
 // ---------------------------------------------------------------------------
 // error reporting
 
-thread_local std::string t_last_error;
+// A fixed buffer: recording an error allocates nothing.
+thread_local char t_last_error[512];
 
 int vfail(int code, const char* fmt, va_list ap) {
-    char buf[512];
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    t_last_error = buf;
+    vsnprintf(t_last_error, sizeof t_last_error, fmt, ap);
     return code;
 }
 
@@ -187,6 +186,8 @@ hipMemPool_t pool_for(int dev) {
     return nullptr;
 }
 
+int resident_workgroups(uint32_t* grid);
+
 int ensure_device_ready(int* decoder) {
     std::lock_guard<std::mutex> lock(g_mutex);
     if (g_decoder < 0) return fail(ICBC_ERR_NOT_INITIALIZED, "icbc_gpu_init has not been called");
@@ -200,6 +201,15 @@ int ensure_device_ready(int* decoder) {
                         prop.gcnArchName);
         hipError_t e = upload_tables(g_tables, nullptr);
         if (e != hipSuccess) return hip_fail(e, "upload_tables");
+        // the no-allocation entry point's scratch address on this device, and
+        // the launch caches it relies on, are settled here, once per device
+        e = noalloc_scratch_setup(dev);
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress (no-allocation scratch)");
+        e = warm_launch_caches();
+        if (e != hipSuccess) return hip_fail(e, "launch caches");
+        uint32_t grid;
+        const int r = resident_workgroups(&grid);  // fills this device's residency cache
+        if (r) return r;
         if (!pool_for(dev)) {
             hipMemPoolProps props;
             memset(&props, 0, sizeof props);
@@ -217,6 +227,22 @@ int ensure_device_ready(int* decoder) {
         g_uploaded.push_back(dev);
     }
     *decoder = g_decoder;
+    return ICBC_OK;
+}
+
+// Like ensure_device_ready, but for a device that must already be set up (by
+// icbc_gpu_init with it current, or by an earlier call on it): never takes
+// the set-up path, which allocates.
+int device_ready_no_setup(int* decoder, int* device) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (g_decoder < 0) return fail(ICBC_ERR_NOT_INITIALIZED, "icbc_gpu_init has not been called");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (std::find(g_uploaded.begin(), g_uploaded.end(), dev) == g_uploaded.end())
+        return fail(ICBC_ERR_NOT_INITIALIZED, "device %d is not set up: call icbc_gpu_init (or any other encode "
+                    "entry point) with it current first", dev);
+    *decoder = g_decoder;
+    *device = dev;
     return ICBC_OK;
 }
 
@@ -791,6 +817,107 @@ int icbc_gpu_compress_bc1_plain(const icbc_bc1_job* jobs, size_t n_jobs, void* s
     return finish(jobs, n_jobs, s, p);
 }
 
+// The batch call without any allocation (no device, pinned or host heap
+// memory): device-memory block jobs only, within the ICBC_NOALLOC_* limits.
+// Job table, tile list, queue counters, level-9 handoff and workspaces live in
+// the library's per-device scratch array (bc1_encoder.hip); the job table
+// reaches it as a kernel argument.  Calls that overlap on one device share
+// that array and race; the caller serialises them.
+static_assert(kNoAllocMaxJobs == ICBC_NOALLOC_MAX_JOBS && kNoAllocMaxTiles == ICBC_NOALLOC_MAX_TILES &&
+                  kNoAllocMaxLevel9Blocks == ICBC_NOALLOC_MAX_LEVEL9_BLOCKS && kNoAllocMaxGrid == ICBC_NOALLOC_MAX_GRID,
+              "no-allocation limits must match icbc_gpu.h");
+
+int icbc_gpu_compress_bc1_batch_noalloc(const icbc_bc1_job* jobs, size_t n_jobs, void* stream) {
+    if (n_jobs > kNoAllocMaxJobs)
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "%zu jobs: the no-allocation call takes at most %u", n_jobs,
+                    kNoAllocMaxJobs);
+    if (n_jobs && !jobs) return fail(ICBC_ERR_INVALID_ARGUMENT, "null jobs");
+    int decoder, dev;
+    int r = device_ready_no_setup(&decoder, &dev);
+    if (r) return r;
+    void* base = noalloc_scratch(dev);
+    if (!base) return fail(ICBC_ERR_NOT_INITIALIZED, "no scratch address for device %d", dev);
+    NoAllocJobs a;
+    memset(&a, 0, sizeof a);
+    a.n = (uint32_t)n_jobs;
+    uint64_t total = 0, level9 = 0;
+    for (size_t k = 0; k < n_jobs; k++) {
+        const icbc_bc1_job& j = jobs[k];
+        r = validate(j, k);
+        if (r) return r;
+        if (j.flags & (ICBC_JOB_HOST_MEMORY | ICBC_JOB_IMAGE))
+            return fail(ICBC_ERR_INVALID_ARGUMENT, "job %zu: the no-allocation call takes device-memory block "
+                        "jobs only (no host memory, no image source)", k);
+        a.jobs[k] = to_dev(j, decoder);
+        total += j.block_count;
+        if (j.level == 9) level9 += j.block_count;
+    }
+    if (level9 > kNoAllocMaxLevel9Blocks)
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "%llu level-9 blocks: the no-allocation call takes at most %llu",
+                    (unsigned long long)level9, (unsigned long long)kNoAllocMaxLevel9Blocks);
+    uint32_t resident;
+    r = resident_workgroups(&resident);
+    if (r) return r;
+    const uint32_t grid_max = std::min<uint32_t>(resident, kNoAllocMaxGrid);
+    // tile sizes as in the batch call (tuned for large launches), for this
+    // call's residency
+    icbc_tuning tune;
+    const bool tuned = tuning_for_current(&tune);
+    for (size_t k = 0; k < n_jobs; k++)
+        if (jobs[k].tile_blocks == 0) {
+            uint32_t t = auto_tile_blocks_impl(a.jobs[k].block_count, total, grid_max);
+            const int lv = a.jobs[k].level;
+            if (tuned && lv >= 0 && lv <= 9 && tune.tile_blocks[lv] && total >= 64ull * grid_max) {
+                t = tune.tile_blocks[lv];
+                while (t > 8 && t / 2 >= a.jobs[k].block_count) t /= 2;
+            }
+            a.jobs[k].tile_blocks = (int32_t)t;
+        }
+    // launch order: dearest tiles first (as the batch call), ties in submission order
+    float cost[10];
+    for (int l = 0; l < 10; l++) cost[l] = static_level_cost(l);
+    bool all_tuned = tuned;
+    for (size_t k = 0; k < n_jobs && all_tuned; k++) {
+        const int lv = a.jobs[k].level;
+        if (lv >= 0 && lv <= 9 && !(tune.ns_per_block[lv] > 0.0f)) all_tuned = false;
+    }
+    if (all_tuned)
+        for (int l = 0; l < 10; l++) cost[l] = tune.ns_per_block[l];
+    float job_cost[kNoAllocMaxJobs];
+    for (uint32_t k = 0; k < a.n; k++) {
+        const JobDev& d = a.jobs[k];
+        job_cost[k] = ((d.level >= 0 && d.level <= 9) ? cost[d.level] : 0.0f) * d.tile_blocks;
+        uint32_t p = k;  // stable insertion by decreasing cost
+        while (p > 0 && job_cost[a.order[p - 1]] < job_cost[k]) {
+            a.order[p] = a.order[p - 1];
+            p--;
+        }
+        a.order[p] = k;
+    }
+    uint64_t ntiles = 0;
+    for (uint32_t p = 0; p < a.n; p++) {
+        a.first_tile[p] = (uint32_t)std::min<uint64_t>(ntiles, 0xffffffffu);
+        const JobDev& d = a.jobs[a.order[p]];
+        ntiles += tiles_of(d.block_count, d.tile_blocks);
+    }
+    if (ntiles > kNoAllocMaxTiles)
+        return fail(ICBC_ERR_INVALID_ARGUMENT, "%llu tiles: the no-allocation call takes at most %u",
+                    (unsigned long long)ntiles, kNoAllocMaxTiles);
+    a.first_tile[a.n] = (uint32_t)ntiles;
+    if (ntiles == 0) return ICBC_OK;
+    char* b = (char*)base;
+    const hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(launch_noalloc_setup(a, base, s));
+    const JobDev* djobs = (const JobDev*)(b + kNoAllocJobsOffset);
+    const uint2* dtiles = n_jobs > 1 ? (const uint2*)(b + kNoAllocTilesOffset) : nullptr;
+    const uint32_t grid = std::min<uint32_t>(grid_max, (uint32_t)ntiles);
+    HIP_TRY(launch_encode(djobs, dtiles, (uint32_t)ntiles, (uint32_t*)b, (float4*)(b + kNoAllocWorkspaceOffset), grid,
+                          encode_mode(a.jobs, a.n), refine_list_at(b + kNoAllocRefineOffset), s));
+    return ICBC_OK;
+}
+
+uint64_t icbc_gpu_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
+
 }  // extern "C"
 
 // ---- plans ------------------------------------------------------------------
@@ -1233,7 +1360,7 @@ int icbc_gpu_evaluate_bc3_error_batch(const uint8_t* rgba, const void* blocks, u
 
 // ---- diagnostics / plumbing ----------------------------------------------------
 
-const char* icbc_gpu_last_error(void) { return t_last_error.c_str(); }
+const char* icbc_gpu_last_error(void) { return t_last_error; }
 
 int icbc_gpu_device_name(char* buf, size_t len) {
     int dev = 0;

@@ -102,6 +102,32 @@ int icbc_gpu_compress_bc1_batch(const icbc_bc1_job* jobs, size_t n_jobs, void* s
  * (the plain comparison path). */
 int icbc_gpu_compress_bc1_plain(const icbc_bc1_job* jobs, size_t n_jobs, void* stream);
 
+/* No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same results as icbc_gpu_compress_bc1_batch, but the call
+ * allocates nothing -- no device, pinned or host heap memory.  Everything a
+ * launch needs besides the caller's inputs and outputs (job table, tile list,
+ * tile-queue counters, level-9 handoff list, workgroup workspaces) lives in a
+ * fixed scratch array of ICBC_NOALLOC_SCRATCH_BYTES that the library holds on
+ * every device it runs on until the process ends.  Limits per call:
+ *   - device-memory block jobs only (no ICBC_JOB_HOST_MEMORY, no ICBC_JOB_IMAGE);
+ *   - at most ICBC_NOALLOC_MAX_JOBS jobs, ICBC_NOALLOC_MAX_TILES tiles and
+ *     ICBC_NOALLOC_MAX_LEVEL9_BLOCKS level-9 blocks (larger calls get
+ *     ICBC_ERR_INVALID_ARGUMENT and launch nothing);
+ *   - the device must be set up first (icbc_gpu_init with it current, or any
+ *     other encode call on it), else ICBC_ERR_NOT_INITIALIZED;
+ *   - launches run on at most ICBC_NOALLOC_MAX_GRID resident workgroups.
+ * Every call on a device uses that device's one scratch array: calls that
+ * overlap on one device (other streams, other threads) race and must be
+ * serialised by the caller; calls on one stream are ordered and safe. */
+#define ICBC_NOALLOC_MAX_JOBS 32
+#define ICBC_NOALLOC_MAX_TILES 65536
+#define ICBC_NOALLOC_MAX_LEVEL9_BLOCKS 262144
+#define ICBC_NOALLOC_MAX_GRID 1024
+int icbc_gpu_compress_bc1_batch_noalloc(const icbc_bc1_job* jobs, size_t n_jobs, void* stream);
+/* Bytes of the scratch array held per device. */
+uint64_t icbc_gpu_noalloc_scratch_bytes(void);
+
 /* One block, host memory: the reference's compress_bc1 signature over C types.
  * Returns the block error (FLT_MAX for an invalid level, output untouched);
  * returns NaN and sets icbc_gpu_last_error() if the device call fails. */
