@@ -231,12 +231,13 @@ def cpu_baseline(blocks, level, tcm, tcb, threads, budget_cpu_s=20.0, seed=12):
 
 
 def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64 << 30,
-                  max_inflight=2, parity=None):
+                  max_inflight=3, parity=None, texels="rgba8"):
     """Poisson request stream against the request server: textures with sides
     log-uniform in [16, 2048], level uniform in 1..9, three-colour mode on for
     half of them, block mix 40/30/20/10, stored as RGBA8 (read by the encoder
-    as c / 255.0f).  Inputs sit in page-locked host memory and each request's
-    texels cross PCIe inside its latency.  First the whole
+    as c / 255.0f) or, with texels="float32", as float RGBA (256 B per block).
+    Inputs sit in page-locked host memory and each request's texels cross
+    PCIe inside its latency.  First the whole
     pool is submitted at once to find the saturation rate; then each QPS
     fraction runs for `seconds`.  Every served result is compared with the
     same texture encoded alone, and every pool texture encoded alone is
@@ -251,8 +252,9 @@ def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64
         w = int(round(np.exp(rng.uniform(np.log(16), np.log(2048)))))
         h = int(round(np.exp(rng.uniform(np.log(16), np.log(2048)))))
         blocks = synth.image_to_blocks(synth.make_image(seed * 10000 + k, w, h, synth.MIX_DEFAULT))
-        blocks8 = np.clip(np.round(blocks * 255), 0, 255).astype(np.uint8)   # RGBA8 texels
-        holder = g.pinned_copy(blocks8)
+        if texels == "rgba8":
+            blocks = np.clip(np.round(blocks * 255), 0, 255).astype(np.uint8)   # RGBA8 texels
+        holder = g.pinned_copy(blocks)
         pool.append({"blocks": holder.array, "holder": holder, "n": len(blocks),
                      "level": int(rng.integers(1, 10)), "tcm": bool(rng.integers(0, 2))})
     alone_runs = [g.compress_many(p["level"], p["blocks"], None, (1, 1, 1), p["tcm"], p["tcm"]) for p in pool]
@@ -260,7 +262,8 @@ def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64
     if parity is not None:
         # the oracle reads RGBA8 texels as c / 255.0f, as the encoder does
         for k, (p, r) in enumerate(zip(pool, alone_runs)):
-            f = np.asarray(p["blocks"], np.float32) / np.float32(255)
+            f = (np.asarray(p["blocks"], np.float32) / np.float32(255) if texels == "rgba8"
+                 else np.asarray(p["blocks"], np.float32))
             parity.check(f"serving pool texture {k}", p["level"], f, None, (1, 1, 1), p["tcm"], p["tcm"],
                          r[0], r[1], sample=1 << 30)
     srv = g.Server(budget, max_inflight=max_inflight)
@@ -350,7 +353,8 @@ def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64
             "pool_textures": pool_size, "mean_blocks_per_request": round(total_blocks / pool_size, 1),
             "memory_budget_bytes": budget, "max_inflight_launches": max_inflight,
             "seconds_per_rate": seconds, "runs": runs,
-            "input": "RGBA8 texels (64 B/block) in page-locked host memory, copied per request"}
+            "input": ("RGBA8 texels (64 B/block)" if texels == "rgba8" else "float32 RGBA texels (256 B/block)")
+                     + " in page-locked host memory, copied per request"}
 
 
 def single_image_levels(g, synth, levels=(1, 5, 9), steps=10, seed=7, parity=None):
@@ -577,6 +581,7 @@ def main():
 
     if args.serve:
         res = serving_bench(g, synth, 60.0, (0.5, 0.8, 0.95), parity=parity)
+        res_f = serving_bench(g, synth, 60.0, (0.8,), parity=parity, texels="float32")
         top = res["runs"][-1]
         finish({"metric": "BC1 serving: Mblocks/s and p50/p99 request latency under Poisson load",
                 "value": top["mblocks_per_s"], "unit": "Mblocks/s", "n_gpus": 1,
@@ -584,7 +589,7 @@ def main():
                 "p99_ms": top["latency_ms_p99"], "p50_ms": top["latency_ms_p50"],
                 "config": {"workload": "Poisson requests, sides log-uniform 16-2048, levels 1-9, "
                                        "three_color_mode 50/50, 64 GB budget", "device": dev},
-                "serving": res})
+                "serving": res, "serving_float32": res_f})
         return
 
     if args.workload != "batch":
@@ -728,7 +733,8 @@ def main():
 
     serving = None
     if not args.no_serve:
-        serving = serving_bench(g, synth, args.serve_seconds, (0.8,), parity=parity)
+        serving = {"rgba8": serving_bench(g, synth, args.serve_seconds, (0.8,), parity=parity),
+                   "float32": serving_bench(g, synth, args.serve_seconds, (0.8,), parity=parity, texels="float32")}
 
     value = total / ms_batched / 1e3
     out = {

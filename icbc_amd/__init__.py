@@ -103,7 +103,8 @@ class RequestInfo(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("launches", C.c_uint64), ("completed", C.c_uint64), ("blocks", C.c_uint64),
-                ("pending", C.c_uint64), ("inflight", C.c_uint64), ("memory_in_use", C.c_uint64)]
+                ("pending", C.c_uint64), ("inflight", C.c_uint64), ("memory_in_use", C.c_uint64),
+                ("memory_committed", C.c_uint64)]
 
 
 class Tuning(C.Structure):
@@ -712,7 +713,7 @@ class Server:
     budget, batched with whatever else is waiting, and run while earlier
     launches are still on the device."""
 
-    def __init__(self, memory_budget, max_inflight=2, max_batch=0):
+    def __init__(self, memory_budget, max_inflight=3, max_batch=0):
         opt = ServerOptions(int(memory_budget), int(max_inflight), int(max_batch))
         h = C.c_void_p()
         check(lib().icbc_server_create(C.byref(opt), C.byref(h)))

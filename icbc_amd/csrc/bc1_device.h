@@ -217,8 +217,24 @@ using Texels = TexelsOf<false>;
 // ---------------------------------------------------------------------------
 // Error of a block for a palette and index word (icbc.h:2955-2958, 3020-3028).
 
+// Channel weights all 1.0 (kUnit, the reference harness's setting): every
+// product by a weight or by the metric w * w is x * 1.0f == x, exact, so the
+// kUnit forms below skip those products and give bit-identical results.
+template <bool kUnit>
+__device__ __forceinline__ f3 wmul(f3 x, f3 w) {
+    if constexpr (kUnit) return x;
+    else return x * w;
+}
+// dot(e, metric) (icbc.h:2742): ((e.x m.x + e.y m.y) + e.z m.z)
+template <bool kUnit>
+__device__ __forceinline__ float metric_dot(f3 e, f3 m) {
+    if constexpr (kUnit) return (e.x + e.y) + e.z;
+    else return dot3(e, m);
+}
+
+template <bool kUnit = false>
 __device__ __forceinline__ float texel_error(f3 p, f3 c, f3 w) {
-    f3 d = ((p - c) * w) * 255.0f;
+    f3 d = wmul<kUnit>(p - c, w) * 255.0f;
     return dot3(d, d);
 }
 
@@ -265,20 +281,20 @@ __device__ __forceinline__ f3 pick(const f3 pal[4], uint32_t k) {
     return (k & 2) ? hi : lo;
 }
 
-template <typename Tx>
+template <bool kUnit, typename Tx>
 __device__ __forceinline__ float palette_error(const Tx& t, f3 cw, const f3 pal[4], uint32_t indices) {
     float e = 0.0f;
 #pragma unroll 4
     for (int i = 0; i < 16; i++)
-        e += t.weight(i) * texel_error(pick(pal, (indices >> (2 * i)) & 3), t.color(i), cw);
+        e += t.weight(i) * texel_error<kUnit>(pick(pal, (indices >> (2 * i)) & 3), t.color(i), cw);
     return e;
 }
 
-template <typename Tx>
+template <bool kUnit, typename Tx>
 __device__ __forceinline__ float code_error(const Tx& t, f3 cw, int decoder, Code c) {
     f3 pal[4];
     palette_f(col0(c), col1(c), decoder, pal);
-    return palette_error(t, cw, pal, c.idx);
+    return palette_error<kUnit>(t, cw, pal, c.idx);
 }
 
 // ---------------------------------------------------------------------------
@@ -288,38 +304,39 @@ __device__ __forceinline__ float code_error(const Tx& t, f3 cw, int decoder, Cod
 // summing the errors would.
 
 // compute_indices4 rule (icbc.h:3097-3131)
-template <typename Src>
+template <bool kUnit, typename Src>
 __device__ __forceinline__ float select4(const Src& t, f3 w, const f3 pal[4], uint32_t& indices) {
-    const f3 p0 = pal[0] * w, p1 = pal[1] * w, p2 = pal[2] * w, p3 = pal[3] * w;
+    const f3 p0 = wmul<kUnit>(pal[0], w), p1 = wmul<kUnit>(pal[1], w), p2 = wmul<kUnit>(pal[2], w),
+             p3 = wmul<kUnit>(pal[3], w);
     uint32_t out = 0;
     float e = 0.0f;
 #pragma unroll TexelUnroll<Src>::value
     for (int i = 0; i < 16; i++) {
         const f3 x = t.color(i);
-        const f3 c = x * w;
+        const f3 c = wmul<kUnit>(x, w);
         const f3 e0 = c - p0, e1 = c - p1, e2 = c - p2, e3 = c - p3;
         const float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2), d3 = dot3(e3, e3);
         const uint32_t hi = ((d1 > d2) & (d0 > d2)) | ((d0 > d3) & (d1 > d3));
         const uint32_t lo = (d0 > d3) & (d2 > d3);
         const uint32_t k = lo | (hi << 1);
         out |= k << (2 * i);
-        e += t.weight(i) * texel_error(pick(pal, k), x, w);
+        e += t.weight(i) * texel_error<kUnit>(pick(pal, k), x, w);
     }
     indices = out;
     return e;
 }
 
 // compute_indices3 rule (icbc.h:3133-3176)
-template <typename Tx>
+template <bool kUnit, typename Tx>
 __device__ __forceinline__ float select3(const Tx& t, f3 w, bool allow_black, const f3 pal[4],
                                          uint32_t& indices) {
-    const f3 p0 = pal[0] * w, p1 = pal[1] * w, p2 = pal[2] * w;
+    const f3 p0 = wmul<kUnit>(pal[0], w), p1 = wmul<kUnit>(pal[1], w), p2 = wmul<kUnit>(pal[2], w);
     uint32_t out = 0;
     float e = 0.0f;
 #pragma unroll 4
     for (int i = 0; i < 16; i++) {
         const f3 x = t.color(i);
-        const f3 c = x * w;
+        const f3 c = wmul<kUnit>(x, w);
         uint32_t k;
         if (allow_black) {
             const f3 e0 = p0 - c, e1 = p1 - c, e2 = p2 - c;
@@ -336,7 +353,7 @@ __device__ __forceinline__ float select3(const Tx& t, f3 w, bool allow_black, co
             k = i1 | (i2 << 1);
         }
         out |= k << (2 * i);
-        e += t.weight(i) * texel_error(pick(pal, k), x, w);
+        e += t.weight(i) * texel_error<kUnit>(pick(pal, k), x, w);
     }
     indices = out;
     return e;
@@ -347,13 +364,14 @@ __device__ __forceinline__ float select3(const Tx& t, f3 w, bool allow_black, co
 // `pal` (the candidate block's palette).
 // Indices of the 16 texels against `sel` (palette of the current best block):
 // nearest entry under the colour weights, lowest index on ties.
-template <typename Src>
+template <bool kUnit, typename Src>
 __device__ __forceinline__ uint32_t select_indices(const Src& t, f3 w, const f3 sel[4]) {
-    const f3 p0 = sel[0] * w, p1 = sel[1] * w, p2 = sel[2] * w, p3 = sel[3] * w;
+    const f3 p0 = wmul<kUnit>(sel[0], w), p1 = wmul<kUnit>(sel[1], w), p2 = wmul<kUnit>(sel[2], w),
+             p3 = wmul<kUnit>(sel[3], w);
     uint32_t out = 0;
 #pragma unroll TexelUnroll<Src>::value
     for (int i = 0; i < 16; i++) {
-        const f3 c = t.color(i) * w;
+        const f3 c = wmul<kUnit>(t.color(i), w);
         const f3 e0 = c - p0, e1 = c - p1, e2 = c - p2, e3 = c - p3;
         const float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2), d3 = dot3(e3, e3);
         const uint32_t i1 = (d1 <= d0) & (d1 < d2) & (d1 < d3);
@@ -364,20 +382,12 @@ __device__ __forceinline__ uint32_t select_indices(const Src& t, f3 w, const f3 
     return out;
 }
 
-// Weighted squared error of fixed `indices` against palette `pal`.
-template <typename Src>
-__device__ __forceinline__ float indexed_error(const Src& t, f3 w, const f3 pal[4], uint32_t indices) {
-    float e = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 16; i++) e += t.weight(i) * texel_error(pick(pal, (indices >> (2 * i)) & 3), t.color(i), w);
-    return e;
-}
 
 
 // ---------------------------------------------------------------------------
 // Block output (icbc.h:3238-3274)
 
-template <typename Src>
+template <bool kUnit, typename Src>
 __device__ __forceinline__ float emit4(const Src& t, f3 cw, int decoder, f3 a, f3 b, Code& out) {
     uint32_t c0 = to565(a), c1 = to565(b);
     if (c0 < c1) {
@@ -388,12 +398,12 @@ __device__ __forceinline__ float emit4(const Src& t, f3 cw, int decoder, f3 a, f
     f3 pal[4];
     palette_f(c0, c1, decoder, pal);
     uint32_t idx;
-    const float e = select4(t, cw, pal, idx);
+    const float e = select4<kUnit>(t, cw, pal, idx);
     out = make_code(c0, c1, idx);
     return e;
 }
 
-template <typename Tx>
+template <bool kUnit, typename Tx>
 __device__ __forceinline__ float emit3(const Tx& t, f3 cw, int decoder, bool allow_black,
                                        f3 a, f3 b, Code& out) {
     uint32_t c0 = to565(a), c1 = to565(b);
@@ -405,7 +415,7 @@ __device__ __forceinline__ float emit3(const Tx& t, f3 cw, int decoder, bool all
     f3 pal[4];
     palette_f(c0, c1, decoder, pal);
     uint32_t idx;
-    const float e = select3(t, cw, allow_black, pal, idx);
+    const float e = select3<kUnit>(t, cw, allow_black, pal, idx);
     out = make_code(c0, c1, idx);
     return e;
 }
@@ -413,7 +423,7 @@ __device__ __forceinline__ float emit3(const Tx& t, f3 cw, int decoder, bool all
 // output_block4 and, when `with3`, output_block3 (icbc.h:3238-3274) sharing
 // one pass over the texels: per texel both index rules run and each error sum
 // accumulates in texel order, as two separate passes would.
-template <typename Tx>
+template <bool kUnit, typename Tx>
 __device__ __forceinline__ float emit43(const Tx& t, f3 w, int decoder, bool allow_black,
                                         bool with3, f3 a4, f3 b4, f3 a3, f3 b3, Code& out4, Code& out3, float& err3) {
     uint32_t c40 = to565(a4), c41 = to565(b4);
@@ -431,15 +441,16 @@ __device__ __forceinline__ float emit43(const Tx& t, f3 w, int decoder, bool all
     f3 pal4[4], pal3[4];
     palette_f(c40, c41, decoder, pal4);
     palette_f(c30, c31, decoder, pal3);
-    const f3 p0 = pal4[0] * w, p1 = pal4[1] * w, p2 = pal4[2] * w, p3 = pal4[3] * w;
-    const f3 q0 = pal3[0] * w, q1 = pal3[1] * w, q2 = pal3[2] * w;
+    const f3 p0 = wmul<kUnit>(pal4[0], w), p1 = wmul<kUnit>(pal4[1], w), p2 = wmul<kUnit>(pal4[2], w),
+             p3 = wmul<kUnit>(pal4[3], w);
+    const f3 q0 = wmul<kUnit>(pal3[0], w), q1 = wmul<kUnit>(pal3[1], w), q2 = wmul<kUnit>(pal3[2], w);
     uint32_t idx4 = 0, idx3 = 0;
     float e4 = 0.0f, e3 = 0.0f;
 #pragma unroll 2
     for (int i = 0; i < 16; i++) {
         const f3 x = t.color(i);
         const float wt = t.weight(i);
-        const f3 c = x * w;
+        const f3 c = wmul<kUnit>(x, w);
         {  // compute_indices4 rule (icbc.h:3097-3131)
             const f3 e0 = c - p0, e1 = c - p1, e2 = c - p2, ee3 = c - p3;
             const float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2), d3 = dot3(ee3, ee3);
@@ -447,7 +458,7 @@ __device__ __forceinline__ float emit43(const Tx& t, f3 w, int decoder, bool all
             const uint32_t lo = (d0 > d3) & (d2 > d3);
             const uint32_t k = lo | (hi << 1);
             idx4 |= k << (2 * i);
-            e4 += wt * texel_error(pick(pal4, k), x, w);
+            e4 += wt * texel_error<kUnit>(pick(pal4, k), x, w);
         }
         if (with3) {  // compute_indices3 rule (icbc.h:3133-3176)
             uint32_t k;
@@ -466,7 +477,7 @@ __device__ __forceinline__ float emit43(const Tx& t, f3 w, int decoder, bool all
                 k = i1 | (i2 << 1);
             }
             idx3 |= k << (2 * i);
-            e3 += wt * texel_error(pick(pal3, k), x, w);
+            e3 += wt * texel_error<kUnit>(pick(pal3, k), x, w);
         }
     }
     out4 = make_code(c40, c41, idx4);
@@ -684,6 +695,7 @@ __device__ __forceinline__ float recip_rn(float b) {
     return y1;
 }
 
+template <bool kUnit>
 __device__ __forceinline__ float candidate4(float4 s0, float4 s1, float4 s2, float4 tot, f3 metric,
                                             const DecoderWeights& dw, f3& a, f3& b) {
     f3 x0 = mk(s0.x, s0.y, s0.z), x1 = mk(s1.x, s1.y, s1.z), x2 = mk(s2.x, s2.y, s2.z);
@@ -707,7 +719,7 @@ __device__ __forceinline__ float candidate4(float4 s0, float4 s1, float4 s2, flo
 
     const f3 e2 = (a * (b * ab - ax) - b * bx) * 2.0f;
     const f3 e1 = (a * a) * alpha2 + ((b * b) * beta2 + e2);
-    return dot3(e1, metric);
+    return metric_dot<kUnit>(e1, metric);
 }
 
 // ---------------------------------------------------------------------------
@@ -744,10 +756,12 @@ __device__ __forceinline__ BoundCoefs bound_coefs(const DecoderWeights& dw) {
 
 // Centred SAT row (`invW` = 1 / total weight, `t` = colour totals), colour
 // part scaled by the channel weights `cw`.
+template <bool kUnit>
 __device__ __forceinline__ float4 centred_row(float4 s, float invW, f3 t, f3 cw) {
     const float f = s.w * invW;
-    return make_float4(__builtin_fmaf(-f, t.x, s.x) * cw.x, __builtin_fmaf(-f, t.y, s.y) * cw.y,
-                       __builtin_fmaf(-f, t.z, s.z) * cw.z, s.w);
+    const f3 d = wmul<kUnit>(mk(__builtin_fmaf(-f, t.x, s.x), __builtin_fmaf(-f, t.y, s.y), __builtin_fmaf(-f, t.z, s.z)),
+                             cw);
+    return make_float4(d.x, d.y, d.z, s.w);
 }
 
 // Sums of the bound's terms over the first two rows of the candidates with
@@ -798,6 +812,7 @@ __device__ __forceinline__ float4 bound_row(const f3 (&c)[16], const float (&w)[
     return make_float4(margin, ok ? 1.0f : 0.0f, 0.0f, 0.0f);
 }
 
+template <bool kUnit>
 __device__ __forceinline__ float candidate3(float4 s0, float4 s1, float4 tot, f3 metric, f3& a, f3& b) {
     f3 x0 = mk(s0.x, s0.y, s0.z), x1 = mk(s1.x, s1.y, s1.z);
     float w0 = s0.w, w1 = s1.w;
@@ -818,7 +833,7 @@ __device__ __forceinline__ float candidate3(float4 s0, float4 s1, float4 tot, f3
 
     const f3 e2 = (a * (b * ab - ax) - b * bx) * 2.0f;
     const f3 e1 = (a * a) * alpha2 + ((b * b) * beta2 + e2);
-    return dot3(e1, metric);
+    return metric_dot<kUnit>(e1, metric);
 }
 
 // ---------------------------------------------------------------------------

@@ -244,6 +244,7 @@ __device__ __forceinline__ int next_block(int my_n, int tile_blocks, int from, i
 // workgroup's workspace through a double-buffered LDS slot: the next block's
 // rows are loaded into registers while the current block is searched.
 // Block b's SAT is at ws + b * kSatRows.
+template <bool kUnit>
 __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile_blocks, int my_n, f3 cw, f3 metric,
                                        const DecoderWeights& dw) {
     constexpr int kMinN = 2;
@@ -261,7 +262,7 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
         if (nb >= kPruneMinColors) {
             const f3 t = mk(lane_value(row.x, nb), lane_value(row.y, nb), lane_value(row.z, nb));
             const float invW = __builtin_amdgcn_rcpf(lane_value(row.w, nb));
-            if (lane <= nb) sh.centred[lane] = centred_row(row, invW, t, cw);
+            if (lane <= nb) sh.centred[lane] = centred_row<kUnit>(row, invW, t, cw);
         }
         __syncthreads();
         // prefetch the next block's rows while this one is searched
@@ -350,7 +351,7 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
             float thr;
             {
                 f3 a, e;
-                float err = candidate4(sat[four_c0(pkey)], sat[four_c1(pkey)], sat[four_c2(pkey)], tot, metric,
+                float err = candidate4<kUnit>(sat[four_c0(pkey)], sat[four_c1(pkey)], sat[four_c2(pkey)], tot, metric,
                                        dw, a, e);
                 thr = wave_min(err == err ? err : FLT_MAX);
             }
@@ -386,8 +387,8 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
                 const int i = sh.survivors[j];
                 const uint32_t pk = sh.four[i];
                 f3 a, e;
-                const float err = candidate4(sat[four_c0(pk)], sat[four_c1(pk)], sat[four_c2(pk)], tot,
-                                             metric, dw, a, e);
+                const float err = candidate4<kUnit>(sat[four_c0(pk)], sat[four_c1(pk)], sat[four_c2(pk)], tot,
+                                                    metric, dw, a, e);
                 if ((err < best) | ((err == best) & (i < besti) & (besti != kNoWinner))) {
                     best = err;
                     besti = i;
@@ -401,8 +402,8 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
                 const uint32_t pk = pk_next;
                 pk_next = sh.four[min(i + kTile, 975)];
                 f3 a, e;
-                const float err = candidate4(sat[four_c0(pk)], sat[four_c1(pk)], sat[four_c2(pk)], tot,
-                                             metric, dw, a, e);
+                const float err = candidate4<kUnit>(sat[four_c0(pk)], sat[four_c1(pk)], sat[four_c2(pk)], tot,
+                                                    metric, dw, a, e);
                 if (err < best) {
                     best = err;
                     besti = i;
@@ -430,6 +431,7 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
 // picks its lowest-index minimum for the owning lane (-1 if none), B moves
 // up, and the freed slot takes the rows of the next block, prefetched into
 // registers one block ahead.
+template <bool kUnit>
 __device__ __forceinline__ int search3(Shared& sh, const float4* ws_in, int alt_offset, int my_alt, int tile_blocks,
                                        int my_n, f3 metric) {
     const int lane = threadIdx.x;
@@ -467,7 +469,7 @@ __device__ __forceinline__ int search3(Shared& sh, const float4* ws_in, int alt_
             const float4 tot = in_a ? sat_a[na] : sat_b[nb];
             const uint32_t pk = sh.three[min(ci, 159)];
             f3 p0, p1;
-            const float err = candidate3(sat[pk & 0xff], sat[(pk >> 8) & 0xff], tot, metric, p0, p1);
+            const float err = candidate3<kUnit>(sat[pk & 0xff], sat[(pk >> 8) & 0xff], tot, metric, p0, p1);
             if (in_a && err < best_a) {
                 best_a = err;
                 bi_a = ci;
@@ -521,16 +523,16 @@ __device__ __forceinline__ int search3(Shared& sh, const float4* ws_in, int alt_
 // Levels 0-1: box fit, output block, and at level 1 least-squares endpoints
 // and a second output block (icbc.h:3769-3786 without reduce_colors, as at
 // these levels).
-template <typename Src>
+template <bool kUnit, typename Src>
 __device__ __forceinline__ float encode_box(const Src& t, bool lsq, f3 cw, int dec, Code& out) {
     f3 c0, c1;
     box_fit([&](int i) { return t.color(i); }, 16, c0, c1);
-    float err = emit4(t, cw, dec, c0, c1, out);
+    float err = emit4<kUnit>(t, cw, dec, c0, c1, out);
     if (lsq) {
         f3 a, b;
         if (lsq_endpoints(out.idx, t, a, b)) {
             Code o2;
-            const float e2 = emit4(t, cw, dec, a, b, o2);
+            const float e2 = emit4<kUnit>(t, cw, dec, a, b, o2);
             if (e2 < err) {
                 err = e2;
                 out = o2;
@@ -591,6 +593,7 @@ __device__ __forceinline__ void stage_tile(const JobDev& job, uint64_t first, in
 }
 
 // A tile of a level 0-1 (or invalid-level) job.
+template <bool kUnit>
 __device__ __forceinline__ void encode_box_tile(const JobDev& job, uint64_t first, SharedBox& sh) {
     const int lane = threadIdx.x;
     const int nblk = (int)min<uint64_t>((uint64_t)job.tile_blocks, job.block_count - first);
@@ -606,10 +609,10 @@ __device__ __forceinline__ void encode_box_tile(const JobDev& job, uint64_t firs
             Code out{0u, 0u};
             if (job.weights) {
                 const TexelsLds<true> t{row, reinterpret_cast<const float*>(job.weights + bi * 4)};
-                err = encode_box(t, p.lsq, cw, job.decoder, out);
+                err = encode_box<kUnit>(t, p.lsq, cw, job.decoder, out);
             } else {
                 const TexelsLds<false> t{row, nullptr};
-                err = encode_box(t, p.lsq, cw, job.decoder, out);
+                err = encode_box<kUnit>(t, p.lsq, cw, job.decoder, out);
             }
             job.codes[bi] = make_uint2(out.cols, out.idx);
         }
@@ -626,7 +629,7 @@ __device__ __forceinline__ void encode_box_tile(const JobDev& job, uint64_t firs
 // everything (the refinement walk's registers weigh only on that one).
 // kRgba8: the job's colours are RGBA8 words (one instance per texel format,
 // chosen per tile, so neither pays for the other's loads).
-template <int kMode, bool kRgba8>
+template <int kMode, bool kRgba8, bool kUnit>
 __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_index, uint64_t first, Shared& sh,
                                             float4* ws, const RefineList& rl) {
     const int lane = threadIdx.x;
@@ -658,7 +661,7 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
     }
     if (!done && n == 1) {  // one colour: optimal single-colour tables (icbc.h:3761-3764)
         out = single_color(to_u8(rc[0].x), to_u8(rc[0].y), to_u8(rc[0].z), dec);
-        err = code_error(tx, cw, dec, out);
+        err = code_error<kUnit>(tx, cw, dec, out);
         written = true;
         done = true;
     }
@@ -667,13 +670,13 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
         f3 c0, c1;
         if (p.cluster) box_fit([&](int i) { return rc[i]; }, n, c0, c1);
         else box_fit([&](int i) { return tx.color(i); }, 16, c0, c1);
-        err = emit4(tx, cw, dec, c0, c1, out);
+        err = emit4<kUnit>(tx, cw, dec, c0, c1, out);
         written = true;
         if (p.lsq) {
             f3 a, b;
             if (lsq_endpoints(out.idx, tx, a, b)) {
                 Code o2;
-                const float e2 = emit4(tx, cw, dec, a, b, o2);
+                const float e2 = emit4<kUnit>(tx, cw, dec, a, b, o2);
                 if (e2 < err) {
                     err = e2;
                     out = o2;
@@ -716,26 +719,26 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
         if (use3) n3 = m;
         __syncthreads();
         // 3-cluster SATs: the skip_blacks table where one was built, else the 4-cluster one
-        const int win4 = search4(sh, ws, tile_blocks, fit ? n : 0, cw, metric, dw);
-        const int win3 = search3(sh, ws, kTile * kSatRows, skip ? 1 : 0, tile_blocks, use3 ? m : 0, metric);
+        const int win4 = search4<kUnit>(sh, ws, tile_blocks, fit ? n : 0, cw, metric, dw);
+        const int win3 = search3<kUnit>(sh, ws, kTile * kSatRows, skip ? 1 : 0, tile_blocks, use3 ? m : 0, metric);
 
         if (fit) {
             f3 a4 = mk(0.0f, 0.0f, 0.0f), b4 = mk(0.0f, 0.0f, 0.0f);
             if (win4 >= 0) {
                 const uint32_t pk = sh.four[win4];
-                candidate4(sat4[four_c0(pk)], sat4[four_c1(pk)], sat4[four_c2(pk)], sat4[n], metric, dw, a4,
+                candidate4<kUnit>(sat4[four_c0(pk)], sat4[four_c1(pk)], sat4[four_c2(pk)], sat4[n], metric, dw, a4,
                            b4);
             }
             f3 a3 = mk(0.0f, 0.0f, 0.0f), b3 = mk(0.0f, 0.0f, 0.0f);
             if (use3 && win3 >= 0) {
                 const float4* s3 = skip ? sat3 : sat4;
                 const uint32_t pk = sh.three[win3];
-                candidate3(s3[pk & 0xff], s3[(pk >> 8) & 0xff], s3[m], metric, a3, b3);
+                candidate3<kUnit>(s3[pk & 0xff], s3[(pk >> 8) & 0xff], s3[m], metric, a3, b3);
             }
             // output_block4 and output_block3 in one pass over the texels
             Code oc, o3;
             float e3;
-            float ec = emit43(tx, cw, dec, job.tcb != 0, use3, a4, b4, a3, b3, oc, o3, e3);
+            float ec = emit43<kUnit>(tx, cw, dec, job.tcb != 0, use3, a4, b4, a3, b3, oc, o3, e3);
             if (use3 && e3 < ec) {
                 ec = e3;
                 oc = o3;
@@ -788,7 +791,8 @@ __device__ __forceinline__ void load_tables(S& sh) {
 // tiles are taken from a queue in list order (the batched path); without, a
 // workgroup takes tiles blockIdx.x, blockIdx.x + gridDim.x, ...  With `tiles`
 // null the launch covers one job, tile k = blocks [64 k, 64 k + 64).
-template <int kMode, int kTex>
+// kUnit: every job of the launch has channel weights (1, 1, 1).
+template <int kMode, int kTex, bool kUnit>
 __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) void bc1_encode_kernel(
     const JobDev* __restrict__ jobs, const uint2* __restrict__ tiles, uint32_t ntiles,
     uint32_t* __restrict__ counter, float4* __restrict__ workspace, RefineList refine) {
@@ -816,12 +820,12 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) v
             job = tile.x;
             first = tile.y;
         }
-        if constexpr (kMode == kModeBox) encode_box_tile(jobs[job], first, sh);  // either format, per tile
+        if constexpr (kMode == kModeBox) encode_box_tile<kUnit>(jobs[job], first, sh);  // either format, per tile
         else if constexpr (kTex == kTexMixed) {
-            if (jobs[job].rgba8) encode_tile<kMode, true>(jobs[job], job, first, sh, ws, refine);
-            else encode_tile<kMode, false>(jobs[job], job, first, sh, ws, refine);
+            if (jobs[job].rgba8) encode_tile<kMode, true, kUnit>(jobs[job], job, first, sh, ws, refine);
+            else encode_tile<kMode, false, kUnit>(jobs[job], job, first, sh, ws, refine);
         } else {
-            encode_tile<kMode, kTex == kTexRgba8>(jobs[job], job, first, sh, ws, refine);
+            encode_tile<kMode, kTex == kTexRgba8, kUnit>(jobs[job], job, first, sh, ws, refine);
         }
         if (queue) {
             uint32_t v = 0;
@@ -864,21 +868,23 @@ constexpr int kRefineThreads = 256;
 // order of operations as indexed_error.
 // With kWeighted false the texels' weights are all 1.0 and the products
 // w * error, which equal the errors exactly, are not formed.
-template <bool kWeighted, typename Src>
+template <bool kWeighted, bool kUnit, typename Src>
 __device__ __forceinline__ float indexed_error_lds(const Src& t, f3 w, const float4 (*pal)[kRefineThreads], int tid,
                                                    uint32_t indices) {
     float e = 0.0f;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         const float4 p = pal[(indices >> (2 * i)) & 3][tid];
-        const float te = texel_error(mk(p.x, p.y, p.z), t.color(i), w);
+        const float te = texel_error<kUnit>(mk(p.x, p.y, p.z), t.color(i), w);
         e += kWeighted ? t.weight(i) * te : te;
     }
     return e;
 }
 
-__global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void bc1_refine_kernel(const JobDev* __restrict__ jobs,
-                                                                     RefineList rl) {
+// kUnit: every level-9 job of the launch has channel weights (1, 1, 1).
+template <bool kUnit>
+__global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void bc1_refine_kernel(
+    const JobDev* __restrict__ jobs, RefineList rl) {
     // each thread's candidate palette, written and read back by the same lane
     __shared__ float4 s_pal[4][kRefineThreads];
     const uint32_t total = rl.counters[0];
@@ -962,7 +968,7 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
             if (has && stale) {
                 f3 sel[4];
                 palette_f(col0(out), col1(out), decoder, sel);
-                idx = select_indices(r, cw, sel);
+                idx = select_indices<kUnit>(r, cw, sel);
                 stale = false;
             }
             continue;
@@ -989,8 +995,8 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
             for (int k = 0; k < 4; k++) s_pal[k][threadIdx.x] = make_float4(pal[k].x, pal[k].y, pal[k].z, 0.0f);
             // the unweighted form when no lane taking a step walks a weighted block
-            const float e = __ballot(weighted) != 0 ? indexed_error_lds<true>(r, cw, s_pal, threadIdx.x, idx)
-                                                    : indexed_error_lds<false>(r, cw, s_pal, threadIdx.x, idx);
+            const float e = __ballot(weighted) != 0 ? indexed_error_lds<true, kUnit>(r, cw, s_pal, threadIdx.x, idx)
+                                                    : indexed_error_lds<false, kUnit>(r, cw, s_pal, threadIdx.x, idx);
             const uint64_t acc = __ballot(valid && e < best) & group_bits;
             bool finished;
             if (acc) {
@@ -1135,7 +1141,7 @@ __global__ void bc1_evaluate_kernel(const uint32_t* __restrict__ rgba, const uin
 
 int encode_workgroups_per_cu() {
     int cluster = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&cluster, bc1_encode_kernel<kModeCluster, kTexFloat>, kTile, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&cluster, bc1_encode_kernel<kModeCluster, kTexFloat, false>, kTile, 0) !=
         hipSuccess)
         cluster = 1;
     return std::max(1, cluster);
@@ -1156,7 +1162,7 @@ uint32_t box_grid() {
         if (c.first == dev) return c.second;
     int cus = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bc1_encode_kernel<kModeBox, kTexFloat>, kTile, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bc1_encode_kernel<kModeBox, kTexFloat, false>, kTile, 0) !=
             hipSuccess ||
         per_cu < 1)
         per_cu = 1;
@@ -1176,7 +1182,7 @@ uint32_t refine_grid() {
         if (c.first == dev) return c.second;
     int cus = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bc1_refine_kernel, kRefineThreads, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bc1_refine_kernel<false>, kRefineThreads, 0) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
     const uint32_t g = (uint32_t)cus * (uint32_t)per_cu;
@@ -1196,7 +1202,8 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    const int tex = mode >> 4;
+    const int tex = (mode >> 4) & 3;
+    const bool unit = (mode & kUnitWeights) != 0;
     mode &= 15;
     if (mode == kModeAll) {
         hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, refine.counters, 2u, stride);
@@ -1205,21 +1212,27 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
     }
     // every schedule covers all tiles with any grid (queue claims or a
     // grid-stride walk), so the level 0-1 instance runs at its own residency
-    if (mode == kModeBox)
-        hipLaunchKernelGGL((bc1_encode_kernel<kModeBox, kTexFloat>), dim3(std::min(grid, box_grid())), dim3(kTile), 0,
-                           stream, jobs, tiles, ntiles, counter, workspace, refine);
-    else if (tex == kTexRgba8)
-        hipLaunchKernelGGL((bc1_encode_kernel<kModeCluster, kTexRgba8>), dim3(grid), dim3(kTile), 0, stream, jobs, tiles,
-                           ntiles, counter, workspace, refine);
-    else if (tex == kTexMixed)
-        hipLaunchKernelGGL((bc1_encode_kernel<kModeCluster, kTexMixed>), dim3(grid), dim3(kTile), 0, stream, jobs, tiles,
-                           ntiles, counter, workspace, refine);
-    else
-        hipLaunchKernelGGL((bc1_encode_kernel<kModeCluster, kTexFloat>), dim3(grid), dim3(kTile), 0, stream, jobs, tiles,
-                           ntiles, counter, workspace, refine);
+    const dim3 g(grid), bg(std::min(grid, box_grid())), blk(kTile);
+#define ENC(M, T, U) hipLaunchKernelGGL((bc1_encode_kernel<M, T, U>), (M) == kModeBox ? bg : g, blk, 0, stream, jobs, \
+                                        tiles, ntiles, counter, workspace, refine)
+    // The unit-weight instances run the level 0-1 kernel and the refinement;
+    // the cluster kernel has only the general one (its unit-weight build was
+    // measured slower: the compiler schedules the search worse).
+    if (mode == kModeBox) {
+        if (unit) ENC(kModeBox, kTexFloat, true);
+        else ENC(kModeBox, kTexFloat, false);
+    } else if (tex == kTexRgba8) {
+        ENC(kModeCluster, kTexRgba8, false);
+    } else if (tex == kTexMixed) {
+        ENC(kModeCluster, kTexMixed, false);
+    } else {
+        ENC(kModeCluster, kTexFloat, false);
+    }
+#undef ENC
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || mode != kModeAll) return e;
-    hipLaunchKernelGGL(bc1_refine_kernel, dim3(refine_grid()), dim3(kRefineThreads), 0, stream, jobs, refine);
+    if (unit) hipLaunchKernelGGL(bc1_refine_kernel<true>, dim3(refine_grid()), dim3(kRefineThreads), 0, stream, jobs, refine);
+    else hipLaunchKernelGGL(bc1_refine_kernel<false>, dim3(refine_grid()), dim3(kRefineThreads), 0, stream, jobs, refine);
     return hipGetLastError();
 }
 

@@ -108,8 +108,15 @@ constexpr int kTexFloat = 0;
 constexpr int kTexRgba8 = 1;
 constexpr int kTexMixed = 2;
 
-// `mode` selects the kernel instance (level mode | texel format << 4);
-// encode_mode() gives the one a set of jobs needs.
+// Every job of a launch has channel weights exactly (1, 1, 1): the level 0-1
+// and refinement kernels' unit-weight instances skip the products by the
+// weights (exact identities); the cluster kernel ignores the flag.
+constexpr int kUnitWeights = 1 << 6;
+
+inline bool unit_weights(const JobDev& j) { return j.cw0 == 1.0f && j.cw1 == 1.0f && j.cw2 == 1.0f; }
+
+// `mode` selects the kernel instance (level mode | texel format << 4 |
+// kUnitWeights); encode_mode() gives the one a set of jobs needs.
 // With kModeAll, `refine` holds room for the launch's level-9 blocks.
 hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, uint32_t* counter,
                          float4* workspace, uint32_t grid, int mode, RefineList refine, hipStream_t stream);
@@ -137,7 +144,10 @@ inline int encode_mode(const JobDev* jobs, size_t n) {
         else any_float = true;
     }
     const int tex = any_rgba8 ? (any_float ? kTexMixed : kTexRgba8) : kTexFloat;
-    return mode | (tex << 4);
+    bool unit = true;
+    for (size_t k = 0; k < n; k++)
+        if (!unit_weights(jobs[k])) unit = false;
+    return mode | (tex << 4) | (unit ? kUnitWeights : 0);
 }
 
 // The one-workgroup-per-tile schedule gives every tile its own workspace

@@ -1166,7 +1166,7 @@ int compress_one(int level, const float* input_colors, const float* input_weight
     static_assert(sizeof(JobDev) <= 128, "job descriptor must fit the block buffer");
     memcpy(h + 384, &jd, sizeof jd);
     HIP_TRY(hipMemcpyAsync(d, h, 512, hipMemcpyHostToDevice, stream));
-    HIP_TRY(launch_encode((const JobDev*)(d + 384), nullptr, 1, nullptr, (float4*)(d + 768), 1, encode_mode_of_level(level),
+    HIP_TRY(launch_encode((const JobDev*)(d + 384), nullptr, 1, nullptr, (float4*)(d + 768), 1, encode_mode(&jd, 1),
                           refine_list_at(d + 768 + kWorkspaceBytesPerWorkgroup), stream));
     HIP_TRY(hipMemcpyAsync(h + 320, d + 320, 32, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));

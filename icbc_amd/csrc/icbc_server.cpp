@@ -1,11 +1,15 @@
 // Request server: continuous batching of BC1 encode requests on one device.
 //
 // A request is one texture (a host-memory icbc_bc1_job).  The server
+//   - holds at most `memory_budget` bytes of device memory in all: request
+//     buffers, each launch's job table, tile list and level-9 handoff list,
+//     and the launch slots' workspaces are all charged against it;
 //   - admits requests in arrival order while their device footprint fits the
-//     memory budget (a request larger than the whole budget is rejected);
+//     budget (a request that could never fit is rejected at submission);
 //   - keeps request buffers resident: device memory comes from a pool of
-//     size-classed blocks that are reused, never returned to the driver
-//     while the server lives;
+//     size-classed blocks carved from slabs and reused; a slab goes back to
+//     the driver only when all its blocks are free and another class needs
+//     the room;
 //   - batches every admitted, not yet launched request into one encode launch
 //     (the same persistent kernel and tile queue as the batch API), with up to
 //     `max_inflight` launches overlapping on separate streams, so requests
@@ -48,6 +52,14 @@ double now_s() {
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// Fewest resident workgroups a server launches with (the workspace a small
+// budget must still hold).
+constexpr uint64_t kMinServerGrid = 64;
+
+// Device memory the runtime itself holds for each launch slot's stream (its
+// queue and signal pools; 4-5 MiB measured on MI355X), charged to the budget.
+constexpr size_t kStreamAllowance = 6u << 20;
+
 // Device footprint of a request: colours (float or RGBA8, as submitted),
 // weights, codes, errors, stats.
 struct Layout {
@@ -73,40 +85,107 @@ Layout layout_of(const icbc_bc1_job& j) {
     return l;
 }
 
-// Size-classed device blocks (powers of two from 64 KiB), reused.  Blocks
-// are carved from large slabs so that steady-state serving never calls the
-// allocator.
+// Size-classed device blocks (powers of two from 64 KiB), reused, carved from
+// slabs.  Every byte the pool commits (slabs, and raw allocations such as the
+// slots' workspaces) is charged against the budget: a new slab is made only
+// while the committed total stays within it, first releasing slabs whose
+// blocks are all free.  Not thread-safe: the server calls it under its lock.
+// The device allocator hands out memory in 2 MiB granules: every allocation
+// is a whole number of them, so that the bytes charged are the bytes held.
+constexpr size_t kGranule = 2u << 20;
+
 class Pool {
 public:
+    explicit Pool(uint64_t budget) : budget_(budget) {}
     ~Pool() {
-        for (void* p : slabs_) (void)hipFree(p);
+        for (auto& kv : slabs_) (void)hipFree(kv.second.base);
+        for (auto& r : raw_) (void)hipFree(r);
     }
     static size_t class_of(size_t bytes) {
         size_t c = 64 << 10;
         while (c < bytes) c <<= 1;
         return c;
     }
+    uint64_t committed() const { return committed_; }
+    // Bytes held outside the pool's own allocations (the runtime's memory
+    // behind a stream), charged for the pool's lifetime.
+    bool charge(size_t bytes) {
+        if (committed_ + bytes > budget_) return false;
+        committed_ += bytes;
+        return true;
+    }
+    // Device memory outside the classes (held until the pool is destroyed).
+    hipError_t alloc_raw(size_t bytes, void** p) {
+        bytes = align_up(bytes, kGranule);
+        if (committed_ + bytes > budget_) return hipErrorOutOfMemory;
+        hipError_t e = hipMalloc(p, bytes);
+        if (e != hipSuccess) return e;
+        raw_.push_back(*p);
+        committed_ += bytes;
+        return hipSuccess;
+    }
+    // A block of class `cls`; hipErrorOutOfMemory when the budget has no room
+    // for it now (blocks given back later may make room).
     hipError_t take(size_t cls, void** p) {
         auto& v = free_[cls];
         if (!v.empty()) {
             *p = v.back();
             v.pop_back();
+            slab_of(*p).used++;
             return hipSuccess;
         }
-        const size_t slab = std::max<size_t>(cls, 64u << 20);
+        // slabs of up to 64 MiB (a sixteenth of a small budget), a whole
+        // number of blocks and of granules (classes are powers of two)
+        const size_t unit = std::max(cls, kGranule);
+        size_t slab = std::max<size_t>(unit, std::min<uint64_t>(64u << 20, budget_ / 16) / unit * unit);
+        if (committed_ + slab > budget_) slab = unit;
+        if (committed_ + slab > budget_ && !release_empty(slab)) return hipErrorOutOfMemory;
         void* base = nullptr;
         hipError_t e = hipMalloc(&base, slab);
         if (e != hipSuccess) return e;
-        slabs_.push_back(base);
+        committed_ += slab;
+        slabs_[(uintptr_t)base] = Slab{(char*)base, slab, cls, 1};
         for (size_t off = cls; off + cls <= slab; off += cls) v.push_back((char*)base + off);
         *p = base;
         return hipSuccess;
     }
-    void give(size_t cls, void* p) { free_[cls].push_back(p); }
+    void give(size_t cls, void* p) {
+        free_[cls].push_back(p);
+        slab_of(p).used--;
+    }
 
 private:
+    struct Slab {
+        char* base;
+        size_t bytes, cls;
+        uint32_t used;  // blocks taken
+    };
+    Slab& slab_of(void* p) {
+        auto it = slabs_.upper_bound((uintptr_t)p);
+        return (--it)->second;
+    }
+    // Frees slabs with no block in use until `need` more bytes fit the budget.
+    bool release_empty(size_t need) {
+        for (auto it = slabs_.begin(); it != slabs_.end() && committed_ + need > budget_;) {
+            Slab& sl = it->second;
+            if (sl.used != 0) {
+                ++it;
+                continue;
+            }
+            auto& v = free_[sl.cls];
+            v.erase(std::remove_if(v.begin(), v.end(),
+                                   [&](void* q) { return (char*)q >= sl.base && (char*)q < sl.base + sl.bytes; }),
+                    v.end());
+            (void)hipFree(sl.base);
+            committed_ -= sl.bytes;
+            it = slabs_.erase(it);
+        }
+        return committed_ + need <= budget_;
+    }
+    uint64_t budget_, committed_ = 0;
+    std::map<uintptr_t, Slab> slabs_;
     std::map<size_t, std::vector<void*>> free_;
-    std::vector<void*> slabs_;
+    std::vector<void*> raw_;
 };
 
 // Page-locked host staging that grows on demand.
@@ -156,12 +235,16 @@ struct Request {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    void* work = nullptr;  // queue counters + workspace slices
-    void* refine = nullptr;  // level-9 handoff list, grown to the largest launch
-    size_t refine_bytes = 0;
-    void* table = nullptr;  // job table + tile list
-    size_t table_bytes = 0;
-    void* host_table = nullptr;  // pinned staging for the table
+    void* work = nullptr;  // queue counters + workspace slices (charged at creation)
+    // this launch's job table + tile list and level-9 handoff list: pool
+    // blocks taken with the batch, given back when it completes
+    void* table = nullptr;
+    size_t table_cls = 0;
+    void* refine = nullptr;
+    size_t refine_cls = 0;
+    std::vector<uint32_t> tile_blocks;  // per request of the batch
+    uint64_t ntiles = 0;
+    Staging host_table;  // pinned staging for the table
     Staging in;   // inputs of requests whose buffers are pageable
     Staging out;  // codes / errors / stats on their way back
     std::vector<size_t> out_off;  // per request: offset of its outputs in `out`
@@ -174,7 +257,6 @@ struct Slot {
 struct icbc_server {
     icbc_server_options opt{};
     int decoder = 0;
-    uint32_t grid = 0;
     std::mutex m;
     std::condition_variable cv_sched, cv_reap, cv_done;
     std::deque<Request*> pending;
@@ -183,7 +265,9 @@ struct icbc_server {
     std::vector<Slot> slots;
     std::unordered_map<uint64_t, Request*> all;
     Pool pool;
-    uint64_t used = 0;
+    uint32_t grid = 0;  // resident workgroups of a launch (its workspace size)
+    uint64_t used = 0;  // bytes of pool blocks taken (requests and launch tables)
+    bool hold_admission = false;  // a formed batch waits for its launch tables' room
     uint64_t next_id = 1;
     bool stop = false;
     int fatal = ICBC_OK;
@@ -191,8 +275,11 @@ struct icbc_server {
     // counters
     uint64_t launches = 0, completed = 0, blocks_done = 0;
 
+    explicit icbc_server(uint64_t budget) : pool(budget) {}
     void run_scheduler();
     void run_reaper();
+    bool launch_resources(Slot& s, const std::vector<Request*>& reqs);
+    void release_launch_resources(Slot& s);
     int launch(Slot& s, std::vector<Request*>& reqs);
 };
 
@@ -205,13 +292,74 @@ int set_fatal(icbc_server* sv, int code) {
 
 }  // namespace
 
-int icbc_server::launch(Slot& s, std::vector<Request*>& reqs) {
-    // device job table for the batch
-    std::vector<JobDev> dev;
-    std::vector<uint64_t> blocks;
-    uint64_t total = 0;
+// Device footprint of one launch besides its requests' buffers: the job table
+// and tile list, and the level-9 handoff list.
+struct LaunchNeeds {
+    std::vector<uint32_t> tile_blocks;
+    uint64_t tiles = 0;
+    size_t table_bytes = 0, refine_bytes = 0;
+};
+
+LaunchNeeds launch_needs(const std::vector<Request*>& reqs, uint32_t grid) {
+    LaunchNeeds n;
+    uint64_t total = 0, level9 = 0;
     for (Request* r : reqs) total += r->job.block_count;
     for (Request* r : reqs) {
+        const uint64_t b = r->job.block_count;
+        const uint32_t t = r->job.tile_blocks ? r->job.tile_blocks : auto_tile_blocks(b, total, grid);
+        n.tile_blocks.push_back(t);
+        n.tiles += (b + t - 1) / t;
+        if (r->job.level == 9) level9 += b;
+    }
+    n.table_bytes = align_up(reqs.size() * sizeof(JobDev), 256) + std::max<uint64_t>(1, n.tiles) * sizeof(uint2);
+    n.refine_bytes = level9 ? refine_list_bytes(level9) : 0;
+    return n;
+}
+
+// Takes this batch's table and handoff blocks from the pool (under the lock);
+// false, with nothing taken, when the budget has no room for them now.
+bool icbc_server::launch_resources(Slot& s, const std::vector<Request*>& reqs) {
+    LaunchNeeds n = launch_needs(reqs, grid);
+    const size_t tc = Pool::class_of(n.table_bytes);
+    void* t = nullptr;
+    if (pool.take(tc, &t) != hipSuccess) return false;
+    void* rf = nullptr;
+    size_t rc = 0;
+    if (n.refine_bytes) {
+        rc = Pool::class_of(n.refine_bytes);
+        if (pool.take(rc, &rf) != hipSuccess) {
+            pool.give(tc, t);
+            return false;
+        }
+    }
+    s.table = t;
+    s.table_cls = tc;
+    s.refine = rf;
+    s.refine_cls = rc;
+    s.tile_blocks = std::move(n.tile_blocks);
+    s.ntiles = n.tiles;
+    used += tc + rc;
+    return true;
+}
+
+void icbc_server::release_launch_resources(Slot& s) {
+    if (s.table) {
+        pool.give(s.table_cls, s.table);
+        used -= s.table_cls;
+    }
+    if (s.refine) {
+        pool.give(s.refine_cls, s.refine);
+        used -= s.refine_cls;
+    }
+    s.table = s.refine = nullptr;
+    s.table_cls = s.refine_cls = 0;
+}
+
+int icbc_server::launch(Slot& s, std::vector<Request*>& reqs) {
+    // device job table and tile list for the batch, in the slot's table block
+    std::vector<JobDev> dev;
+    for (size_t k = 0; k < reqs.size(); k++) {
+        Request* r = reqs[k];
         icbc_bc1_job dj = r->job;
         char* base = (char*)r->dev;
         dj.flags = r->job.flags & ICBC_JOB_COLORS_RGBA8;  // RGBA8 texels are read as they are
@@ -221,38 +369,20 @@ int icbc_server::launch(Slot& s, std::vector<Request*>& reqs) {
         dj.errors = r->job.errors ? (float*)(base + r->lay.errors) : nullptr;
         dj.stats = r->job.stats ? (int32_t*)(base + r->lay.stats) : nullptr;
         JobDev d = job_to_device(dj, decoder);
-        if (dj.tile_blocks == 0) d.tile_blocks = (int32_t)auto_tile_blocks(dj.block_count, total, grid);
+        d.tile_blocks = (int32_t)s.tile_blocks[k];
         dev.push_back(d);
     }
     std::vector<uint2> tiles = tiles_for(dev, true);
+    if (tiles.size() != s.ntiles) return ICBC_ERR_DEVICE;  // the table block was sized for s.ntiles
     const size_t jbytes = align_up(dev.size() * sizeof(JobDev), 256);
     const size_t need = jbytes + std::max<size_t>(1, tiles.size()) * sizeof(uint2);
-    if (need > s.table_bytes) {
-        if (s.table) (void)hipFree(s.table);
-        if (s.host_table) (void)hipHostFree(s.host_table);
-        s.table = nullptr;
-        s.host_table = nullptr;
-        s.table_bytes = 0;
-        const size_t cap = std::max<size_t>(need * 2, 64 << 10);
-        if (hipMalloc(&s.table, cap) != hipSuccess || hipHostMalloc(&s.host_table, cap, hipHostMallocDefault) != hipSuccess)
-            return ICBC_ERR_OUT_OF_MEMORY;
-        s.table_bytes = cap;
-    }
-    memcpy(s.host_table, dev.data(), dev.size() * sizeof(JobDev));
-    memcpy((char*)s.host_table + jbytes, tiles.data(), tiles.size() * sizeof(uint2));
-    if (hipMemcpyAsync(s.table, s.host_table, need, hipMemcpyHostToDevice, s.stream) != hipSuccess)
+    if (!s.host_table.reserve(need)) return ICBC_ERR_OUT_OF_MEMORY;
+    memcpy(s.host_table.p, dev.data(), dev.size() * sizeof(JobDev));
+    memcpy((char*)s.host_table.p + jbytes, tiles.data(), tiles.size() * sizeof(uint2));
+    if (hipMemcpyAsync(s.table, s.host_table.p, need, hipMemcpyHostToDevice, s.stream) != hipSuccess)
         return ICBC_ERR_DEVICE;
-    const size_t rbytes = refine_list_bytes(level9_blocks(dev.data(), dev.size()));
-    if (rbytes > s.refine_bytes) {
-        // the slot's stream has finished its previous launch (the slot was free)
-        if (s.refine) (void)hipFree(s.refine);
-        s.refine = nullptr;
-        s.refine_bytes = 0;
-        const size_t cap = std::max<size_t>(rbytes * 2, 64 << 10);
-        if (hipMalloc(&s.refine, cap) != hipSuccess) return ICBC_ERR_OUT_OF_MEMORY;
-        s.refine_bytes = cap;
-    }
-    const RefineList rl{(RefineItem*)((char*)s.refine + kRefineCounterBytes), (uint32_t*)s.refine};
+    const RefineList rl = s.refine ? RefineList{(RefineItem*)((char*)s.refine + kRefineCounterBytes), (uint32_t*)s.refine}
+                                   : RefineList{nullptr, nullptr};
     if (launch_encode((const JobDev*)s.table, (const uint2*)((char*)s.table + jbytes), (uint32_t)tiles.size(),
                       (uint32_t*)s.work, (float4*)((char*)s.work + kQueueBytes), grid, encode_mode(dev.data(), dev.size()),
                       rl, s.stream) != hipSuccess)
@@ -313,15 +443,17 @@ void icbc_server::run_scheduler() {
             pending.clear();
             cv_done.notify_all();
         }
-        // admit in arrival order while the budget allows
-        while (!pending.empty()) {
+        // admit in arrival order while the budget allows (the pool refuses
+        // blocks the budget has no room for); not while a batch waits for the
+        // room its launch tables need, which admissions would take again
+        while (!pending.empty() && !hold_admission) {
             Request* r = pending.front();
-            if (used + r->cls > opt.memory_budget) break;
             // the pool is only touched under the lock (the reaper returns blocks)
-            if (pool.take(r->cls, &r->dev) != hipSuccess) {
+            const hipError_t te = pool.take(r->cls, &r->dev);
+            if (te != hipSuccess) {
                 if (!admitted.empty() || !inflight.empty()) break;  // retry once memory frees up
                 pending.pop_front();
-                r->status = ICBC_ERR_OUT_OF_MEMORY;
+                r->status = te == hipErrorOutOfMemory ? ICBC_ERR_BUDGET : ICBC_ERR_DEVICE;
                 r->state = kDone;
                 r->t_done = now_s();
                 cv_done.notify_all();
@@ -341,9 +473,55 @@ void icbc_server::run_scheduler() {
             }
         if (!admitted.empty() && free_slot && fatal == ICBC_OK) {
             std::vector<Request*> batch;
-            const size_t take = opt.max_batch ? std::min<size_t>(opt.max_batch, admitted.size()) : admitted.size();
-            batch.assign(admitted.begin(), admitted.begin() + take);
+            size_t take = opt.max_batch ? std::min<size_t>(opt.max_batch, admitted.size()) : admitted.size();
+            // the launch's table and handoff list come from the budget too: a
+            // batch they do not fit in is halved until they do
+            bool ok = false;
+            for (;;) {
+                batch.assign(admitted.begin(), admitted.begin() + take);
+                if (launch_resources(*free_slot, batch)) {
+                    ok = true;
+                    break;
+                }
+                if (take == 1) break;
+                take = (take + 1) / 2;
+            }
+            if (!ok) {
+                hold_admission = true;
+                if (inflight.empty() && admitted.size() > 1) {
+                    // the other admitted requests hold the room this launch
+                    // needs: they go back to the queue (in order) and wait
+                    for (size_t k = admitted.size(); k-- > 1;) {
+                        Request* r = admitted[k];
+                        pool.give(r->cls, r->dev);
+                        r->dev = nullptr;
+                        used -= r->cls;
+                        r->state = kPending;
+                        pending.push_front(r);
+                    }
+                    admitted.resize(1);
+                    continue;
+                }
+                if (inflight.empty()) {
+                    // nothing else holds memory: this request can never run
+                    // within the budget
+                    hold_admission = false;
+                    Request* r = admitted.front();
+                    admitted.erase(admitted.begin());
+                    pool.give(r->cls, r->dev);
+                    r->dev = nullptr;
+                    used -= r->cls;
+                    r->status = ICBC_ERR_BUDGET;
+                    r->state = kDone;
+                    r->t_done = now_s();
+                    cv_done.notify_all();
+                    continue;
+                }
+                cv_sched.wait(lk);  // a completing launch gives memory back
+                continue;
+            }
             admitted.erase(admitted.begin(), admitted.begin() + take);
+            hold_admission = false;
             free_slot->busy = true;
             free_slot->reqs = batch;
             lk.unlock();
@@ -446,6 +624,7 @@ void icbc_server::run_reaper() {
             blocks_done += r->job.block_count;
         }
         if (e != hipSuccess) set_fatal(this, ICBC_ERR_DEVICE);
+        release_launch_resources(*s);
         s->reqs.clear();
         s->busy = false;
         cv_done.notify_all();
@@ -466,17 +645,32 @@ int icbc_server_create(const icbc_server_options* opt, icbc_server** out) {
     uint32_t grid;
     r = icbc_internal_resident(&grid);
     if (r) return r;
-    icbc_server* sv = new icbc_server();
-    sv->opt = *opt;
-    if (sv->opt.max_inflight == 0) sv->opt.max_inflight = 2;
-    if (sv->opt.max_inflight > 8) sv->opt.max_inflight = 8;
+    icbc_server_options o = *opt;
+    if (o.max_inflight == 0) o.max_inflight = 3;
+    if (o.max_inflight > 8) o.max_inflight = 8;
+    // the launch slots (stream allowance and workspace) take at most half
+    // the budget: as many resident workgroups as that allows, up to the
+    // device's residency, and no fewer than kMinServerGrid
+    const size_t per_wg = kWorkspaceBytesPerWorkgroup;
+    const uint64_t share = o.memory_budget / 2 / o.max_inflight / kGranule * kGranule;  // per slot
+    const uint64_t fit = share > kQueueBytes + kStreamAllowance ? (share - kQueueBytes - kStreamAllowance) / per_wg : 0;
+    if (fit < kMinServerGrid)
+        return icbc_internal_fail(ICBC_ERR_BUDGET, "memory budget %llu B is below the %llu B that %u launch slots "
+                                  "need", (unsigned long long)o.memory_budget,
+                                  (unsigned long long)(2ull * o.max_inflight *
+                                                       (kStreamAllowance +
+                                                        align_up(kQueueBytes + kMinServerGrid * per_wg, kGranule))),
+                                  o.max_inflight);
+    icbc_server* sv = new icbc_server(o.memory_budget);
+    sv->opt = o;
     sv->decoder = decoder;
-    sv->grid = grid;
+    sv->grid = (uint32_t)std::min<uint64_t>(grid, fit);
     sv->slots.resize(sv->opt.max_inflight);
     for (Slot& s : sv->slots) {
         if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
-            hipMalloc(&s.work, kQueueBytes + (size_t)grid * kWorkspaceBytesPerWorkgroup) != hipSuccess) {
+            !sv->pool.charge(kStreamAllowance) ||
+            sv->pool.alloc_raw(kQueueBytes + (size_t)sv->grid * per_wg, &s.work) != hipSuccess) {
             icbc_server_destroy(sv);
             return icbc_internal_fail(ICBC_ERR_OUT_OF_MEMORY, "server launch slot allocation failed");
         }
@@ -507,10 +701,21 @@ int icbc_server_submit(icbc_server* sv, const icbc_bc1_job* job, uint64_t* ticke
         delete q;
         return icbc_internal_fail(ICBC_ERR_CANCELLED, "server is shutting down");
     }
-    if (q->cls > sv->opt.memory_budget) {
-        delete q;
-        return icbc_internal_fail(ICBC_ERR_BUDGET, "request needs %zu bytes, budget is %llu", q->cls,
-                                  (unsigned long long)sv->opt.memory_budget);
+    {
+        // alone in a launch: its buffers, its table and tile list, its handoff
+        // list, next to the slots' workspaces
+        std::vector<Request*> one{q};
+        const LaunchNeeds n = launch_needs(one, sv->grid);
+        const uint64_t need = q->cls + Pool::class_of(n.table_bytes) +
+                              (n.refine_bytes ? Pool::class_of(n.refine_bytes) : 0);
+        const uint64_t room = sv->opt.memory_budget -
+                              sv->slots.size() * (kStreamAllowance +
+                                                  align_up(kQueueBytes + (uint64_t)sv->grid * kWorkspaceBytesPerWorkgroup, kGranule));
+        if (need > room) {
+            delete q;
+            return icbc_internal_fail(ICBC_ERR_BUDGET, "request needs %llu bytes, the budget leaves %llu for requests",
+                                      (unsigned long long)need, (unsigned long long)room);
+        }
     }
     q->id = sv->next_id++;
     sv->all[q->id] = q;
@@ -578,6 +783,7 @@ int icbc_server_stats(icbc_server* sv, icbc_server_stats_t* st) {
     st->pending = sv->pending.size();
     st->inflight = sv->inflight.size();
     st->memory_in_use = sv->used;
+    st->memory_committed = sv->pool.committed();
     return ICBC_OK;
 }
 
@@ -595,10 +801,6 @@ int icbc_server_destroy(icbc_server* sv) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
         if (s.done) (void)hipEventDestroy(s.done);
         if (s.stream) (void)hipStreamDestroy(s.stream);
-        if (s.work) (void)hipFree(s.work);
-        if (s.refine) (void)hipFree(s.refine);
-        if (s.table) (void)hipFree(s.table);
-        if (s.host_table) (void)hipHostFree(s.host_table);
     }
     for (auto& kv : sv->all) delete kv.second;
     delete sv;

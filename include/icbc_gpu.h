@@ -108,8 +108,8 @@ int icbc_gpu_compress_bc1_plain(const icbc_bc1_job* jobs, size_t n_jobs, void* s
  * allocates nothing -- no device, pinned or host heap memory.  Everything a
  * launch needs besides the caller's inputs and outputs (job table, tile list,
  * tile-queue counters, level-9 handoff list, workgroup workspaces) lives in a
- * fixed scratch array of ICBC_NOALLOC_SCRATCH_BYTES that the library holds on
- * every device it runs on until the process ends.  Limits per call:
+ * fixed scratch array (icbc_gpu_noalloc_scratch_bytes(), 56.5 MiB) that the
+ * library holds on every device it runs on until the process ends.  Limits per call:
  *   - device-memory block jobs only (no ICBC_JOB_HOST_MEMORY, no ICBC_JOB_IMAGE);
  *   - at most ICBC_NOALLOC_MAX_JOBS jobs, ICBC_NOALLOC_MAX_TILES tiles and
  *     ICBC_NOALLOC_MAX_LEVEL9_BLOCKS level-9 blocks (larger calls get
@@ -165,8 +165,11 @@ uint64_t icbc_gpu_plan_tiles(const icbc_bc1_plan* plan);
 typedef struct icbc_server icbc_server;
 
 typedef struct icbc_server_options {
-    uint64_t memory_budget; /* device bytes for resident request buffers */
-    uint32_t max_inflight;  /* launches overlapping on the device (0 = 2, at most 8) */
+    uint64_t memory_budget; /* every device byte the server holds: request buffers, launch
+                               tables and level-9 lists, the launch slots' workspaces and an
+                               allowance for each slot's stream (the slots take at most half
+                               of it; at least 20 MiB per slot) */
+    uint32_t max_inflight;  /* launches overlapping on the device (0 = 3, at most 8) */
     uint32_t max_batch;     /* requests merged into one launch (0 = no limit) */
 } icbc_server_options;
 
@@ -178,7 +181,9 @@ typedef struct icbc_request_info {
 } icbc_request_info;
 
 typedef struct icbc_server_stats_t {
-    uint64_t launches, completed, blocks, pending, inflight, memory_in_use;
+    uint64_t launches, completed, blocks, pending, inflight;
+    uint64_t memory_in_use;    /* bytes of pool blocks held by requests and launches */
+    uint64_t memory_committed; /* device bytes the server holds in all (<= memory_budget) */
 } icbc_server_stats_t;
 
 int icbc_server_create(const icbc_server_options* opt, icbc_server** server);

@@ -1,7 +1,9 @@
 """GPU tests of the request server (continuous batching under a memory
 budget): every request's result is the same alone and in the mix (and equal
-to the oracle), admission respects the budget, oversize and cancelled requests
-get defined statuses, and launches overlap."""
+to the oracle), the device memory the server holds stays within its budget
+(measured with hipMemGetInfo), admission is FIFO, oversize and cancelled
+requests get defined statuses, and launches overlap."""
+import ctypes
 import threading
 import time
 
@@ -53,32 +55,82 @@ def test_results_same_alone_and_in_mix(gpu, oracle):
     assert st["completed"] == len(reqs)
 
 
-def test_budget_admission(gpu):
-    reqs = _requests(2, 60, max_side=512)
-    budget = 32 << 20
-    srv = gpu.Server(memory_budget=budget, max_inflight=2)
-    peak = [0]
+def _device_free_bytes():
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    return free.value
+
+
+# the device's free-memory count moves in 2 MiB granules
+SLACK = 2 << 20
+
+
+def _warm_runtime(gpu):
+    """Process-wide device memory the runtime sets up on first use -- the
+    library's code object and each hardware queue's scratch for the encoder's
+    register spills -- is taken before measuring: every kernel instance runs
+    once on each of several streams (streams share the device's few
+    hardware queues)."""
+    colors, _ = synth.make_image_blocks(11, 256, 64, synth.MIX_DEFAULT)
+    w = np.random.default_rng(11).random((len(colors), 16)).astype(np.float32)
+    streams = [gpu.Stream() for _ in range(8)]
+    try:
+        for s in streams:
+            for lv, wt in ((1, None), (1, w), (8, None), (9, w)):
+                codes = np.zeros((len(colors), 8), np.uint8)
+                gpu.compress_batch([gpu.make_job(lv, len(colors), colors, codes, wt, None, None, (1, 1, 1), True, True,
+                                                 host=True)], stream=s)
+            s.synchronize()
+    finally:
+        for s in streams:
+            s.close()
+
+
+@pytest.mark.parametrize("budget,inflight", [(48 << 20, 1), (64 << 20, 2), (96 << 20, 3)])
+def test_device_memory_within_budget(gpu, budget, inflight):
+    # the budget bounds every device byte the server holds -- request buffers,
+    # launch tables, level-9 lists and slot workspaces -- checked against the
+    # device's own free-memory count while requests (more than the budget
+    # leaves for them, levels 0-9, some weighted) flow through
+    reqs = _requests(2, 300, max_side=512)
+    need = sum(len(b) * (256 + 8 + 4) for _, b, _, _ in reqs)
+    assert need > budget // 2   # more than the room the launch slots leave for requests
+    _warm_runtime(gpu)
+    free0 = _device_free_bytes()
+    srv = gpu.Server(memory_budget=budget, max_inflight=inflight)
+    peak = {"committed": 0, "in_use": 0, "device": 0}
     stop = threading.Event()
 
     def watch():
         while not stop.is_set():
-            peak[0] = max(peak[0], srv.stats()["memory_in_use"])
+            st = srv.stats()
+            peak["committed"] = max(peak["committed"], st["memory_committed"])
+            peak["in_use"] = max(peak["in_use"], st["memory_in_use"])
+            peak["device"] = max(peak["device"], free0 - _device_free_bytes())
             time.sleep(0.0005)
 
     th = threading.Thread(target=watch)
     th.start()
     try:
         tickets = [srv.submit(lv, b, w, (1, 1, 1), tcm, tcm) for lv, b, w, tcm in reqs]
-        infos = [srv.wait(t)[2] for t in tickets]
+        results = [srv.wait(t) for t in tickets]
+        peak["device"] = max(peak["device"], free0 - _device_free_bytes())
     finally:
         stop.set()
         th.join()
         srv.close()
-    assert 0 < peak[0] <= budget
+    print(f"budget {budget}: peak {peak}", flush=True)
+    assert 0 < peak["in_use"] <= peak["committed"] <= budget, peak
+    assert peak["device"] <= budget + SLACK, peak
+    infos = [r[2] for r in results]
     # FIFO admission: admit times follow submit order
     admits = [i.admit_s for i in infos]
     assert all(a <= b for a, b in zip(admits, admits[1:]))
     assert all(i.status == 0 for i in infos)
+    for (lv, b, w, tcm), (codes, errs, _) in zip(reqs[:12], results[:12]):
+        alone = gpu.compress_many(lv, b, w, (1, 1, 1), tcm, tcm)
+        assert np.array_equal(codes, alone[0])
 
 
 def test_launches_overlap(gpu):
@@ -104,9 +156,13 @@ def test_launches_overlap(gpu):
 
 
 def test_oversize_request_rejected(gpu):
-    srv = gpu.Server(memory_budget=1 << 20)
+    # a budget below the launch slots' minimum workspace is refused at creation
+    with pytest.raises(gpu.IcbcError) as e:
+        gpu.Server(memory_budget=1 << 20)
+    assert e.value.code == gpu.ERR_BUDGET
+    srv = gpu.Server(memory_budget=24 << 20, max_inflight=1)
     try:
-        big = np.zeros((8192, 16, 4), np.float32)   # 2 MiB of texels
+        big = np.zeros((131072, 16, 4), np.float32)   # 32 MiB of texels
         with pytest.raises(gpu.IcbcError) as e:
             srv.submit(8, big)
         assert e.value.code == gpu.ERR_BUDGET
@@ -117,7 +173,7 @@ def test_oversize_request_rejected(gpu):
 def test_cancel_pending_request(gpu):
     # a budget that holds one request at a time keeps the later ones pending
     blocks, _ = synth.make_image_blocks(3, 1024, 1024, synth.MIX_DEFAULT)   # 65536 blocks, ~17 MiB
-    srv = gpu.Server(memory_budget=32 << 20, max_inflight=1)
+    srv = gpu.Server(memory_budget=80 << 20, max_inflight=1)
     try:
         t1 = srv.submit(9, blocks, None, (1, 1, 1), True, True)
         t2 = srv.submit(9, blocks, None, (1, 1, 1), True, True)
@@ -135,7 +191,7 @@ def test_cancel_pending_request(gpu):
 
 def test_invalid_level_request(gpu):
     blocks, _ = synth.make_image_blocks(4, 32, 32, synth.MIX_DEFAULT)
-    srv = gpu.Server(memory_budget=16 << 20)
+    srv = gpu.Server(memory_budget=24 << 20, max_inflight=1)
     try:
         prior = np.full((len(blocks), 8), 0xab, np.uint8)
         t = srv.submit(11, blocks, codes=prior)
@@ -149,7 +205,7 @@ def test_invalid_level_request(gpu):
 
 def test_concurrent_clients(gpu, oracle):
     reqs = _requests(5, 24)
-    srv = gpu.Server(memory_budget=32 << 20, max_inflight=2)
+    srv = gpu.Server(memory_budget=64 << 20, max_inflight=2)
     results = {}
 
     def client(k):
@@ -177,7 +233,7 @@ def test_identity_under_load(gpu):
     pinned = [gpu.pinned_copy(b) for _, b, _, _ in reqs]
     alone = [gpu.compress_many(lv, p.array, w, (1, 1, 1), tcm, tcm)[0]
              for (lv, _, w, tcm), p in zip(reqs, pinned)]
-    srv = gpu.Server(memory_budget=24 << 20, max_inflight=3)
+    srv = gpu.Server(memory_budget=96 << 20, max_inflight=3)
     rng = np.random.default_rng(9)
     try:
         order = rng.integers(0, len(reqs), size=400)
