@@ -357,6 +357,42 @@ def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64
                      + " in page-locked host memory, copied per request"}
 
 
+def drop_in_per_block(g, synth, thread_counts, level=8, repeats=3, parity=None):
+    """The reference's own call pattern through the drop-in C++ API: the
+    harness's loop (icbc_test.cpp:389-413) compiled against libicbc_amd.so
+    (icbc_amd/drop_in_bench), worker threads taking chunks of 32 blocks and
+    calling icbc::compress_bc1 once per block, over the 512x512 plain-content
+    image (16,384 blocks) at level 8 with three-colour mode and transparent
+    black on.  Concurrent calls share launches (the single-block ring)."""
+    import subprocess
+    import tempfile
+    from icbc_amd import build as hip_build
+    colors, _ = synth.make_image_blocks(1, 512, 512, synth.MIX_PLAIN)
+    n = len(colors)
+    runs = []
+    with tempfile.TemporaryDirectory() as tmp:
+        src = os.path.join(tmp, "blocks.f32")
+        np.ascontiguousarray(colors, np.float32).tofile(src)
+        for t in thread_counts:
+            out = os.path.join(tmp, f"codes{t}.bin")
+            res = subprocess.run([hip_build.BENCH_BIN, src, str(n), str(t), str(level), str(repeats), out],
+                                 capture_output=True, text=True, timeout=300)
+            if res.returncode != 0:
+                raise RuntimeError(f"drop_in_bench failed: {res.stderr}")
+            r = json.loads(res.stdout.strip().splitlines()[-1])
+            codes = np.fromfile(out, np.uint8).reshape(n, 8)
+            if parity is not None:
+                want = g.compress_many(level, colors, None, (1, 1, 1), True, True)
+                r["parity"] = parity.check(f"drop-in per-block calls, {t} threads", level, colors, None, (1, 1, 1),
+                                           1, 1, codes, want[1], sample=4096, seed=400 + t)
+                r["codes_identical_to_batch_path"] = bool(np.array_equal(codes, want[0]))
+            runs.append(r)
+            log(f"drop-in per-block, {t} threads: {r}")
+    return {"workload": "512x512 plain-content image (16,384 blocks), level 8, icbc::compress_bc1 per block "
+                        "from worker threads (chunks of 32), as the reference's harness calls it",
+            "runs": runs}
+
+
 def single_image_levels(g, synth, levels=(1, 5, 9), steps=10, seed=7, parity=None):
     """The 4096x4096 single-image configuration: one plan per level over its
     1,048,576 blocks (40/30/20/10 gradient / white noise / near-black / flat,
@@ -530,6 +566,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the short serving run included in the default benchmark")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-drop-in", action="store_true",
+                    help="skip the per-block icbc::compress_bc1 run from worker threads")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-level (1-9) batched-vs-plain run on the same textures")
     ap.add_argument("--no-levels", action="store_true",
@@ -731,6 +769,11 @@ def main():
                                   codes_of=lambda: (dcode.download(np.uint8, (total, 8)),
                                                     derr.download(np.float32, total)))
 
+    drop_in = None
+    if not args.no_drop_in:
+        t = host_threads()
+        drop_in = drop_in_per_block(g, synth, (t, 8 * t), parity=parity)
+
     serving = None
     if not args.no_serve:
         serving = {"rgba8": serving_bench(g, synth, args.serve_seconds, (0.8,), parity=parity),
@@ -756,6 +799,7 @@ def main():
         "quality": batch_quality,
         "cpu_baseline": cpu,
         "serving": serving,
+        "drop_in_per_block": drop_in,
         "single_image_levels": levels_run,
         "weighted_level9": weighted,
         "mixed_level_batch": mixed,

@@ -29,7 +29,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libicbc_amd.so")
-SOURCES = ["bc1_encoder.hip", "texture_prepass.hip", "icbc_runtime.cpp", "icbc_server.cpp", "containers.cpp", "autotune.cpp"]
+SOURCES = ["bc1_encoder.hip", "texture_prepass.hip", "icbc_runtime.cpp", "icbc_server.cpp", "containers.cpp",
+           "autotune.cpp", "block_ring.cpp"]
 ARCH = "gfx950"
 
 HIPCC_FLAGS = [
@@ -48,6 +49,21 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+# The drop-in per-block benchmark program (a C++ caller of the icbc:: API,
+# run by bench.py).
+BENCH_SRC = os.path.join(CSRC, "drop_in_bench.cpp")
+BENCH_BIN = os.path.join(HERE, "drop_in_bench")
+
+
+def build_drop_in_bench(verbose=False):
+    cmd = ["g++", "-std=c++17", "-O2", "-pthread", "-I", os.path.join(ROOT, "include"), BENCH_SRC,
+           "-L", HERE, "-licbc_amd", "-Wl,-rpath,$ORIGIN", "-o", BENCH_BIN]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    return BENCH_BIN
+
+
 def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
@@ -57,6 +73,7 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    build_drop_in_bench(verbose)
     return LIB
 
 

@@ -17,6 +17,10 @@ int icbc_internal_ready(int* decoder);
 int icbc_internal_resident(uint32_t* grid);
 // Argument checks shared by every entry point taking a job.
 int icbc_internal_validate(const icbc_bc1_job& job);
+// One block through the coalescing ring of single-block calls (block_ring.cpp):
+// the reference's compress_bc1 semantics, a status code out.
+int icbc_internal_compress_one(int level, const float* colors, const float* weights, const float cw[3], int tcm,
+                               int tcb, void* output, float* error);
 // The current device's tuning (false when no level is tuned) / replace it.
 bool icbc_internal_get_tuning(icbc_tuning* out);
 void icbc_internal_set_tuning(const icbc_tuning* t);

@@ -1094,12 +1094,10 @@ uint64_t icbc_gpu_plan_tiles(const icbc_bc1_plan* pl) { return pl ? pl->ntiles :
 
 namespace {
 
-// Device buffer of a single-block context: colors 256 | weights 64 | code 8 |
-// error 4 (at 336) | job 128 (at 384) | workspace and level-9 handoff (at 768);
-// decode / evaluate use its first 128 bytes.  Host mirror: the first 512 bytes.
-constexpr size_t kBlockDeviceBytes = 768 + kWorkspaceBytesPerWorkgroup + 4096;
-constexpr size_t kBlockHostBytes = 512;
-static_assert(refine_list_bytes(1) <= 4096, "level-9 handoff of one block must fit");
+// Buffers of a thread's single-block decode / evaluate calls: a BC1 or BC3
+// block (16 B) and 64 B of RGBA8 texels in, 64 B of RGBA8 or a float out.
+constexpr size_t kBlockDeviceBytes = 256;
+constexpr size_t kBlockHostBytes = 256;
 
 struct HipBlockApi {
     int stream_create(void** s) {
@@ -1119,9 +1117,10 @@ struct HipBlockApi {
     void host_free(void* p) { (void)hipHostFree(p); }
 };
 
-// The reference's compress_bc1 is reentrant; each calling thread gets its own
-// stream and buffers per device (never released: process teardown may already
-// have destroyed the runtime when thread-local destructors run).
+// decode_bc1 / evaluate_bc1_error are reentrant in the reference; each calling
+// thread gets its own stream and buffers per device (never released: process
+// teardown may already have destroyed the runtime when thread-local
+// destructors run).
 thread_local std::vector<BlockResources> t_block;
 
 int block_context(BlockResources*& ctx) {
@@ -1131,48 +1130,13 @@ int block_context(BlockResources*& ctx) {
     return block_resources_for(api, t_block, dev, ctx);
 }
 
-// One block through the same kernel as the batch path; status code out.
+// One block: the coalescing ring (block_ring.cpp) shares launches between
+// concurrent callers.
 int compress_one(int level, const float* input_colors, const float* input_weights,
                  const float color_weights[3], int three_color_mode, int three_color_black,
                  void* output, float* error) {
-    if (!input_colors || !input_weights || !color_weights || !output)
-        return fail(ICBC_ERR_INVALID_ARGUMENT, "null argument");
-    int decoder;
-    int r = ensure_device_ready(&decoder);
-    if (r) return r;
-    BlockResources* ctx;
-    r = block_context(ctx);
-    if (r) return r;
-    const hipStream_t stream = (hipStream_t)ctx->stream;
-    char* h = (char*)ctx->hbuf;
-    char* d = (char*)ctx->dbuf;
-    memcpy(h, input_colors, 256);
-    memcpy(h + 256, input_weights, 64);
-    memcpy(h + 320, output, 8);
-    icbc_bc1_job j;
-    memset(&j, 0, sizeof j);
-    j.level = level;
-    j.three_color_mode = three_color_mode;
-    j.three_color_black = three_color_black;
-    j.color_weights[0] = color_weights[0];
-    j.color_weights[1] = color_weights[1];
-    j.color_weights[2] = color_weights[2];
-    j.block_count = 1;
-    j.colors = (const float*)d;
-    j.weights = (const float*)(d + 256);
-    j.codes = d + 320;
-    j.errors = (float*)(d + 336);
-    JobDev jd = to_dev(j, decoder);
-    static_assert(sizeof(JobDev) <= 128, "job descriptor must fit the block buffer");
-    memcpy(h + 384, &jd, sizeof jd);
-    HIP_TRY(hipMemcpyAsync(d, h, 512, hipMemcpyHostToDevice, stream));
-    HIP_TRY(launch_encode((const JobDev*)(d + 384), nullptr, 1, nullptr, (float4*)(d + 768), 1, encode_mode(&jd, 1),
-                          refine_list_at(d + 768 + kWorkspaceBytesPerWorkgroup), stream));
-    HIP_TRY(hipMemcpyAsync(h + 320, d + 320, 32, hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipStreamSynchronize(stream));
-    memcpy(output, h + 320, 8);
-    memcpy(error, h + 336, 4);
-    return ICBC_OK;
+    return icbc_internal_compress_one(level, input_colors, input_weights, color_weights, three_color_mode,
+                                      three_color_black, output, error);
 }
 
 // decode of one BC1 (8 B) or BC3 (16 B) block into 64 bytes of RGBA8, on the

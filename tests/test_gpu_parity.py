@@ -174,6 +174,59 @@ def test_single_block_c_abi(gpu, oracle):
             assert np.array_equal(out2, want_c) and (np.float32(e2) == np.float32(want_e) or np.isnan(e2))
 
 
+def test_single_block_calls_from_many_threads(gpu, oracle):
+    # icbc_gpu_compress_bc1 called per block from 24 threads at once, each
+    # thread with its own level and options, so that concurrent calls share
+    # launches and runs of different options meet in one launch
+    import threading
+    colors, weights = synth.make_edge_blocks(61, 24 * 40)
+    prior = np.random.default_rng(62).integers(0, 256, size=(len(colors), 8), dtype=np.uint8)
+    cws = [(1, 1, 1), (0.3, 0.59, 0.11)]
+    results = {}
+
+    fn = gpu.lib().icbc_gpu_compress_bc1  # the C ABI (ctypes releases the GIL during the call)
+
+    def worker(t):
+        level, tcm, cw = t % 11 - 1, t % 3, np.array(cws[t % 2], np.float32)
+        for b in range(t * 40, t * 40 + 40):
+            out = prior[b].copy()
+            c = np.ascontiguousarray(colors[b].reshape(64))
+            w = np.ascontiguousarray(weights[b])
+            e = fn(level, c.ctypes.data, w.ctypes.data, cw.ctypes.data, int(tcm > 0), int(tcm > 1), out.ctypes.data)
+            results[b] = (out, e)
+
+    ths = [threading.Thread(target=worker, args=(t,)) for t in range(24)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    for t in range(24):
+        level, tcm, cw = t % 11 - 1, t % 3, cws[t % 2]
+        sl = slice(t * 40, t * 40 + 40)
+        want_c, want_e, _ = _oracle_run(oracle, level, colors[sl], weights[sl], cw, tcm > 0, tcm > 1, prior[sl])
+        got_c = np.array([results[b][0] for b in range(t * 40, t * 40 + 40)])
+        got_e = np.array([results[b][1] for b in range(t * 40, t * 40 + 40)], np.float32)
+        assert_same(got_c, got_e, want_c, want_e, f"thread {t} level {level}")
+
+
+def test_drop_in_bench_program(gpu, oracle, tmp_path):
+    # the C++ harness-style caller of icbc::compress_bc1 (8 threads)
+    import json
+    import subprocess
+    from icbc_amd import build as hip_build
+    colors, _ = synth.make_image_blocks(1, 128, 128, synth.MIX_PLAIN)
+    src = tmp_path / "b.f32"
+    np.ascontiguousarray(colors, np.float32).tofile(src)
+    out = tmp_path / "codes.bin"
+    res = subprocess.run([hip_build.BENCH_BIN, str(src), str(len(colors)), "8", "8", "1", str(out)],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    r = json.loads(res.stdout.strip().splitlines()[-1])
+    assert r["blocks"] == len(colors) and r["mblocks_per_s_best"] > 0
+    want = _oracle_run(oracle, 8, colors, None, (1, 1, 1), 1, 1)
+    assert np.array_equal(np.fromfile(out, np.uint8).reshape(-1, 8), want[0])
+
+
 def test_cpp_namespace_api_against_golden(gpu, golden):
     # the exported icbc:: functions themselves (include/icbc.h), called by
     # their C++ symbols as a program linked against the library would
