@@ -110,13 +110,23 @@ class ServerStats(C.Structure):
 class Tuning(C.Structure):
     """icbc_tuning (include/icbc_gpu.h): per-level launch settings."""
     _fields_ = [("tile_blocks", C.c_uint32 * 10), ("workgroups_per_cu", C.c_int32 * 10),
-                ("schedule", C.c_int32 * 10), ("ns_per_block", C.c_float * 10)]
+                ("schedule", C.c_int32 * 10), ("ns_per_block", C.c_float * 10),
+                ("prune_min_colors", C.c_int32 * 10), ("default_ns_per_block", C.c_float * 10)]
 
     def as_dict(self):
-        return {lv: {"tile_blocks": self.tile_blocks[lv], "workgroups_per_cu": self.workgroups_per_cu[lv],
-                     "schedule": "persistent" if self.schedule[lv] == 1 else "grid",
-                     "ns_per_block": round(float(self.ns_per_block[lv]), 4)}
-                for lv in range(10) if self.tile_blocks[lv]}
+        out = {}
+        for lv in range(10):
+            if not self.tile_blocks[lv]:
+                continue
+            d = {"tile_blocks": self.tile_blocks[lv], "workgroups_per_cu": self.workgroups_per_cu[lv],
+                 "schedule": "persistent" if self.schedule[lv] == 1 else "grid",
+                 "prune_min_colors": self.prune_min_colors[lv] or 13,
+                 "ns_per_block": round(float(self.ns_per_block[lv]), 4),
+                 "default_ns_per_block": round(float(self.default_ns_per_block[lv]), 4)}
+            if self.default_ns_per_block[lv] > 0:
+                d["speedup_over_default"] = round(float(self.default_ns_per_block[lv] / self.ns_per_block[lv]), 4)
+            out[lv] = d
+        return out
 
 
 class PlanOptions(C.Structure):

@@ -1,7 +1,9 @@
 // Autotuner: per quality level, the launch setting (blocks per wavefront tile,
-// resident workgroups per CU, schedule) that encodes the caller's sample
-// fastest on the current device.  Each candidate is a graph-captured plan,
-// timed with HIP events over a few executes after one warm-up.
+// resident workgroups per CU, schedule) and the four-cluster search schedule
+// (the colour count from which the pruning bound pass runs) that encode the
+// caller's sample fastest on the current device.  Each candidate is a
+// graph-captured plan, timed with HIP events over a few executes after one
+// warm-up; the untuned default's time is kept beside the pick.
 
 #include <hip/hip_runtime.h>
 #include <string.h>
@@ -22,11 +24,16 @@ struct Setting {
     uint32_t tile;
     int32_t schedule;
     int32_t per_cu;  // 0 = every workgroup that fits
+    int32_t prune;   // four-cluster pruning from this many colours (0 = default)
 };
 
 // Milliseconds per execute (best of `reps`) of one setting, or < 0 on failure.
-float time_setting(std::vector<icbc_bc1_job>& jobs, const Setting& st, hipStream_t s, hipEvent_t e0,
-                   hipEvent_t e1, int reps) {
+// The jobs' search schedule comes from the device's tuning table (read when
+// the plan is built), so `tune` is installed with the setting's for `level`.
+float time_setting(std::vector<icbc_bc1_job>& jobs, const Setting& st, int level, icbc_tuning& tune, hipStream_t s,
+                   hipEvent_t e0, hipEvent_t e1, int reps) {
+    tune.prune_min_colors[level] = st.prune;
+    icbc_internal_set_tuning(&tune);
     for (auto& j : jobs) j.tile_blocks = st.tile;
     icbc_plan_options o{st.schedule, 1, st.per_cu, 1};
     icbc_bc1_plan* plan = nullptr;
@@ -79,11 +86,11 @@ int icbc_gpu_autotune(const icbc_bc1_job* sample, size_t n_jobs, uint32_t level_
     // waves of workgroups.
     std::vector<Setting> settings;
     for (uint32_t tile : {8u, 16u, 32u, 64u}) {
-        settings.push_back({tile, ICBC_SCHEDULE_PERSISTENT, 0});
-        if (full > 1) settings.push_back({tile, ICBC_SCHEDULE_PERSISTENT, std::max(1, full / 2)});
+        settings.push_back({tile, ICBC_SCHEDULE_PERSISTENT, 0, 0});
+        if (full > 1) settings.push_back({tile, ICBC_SCHEDULE_PERSISTENT, std::max(1, full / 2), 0});
         uint64_t tiles = 0;
         for (size_t k = 0; k < n_jobs; k++) tiles += (sample[k].block_count + tile - 1) / tile;
-        if (tiles <= kGridScheduleMaxWaves * (uint64_t)resident) settings.push_back({tile, ICBC_SCHEDULE_GRID, 0});
+        if (tiles <= kGridScheduleMaxWaves * (uint64_t)resident) settings.push_back({tile, ICBC_SCHEDULE_GRID, 0, 0});
     }
 
     icbc_tuning tune;
@@ -98,28 +105,48 @@ int icbc_gpu_autotune(const icbc_bc1_job* sample, size_t n_jobs, uint32_t level_
     }
     std::vector<icbc_bc1_job> jobs(sample, sample + n_jobs);
     r = ICBC_OK;
+    const icbc_tuning before = tune;
     for (int level = 0; level < 10 && r == ICBC_OK; level++) {
         if (!(level_mask & (1u << level))) continue;
         for (auto& j : jobs) j.level = level;
-        float best = -1.0f;
+        float best = -1.0f, deflt = -1.0f;
         Setting pick{};
-        for (const Setting& st : settings) {
-            const float ms = time_setting(jobs, st, s, e0, e1, 3);
+        auto consider = [&](const Setting& st) {
+            const float ms = time_setting(jobs, st, level, tune, s, e0, e1, 3);
             if (ms < 0.0f) {
                 r = icbc_internal_fail(ICBC_ERR_DEVICE, "autotune: level %d, tile %u failed", level, st.tile);
-                break;
+                return;
             }
+            if (st.tile == 64 && st.schedule == ICBC_SCHEDULE_PERSISTENT && st.per_cu == 0 && st.prune == 0)
+                deflt = ms;
             if (best < 0.0f || ms < best) {
                 best = ms;
                 pick = st;
+            }
+        };
+        // stage 1: launch settings, default search schedule
+        for (const Setting& st : settings) {
+            consider(st);
+            if (r) break;
+        }
+        // stage 2: the search schedule on the fastest launch setting (levels
+        // 2-9 run the four-cluster search)
+        if (!r && level >= 2) {
+            const Setting base = pick;
+            for (int32_t prune : {9, 11, 15, icbc_gpu::kNoPruneColors}) {
+                consider(Setting{base.tile, base.schedule, base.per_cu, prune});
+                if (r) break;
             }
         }
         if (r) break;
         tune.tile_blocks[level] = pick.tile;
         tune.schedule[level] = pick.schedule;
         tune.workgroups_per_cu[level] = pick.per_cu;
+        tune.prune_min_colors[level] = pick.prune;
         tune.ns_per_block[level] = (float)(best * 1e6 / (double)blocks);
+        tune.default_ns_per_block[level] = (float)(deflt * 1e6 / (double)blocks);
     }
+    if (r) icbc_internal_set_tuning(&before);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(s);
@@ -139,7 +166,10 @@ int icbc_gpu_set_tuning(const icbc_tuning* tuning) {
     if (tuning)
         for (int l = 0; l < 10; l++) {
             const uint32_t t = tuning->tile_blocks[l];
-            if ((t != 0 && t != 8 && t != 16 && t != 32 && t != 64) ||
+            const int32_t pr = tuning->prune_min_colors[l];
+            if ((pr != 0 && (pr < icbc_gpu::kMinPruneColors || pr > icbc_gpu::kNoPruneColors)) ||
+                !(tuning->default_ns_per_block[l] >= 0.0f) ||
+                (t != 0 && t != 8 && t != 16 && t != 32 && t != 64) ||
                 (tuning->schedule[l] != ICBC_SCHEDULE_GRID && tuning->schedule[l] != ICBC_SCHEDULE_PERSISTENT) ||
                 tuning->workgroups_per_cu[l] < 0 || !(tuning->ns_per_block[l] >= 0.0f))
                 return icbc_internal_fail(ICBC_ERR_INVALID_ARGUMENT, "tuning entry for level %d is invalid", l);

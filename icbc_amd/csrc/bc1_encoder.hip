@@ -160,9 +160,6 @@ struct TexelUnroll<TexelsLds<kWeighted>> {
     static constexpr int value = 8;
 };
 
-// Four-cluster searches with at least this many colours run the bound pass
-// first (smaller ones have too few candidates to gain from it).
-constexpr int kPruneMinColors = 9;
 
 // Wave-wide minimum (all lanes get it); NaN inputs must be mapped out first.
 // Which of -0 and +0 survives is unspecified (the value serves as a bound).
@@ -244,9 +241,12 @@ __device__ __forceinline__ int next_block(int my_n, int tile_blocks, int from, i
 // workgroup's workspace through a double-buffered LDS slot: the next block's
 // rows are loaded into registers while the current block is searched.
 // Block b's SAT is at ws + b * kSatRows.
+// Blocks with at least `prune_min` colours run the bound pass first (smaller
+// ones have too few candidates to gain from it; the job's setting, tuned per
+// level).
 template <bool kUnit>
 __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile_blocks, int my_n, f3 cw, f3 metric,
-                                       const DecoderWeights& dw) {
+                                       const DecoderWeights& dw, int prune_min) {
     constexpr int kMinN = 2;
     const int lane = threadIdx.x;
     const float4* ws = uniform_ptr(ws_in);  // the workgroup's workspace: scalar base
@@ -259,7 +259,7 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
     int buf = 0;
     while (b < tile_blocks) {
         if (lane <= nb || lane == kBoundRow) sh.sat[buf][lane] = row;
-        if (nb >= kPruneMinColors) {
+        if (nb >= prune_min) {
             const f3 t = mk(lane_value(row.x, nb), lane_value(row.y, nb), lane_value(row.z, nb));
             const float invW = __builtin_amdgcn_rcpf(lane_value(row.w, nb));
             if (lane <= nb) sh.centred[lane] = centred_row<kUnit>(row, invW, t, cw);
@@ -275,7 +275,7 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
         float best = FLT_MAX;
         int besti = kNoWinner;
         const float4 brow = sat[kBoundRow];
-        if (nb >= kPruneMinColors && brow.y != 0.0f) {
+        if (nb >= prune_min && brow.y != 0.0f) {
             const int total = (nb + 3) * (nb + 2) * (nb + 1) / 6 - 1;  // >= 219: every lane has 3+
             const int rounds = (total + kTile - 1) / kTile;
             const float margin = brow.x;
@@ -719,7 +719,8 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
         if (use3) n3 = m;
         __syncthreads();
         // 3-cluster SATs: the skip_blacks table where one was built, else the 4-cluster one
-        const int win4 = search4<kUnit>(sh, ws, tile_blocks, fit ? n : 0, cw, metric, dw);
+        const int win4 = search4<kUnit>(sh, ws, tile_blocks, fit ? n : 0, cw, metric, dw,
+                                        job.prune_min ? max(job.prune_min, kMinPruneColors) : kDefaultPruneMinColors);
         const int win3 = search3<kUnit>(sh, ws, kTile * kSatRows, skip ? 1 : 0, tile_blocks, use3 ? m : 0, metric);
 
         if (fit) {

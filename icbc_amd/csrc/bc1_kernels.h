@@ -23,7 +23,20 @@ struct JobDev {
     float cw0, cw1, cw2;
     int32_t tile_blocks;  // blocks per tile (8, 16, 32 or 64)
     int32_t rgba8;        // colors holds 16 RGBA8 words per block, read as c / 255.0f
+    int32_t prune_min;    // four-cluster searches over at least this many colours run the
+                          // pruning bound pass first (0 = kDefaultPruneMinColors)
 };
+
+// Colour count from which the four-cluster search prunes by default (the
+// autotuner may pick another per level: 17 never prunes).  Measured on the
+// benchmark content, pruning from 13 colours is 1-4 % faster than from 9 at
+// levels 2-9: below 13 colours the bound pass costs about what it saves.
+// The pruned search
+// needs every lane to hold at least three candidates (C(n+3,3)-1 >= 219, so
+// n >= 9): kMinPruneColors is the lowest setting.
+constexpr int kDefaultPruneMinColors = 13;
+constexpr int kMinPruneColors = 9;
+constexpr int kNoPruneColors = 17;
 
 // A four-cluster table entry holds the partition's boundaries c0 <= c1 <= c2
 // and the index of its pair (c0, c1) among the 153 pairs, laid out so that

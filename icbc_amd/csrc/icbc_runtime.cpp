@@ -416,6 +416,16 @@ uint32_t auto_tile_blocks_impl(uint64_t job_blocks, uint64_t total_blocks, uint3
     return t;
 }
 
+// The four-cluster search schedule tuned for `level` on the current device
+// (0: the default).
+int32_t prune_for_level(int level) {
+    if (level < 0 || level > 9) return 0;
+    icbc_tuning t;
+    memset(&t, 0, sizeof t);
+    (void)tuning_for_current(&t);
+    return t.prune_min_colors[level];
+}
+
 JobDev to_dev(const icbc_bc1_job& j, int decoder) {
     JobDev d;
     d.colors = reinterpret_cast<const float4*>(j.colors);
@@ -433,6 +443,7 @@ JobDev to_dev(const icbc_bc1_job& j, int decoder) {
     d.cw2 = j.color_weights[2];
     d.tile_blocks = j.tile_blocks ? (int32_t)j.tile_blocks : kTile;
     d.rgba8 = (j.flags & ICBC_JOB_COLORS_RGBA8) ? 1 : 0;  // the kernels read RGBA8 texels directly
+    d.prune_min = prune_for_level(j.level);
     return d;
 }
 

@@ -233,14 +233,21 @@ typedef struct icbc_tuning {
     int32_t workgroups_per_cu[10]; /* resident workgroups per CU (persistent schedule) */
     int32_t schedule[10];          /* ICBC_SCHEDULE_* */
     float ns_per_block[10];        /* measured time of the chosen setting */
+    int32_t prune_min_colors[10];  /* four-cluster search schedule: blocks with at least this
+                                      many reduced colours run the pruning bound pass before the
+                                      exact evaluations (9..16, 17 = never; 0 = the default, 13) */
+    float default_ns_per_block[10]; /* measured time of the untuned default setting (tile 64,
+                                       persistent, full occupancy, prune from 13 colours) */
 } icbc_tuning;
 
-/* Times every setting (tile 8/16/32/64 x {persistent at full and half
- * occupancy, one workgroup per tile}) for each level in `level_mask` (bit L =
- * level L) on `sample` (device-memory jobs; their `level` and `tile_blocks`
- * are overridden, their outputs are overwritten), keeps the fastest per level
- * as the current device's tuning (merged with levels tuned before) and
- * returns the whole table in `out` (may be NULL). */
+/* Times, for each level in `level_mask` (bit L = level L) on `sample`
+ * (device-memory jobs; their `level` and `tile_blocks` are overridden, their
+ * outputs are overwritten): first every launch setting (tile 8/16/32/64 x
+ * {persistent at full and half occupancy, one workgroup per tile}), then the
+ * four-cluster search schedule (pruning from 9, 11 or 15 colours, or never) on
+ * the fastest one.  Keeps the fastest per level as the current device's tuning
+ * (merged with levels tuned before) and returns the whole table in `out` (may
+ * be NULL). */
 int icbc_gpu_autotune(const icbc_bc1_job* sample, size_t n_jobs, uint32_t level_mask, icbc_tuning* out);
 /* The current device's tuning (all zero when none). */
 int icbc_gpu_get_tuning(icbc_tuning* out);

@@ -556,15 +556,26 @@ def test_nonfinite_blocks_every_level(gpu, oracle):
                 assert_same(got[0], got[1], want[0], want[1], f"non-finite L{level} tcm={tcm}")
 
 
+@pytest.mark.parametrize("prune", [0, 9])
 @pytest.mark.parametrize("cw", [(1, 1, 1), (0.3, 0.59, 0.11), (0, 1, 0), (5, 0.001, 1)])
-def test_pruned_search_adversarial_blocks(gpu, oracle, cw):
+def test_pruned_search_adversarial_blocks(gpu, oracle, cw, prune):
+    # prune 0: the default schedule (blocks of 13+ colours pruned); 9: every
+    # block of 9+ colours through the bound pass
     blocks, weights = _adversarial_blocks(31, 6000)
-    for level, tcm in ((5, False), (8, True), (8, False)):
-        want = oracle.compress_many(level, blocks, weights, cw, tcm, tcm, nthreads=8)
-        got = gpu.compress_many(level, blocks, weights, cw, tcm, tcm)
-        bad = np.nonzero(np.any(got[0] != want[0], axis=1))[0]
-        assert len(bad) == 0, f"level {level} cw {cw}: {len(bad)} blocks differ, first {bad[:5]}"
-        assert np.array_equal(got[1].view(np.uint32), want[1].view(np.uint32))
+    t = gpu.Tuning()
+    for lv in range(10):
+        t.tile_blocks[lv], t.schedule[lv], t.ns_per_block[lv], t.prune_min_colors[lv] = 64, 1, 1.0, prune
+    try:
+        if prune:
+            gpu.set_tuning(t)
+        for level, tcm in ((5, False), (8, True), (8, False)):
+            want = oracle.compress_many(level, blocks, weights, cw, tcm, tcm, nthreads=8)
+            got = gpu.compress_many(level, blocks, weights, cw, tcm, tcm)
+            bad = np.nonzero(np.any(got[0] != want[0], axis=1))[0]
+            assert len(bad) == 0, f"level {level} cw {cw}: {len(bad)} blocks differ, first {bad[:5]}"
+            assert np.array_equal(got[1].view(np.uint32), want[1].view(np.uint32))
+    finally:
+        gpu.set_tuning(None)
 
 
 def test_refinement_group_size_does_not_change_results(gpu, oracle):
