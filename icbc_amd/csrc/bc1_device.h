@@ -193,6 +193,50 @@ __device__ __forceinline__ void palette_f(uint32_t c0, uint32_t c1, int decoder,
 // The 16 texels and weights of one block, read from memory on demand (L1/L2
 // resident after the first pass) rather than held in registers.
 
+// Global-memory loads and stores.  A pointer read from a job descriptor is
+// generic to the compiler, and accesses through it become flat instructions,
+// whose loads also wait on the LDS counter; these address global memory
+// (global_* instructions, with a scalar base where the address is a wave-
+// uniform pointer plus a lane offset).  Vectors move as the compiler's native
+// vector types.
+template <typename T>
+using InGlobal = __attribute__((address_space(1))) T;
+typedef float NativeF4 __attribute__((ext_vector_type(4)));
+typedef uint32_t NativeU4 __attribute__((ext_vector_type(4)));
+typedef uint32_t NativeU2 __attribute__((ext_vector_type(2)));
+typedef int NativeI4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float ld_global(const float* p) { return *(const InGlobal<float>*)p; }
+__device__ __forceinline__ uint32_t ld_global(const uint32_t* p) { return *(const InGlobal<uint32_t>*)p; }
+__device__ __forceinline__ float4 ld_global(const float4* p) {
+    const NativeF4 v = *(const InGlobal<NativeF4>*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+// The RGB of a float RGBA texel: a 12-byte load (alpha is never read).
+typedef float NativeF3 __attribute__((ext_vector_type(3)));
+__device__ __forceinline__ f3 ld_global_rgb(const float4* p) {
+    const NativeF3 v = *(const InGlobal<NativeF3>*)p;
+    return mk(v.x, v.y, v.z);
+}
+__device__ __forceinline__ uint4 ld_global(const uint4* p) {
+    const NativeU4 v = *(const InGlobal<NativeU4>*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 ld_global(const uint2* p) {
+    const NativeU2 v = *(const InGlobal<NativeU2>*)p;
+    return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ void st_global(float* p, float v) { *(InGlobal<float>*)p = v; }
+__device__ __forceinline__ void st_global(int* p, int v) { *(InGlobal<int>*)p = v; }
+__device__ __forceinline__ void st_global(uint2* p, uint2 v) {
+    const NativeU2 n = {v.x, v.y};
+    *(InGlobal<NativeU2>*)p = n;
+}
+__device__ __forceinline__ void st_global(int4* p, int4 v) {
+    const NativeI4 n = {v.x, v.y, v.z, v.w};
+    *(InGlobal<NativeI4>*)p = n;
+}
+
 // A block's 16 texels in HBM: float RGBA (alpha unused), or with kRgba8 16
 // RGBA8 words (R in the low byte) read as c / 255.0f, as the reference's
 // harness converts 8-bit images (icbc_test.cpp:401-405); 16 weights or null
@@ -203,14 +247,13 @@ struct TexelsOf {
     const float* w;
     __device__ __forceinline__ f3 color(int i) const {
         if constexpr (kRgba8) {
-            const uint32_t v = static_cast<const uint32_t*>(c)[i];
+            const uint32_t v = ld_global(static_cast<const uint32_t*>(c) + i);
             return mk(u8_to_unit(v & 0xffu), u8_to_unit((v >> 8) & 0xffu), u8_to_unit((v >> 16) & 0xffu));
         } else {
-            const float4 v = static_cast<const float4*>(c)[i];
-            return mk(v.x, v.y, v.z);
+            return ld_global_rgb(static_cast<const float4*>(c) + i);
         }
     }
-    __device__ __forceinline__ float weight(int i) const { return w ? w[i] : 1.0f; }
+    __device__ __forceinline__ float weight(int i) const { return w ? ld_global(w + i) : 1.0f; }
 };
 using Texels = TexelsOf<false>;
 
@@ -257,7 +300,7 @@ struct TexelRegs {
         for (int i = 0; i < 16; i++) c[i] = t.color(i);
         if (kWeighted) {
 #pragma unroll
-            for (int i = 0; i < 16; i++) w[kWeighted ? i : 0] = t.w ? t.w[i] : 1.0f;
+            for (int i = 0; i < 16; i++) w[kWeighted ? i : 0] = t.weight(i);
         }
     }
     __device__ __forceinline__ f3 color(int i) const { return c[i]; }

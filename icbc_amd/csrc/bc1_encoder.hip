@@ -150,7 +150,7 @@ struct TexelsLds {
     const float* c;  // 48 floats: texel i at 3 i
     const float* w;  // 16 weights (kWeighted)
     __device__ __forceinline__ f3 color(int i) const { return mk(c[3 * i], c[3 * i + 1], c[3 * i + 2]); }
-    __device__ __forceinline__ float weight(int i) const { return kWeighted ? w[i] : 1.0f; }
+    __device__ __forceinline__ float weight(int i) const { return kWeighted ? ld_global(w + i) : 1.0f; }
 };
 
 // Texel loops over LDS rows unroll by 8: the reads of 8 texels are issued
@@ -213,6 +213,16 @@ __device__ __forceinline__ TexelsOf<kRgba8> block_texels(const JobDev& job, uint
     return t;
 }
 
+// threadIdx.x, opaque to the optimiser.  The tile loop's per-lane values
+// (workspace and LDS row addresses, lane masks) derive from it; computed
+// where they are used they cost a few instructions per tile, while hoisted out
+// of the loop they would be held in registers across every tile's search.
+__device__ __forceinline__ int tile_lane() {
+    int l = threadIdx.x;
+    asm volatile("" : "+v"(l));
+    return l;
+}
+
 // Lane l's value of x (l wave-uniform), as a scalar.
 __device__ __forceinline__ float lane_value(float x, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
@@ -225,6 +235,42 @@ __device__ __forceinline__ T* uniform_ptr(T* p) {
     const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
     const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
     return reinterpret_cast<T*>(lo | (hi << 32));
+}
+
+// Texel source of one lane's block within a tile (a tile's blocks are
+// consecutive blocks of one job): the tile's colours and weights as wave-
+// uniform base pointers in scalar registers, plus the lane's block number, so
+// every texel load is a scalar base + a 32-bit lane offset + an immediate,
+// and no per-lane 64-bit address is held across the block's stages.
+template <bool kRgba8>
+struct TileTexels {
+    static constexpr uint32_t kColorBytes = kRgba8 ? 64u : 256u;  // per block
+    const char* c;  // the tile's first block
+    const char* w;  // its weights (64 bytes per block), or null for all 1.0
+    uint32_t blk;   // the lane's block within the tile
+    __device__ __forceinline__ f3 color(int i) const {
+        const char* p = c + (blk * kColorBytes + (uint32_t)i * (kColorBytes / 16));
+        if constexpr (kRgba8) {
+            const uint32_t v = ld_global(reinterpret_cast<const uint32_t*>(p));
+            return mk(u8_to_unit(v & 0xffu), u8_to_unit((v >> 8) & 0xffu), u8_to_unit((v >> 16) & 0xffu));
+        } else {
+            return ld_global_rgb(reinterpret_cast<const float4*>(p));
+        }
+    }
+    __device__ __forceinline__ float weight(int i) const {
+        return w ? ld_global(reinterpret_cast<const float*>(w + (blk * 64u + (uint32_t)i * 4u))) : 1.0f;
+    }
+};
+
+// Texel source of block first + blk of a job (blk < 64, first + blk inside
+// the job).
+template <bool kRgba8>
+__device__ __forceinline__ TileTexels<kRgba8> tile_texels(const JobDev& job, uint64_t first, uint32_t blk) {
+    TileTexels<kRgba8> t;
+    t.c = uniform_ptr(reinterpret_cast<const char*>(job.colors) + first * TileTexels<kRgba8>::kColorBytes);
+    t.w = job.weights ? uniform_ptr(reinterpret_cast<const char*>(job.weights) + first * 64u) : nullptr;
+    t.blk = blk;
+    return t;
 }
 
 // First block b >= from (wave-uniform) whose owner lane holds my_n >= min_n.
@@ -240,7 +286,7 @@ __device__ __forceinline__ int next_block(int my_n, int tile_blocks, int from, i
 // returned in the owning lane (-1 if none).  SAT rows stream from the
 // workgroup's workspace through a double-buffered LDS slot: the next block's
 // rows are loaded into registers while the current block is searched.
-// Block b's SAT is at ws + b * kSatRows.
+// Block b's SAT is at ws + b * kLaneRows.
 // Blocks with at least `prune_min` colours run the bound pass first (smaller
 // ones have too few candidates to gain from it; the job's setting, tuned per
 // level).
@@ -248,14 +294,14 @@ template <bool kUnit>
 __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile_blocks, int my_n, f3 cw, f3 metric,
                                        const DecoderWeights& dw, int prune_min) {
     constexpr int kMinN = 2;
-    const int lane = threadIdx.x;
+    const int lane = tile_lane();
     const float4* ws = uniform_ptr(ws_in);  // the workgroup's workspace: scalar base
     int my_win = -1;
     int b = next_block(my_n, tile_blocks, 0, kMinN);
     int nb = b < tile_blocks ? __builtin_amdgcn_readlane(my_n, b) : 0;
     float4 row = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    auto base = [&](int blk) { return ws + blk * kSatRows; };
-    if (b < tile_blocks && (lane <= nb || lane == kBoundRow)) row = base(b)[lane];
+    auto base = [&](int blk) { return ws + blk * kLaneRows; };
+    if (b < tile_blocks && (lane <= nb || lane == kBoundRow)) row = ld_global(base(b) + lane);
     int buf = 0;
     while (b < tile_blocks) {
         if (lane <= nb || lane == kBoundRow) sh.sat[buf][lane] = row;
@@ -268,7 +314,7 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
         // prefetch the next block's rows while this one is searched
         const int b2 = next_block(my_n, tile_blocks, b + 1, kMinN);
         const int nb2 = b2 < tile_blocks ? __builtin_amdgcn_readlane(my_n, b2) : 0;
-        if (b2 < tile_blocks && (lane <= nb2 || lane == kBoundRow)) row = base(b2)[lane];
+        if (b2 < tile_blocks && (lane <= nb2 || lane == kBoundRow)) row = ld_global(base(b2) + lane);
 
         const float4* sat = sh.sat[buf];
         const float4 tot = sat[nb];
@@ -434,10 +480,10 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
 template <bool kUnit>
 __device__ __forceinline__ int search3(Shared& sh, const float4* ws_in, int alt_offset, int my_alt, int tile_blocks,
                                        int my_n, f3 metric) {
-    const int lane = threadIdx.x;
+    const int lane = tile_lane();
     const float4* ws = uniform_ptr(ws_in);
     auto base = [&](int blk) {
-        return ws + blk * kSatRows + (__builtin_amdgcn_readlane(my_alt, blk) ? alt_offset : 0);
+        return ws + blk * kLaneRows + (__builtin_amdgcn_readlane(my_alt, blk) ? alt_offset : 0);
     };
     auto total_of = [](int n) { return (n + 2) * (n + 1) / 2 - 1; };  // k_three_total[n - 1]
     auto count_of = [&](int blk) { return blk < tile_blocks ? __builtin_amdgcn_readlane(my_n, blk) : 0; };
@@ -447,10 +493,10 @@ __device__ __forceinline__ int search3(Shared& sh, const float4* ws_in, int alt_
     int c = b < tile_blocks ? next_block(my_n, tile_blocks, b + 1, 1) : tile_blocks;
     int na = count_of(a), nb = count_of(b), nc = count_of(c);
     int sa = 0;  // A's slot; B's is sa ^ 1
-    if (a < tile_blocks && lane <= na) sh.sat[0][lane] = base(a)[lane];
-    if (b < tile_blocks && lane <= nb) sh.sat[1][lane] = base(b)[lane];
+    if (a < tile_blocks && lane <= na) sh.sat[0][lane] = ld_global(base(a) + lane);
+    if (b < tile_blocks && lane <= nb) sh.sat[1][lane] = ld_global(base(b) + lane);
     float4 row = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // rows of block c
-    if (c < tile_blocks && lane <= nc) row = base(c)[lane];
+    if (c < tile_blocks && lane <= nc) row = ld_global(base(c) + lane);
     __syncthreads();
     int done = 0;  // A's candidates evaluated
     float best_a = FLT_MAX, best_b = FLT_MAX;
@@ -502,7 +548,7 @@ __device__ __forceinline__ int search3(Shared& sh, const float4* ws_in, int alt_
             if (b < tile_blocks && lane <= nb) sh.sat[freed][lane] = row;
             c = b < tile_blocks ? next_block(my_n, tile_blocks, b + 1, 1) : tile_blocks;
             nc = count_of(c);
-            if (c < tile_blocks && lane <= nc) row = base(c)[lane];
+            if (c < tile_blocks && lane <= nc) row = ld_global(base(c) + lane);
             // the new A may already be complete (the round also covered all of it)
             if (a < tile_blocks && done >= total_of(na)) {
                 done = 0;
@@ -518,7 +564,7 @@ __device__ __forceinline__ int search3(Shared& sh, const float4* ws_in, int alt_
 
 // ---------------------------------------------------------------------------
 // One tile: the reference's compress_bc1 driver (icbc.h:3734-3808) per lane.
-// `ws` is this workgroup's workspace: 64 blocks x 17 SAT rows.
+// `ws` is this workgroup's workspace: kLaneRows rows per block of the tile.
 
 // Levels 0-1: box fit, output block, and at level 1 least-squares endpoints
 // and a second output block (icbc.h:3769-3786 without reduce_colors, as at
@@ -557,7 +603,8 @@ __device__ __forceinline__ void stage_tile(const JobDev& job, uint64_t first, in
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int g = j * kTile + lane;
-            u[j] = g < nq ? src8[g] : make_uint4(0u, 0u, 0u, 0u);
+            u[j] = make_uint4(0u, 0u, 0u, 0u);
+            if (g < nq) u[j] = ld_global(src8 + g);
         }
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -575,11 +622,12 @@ __device__ __forceinline__ void stage_tile(const JobDev& job, uint64_t first, in
         return;
     }
     const float4* src = job.colors + first * 16;
-    float4 v[16];
+    f3 v[16];
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         const int f = j * kTile + lane;
-        v[j] = f < nf ? src[f] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        v[j] = mk(0.0f, 0.0f, 0.0f);
+        if (f < nf) v[j] = ld_global_rgb(src + f);
     }
 #pragma unroll
     for (int j = 0; j < 16; j++) {
@@ -614,13 +662,13 @@ __device__ __forceinline__ void encode_box_tile(const JobDev& job, uint64_t firs
                 const TexelsLds<false> t{row, nullptr};
                 err = encode_box<kUnit>(t, p.lsq, cw, job.decoder, out);
             }
-            job.codes[bi] = make_uint2(out.cols, out.idx);
+            st_global(job.codes + bi, make_uint2(out.cols, out.idx));
         }
         __syncthreads();  // the next tile restages the rows
     }
     if (active) {
-        if (job.errors) job.errors[bi] = err;
-        if (job.stats) job.stats[bi] = make_int4(16, -1, 0, 0);
+        if (job.errors) st_global(job.errors + bi, err);
+        if (job.stats) st_global(job.stats + bi, make_int4(16, -1, 0, 0));
     }
 }
 
@@ -632,16 +680,14 @@ __device__ __forceinline__ void encode_box_tile(const JobDev& job, uint64_t firs
 template <int kMode, bool kRgba8, bool kUnit>
 __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_index, uint64_t first, Shared& sh,
                                             float4* ws, const RefineList& rl) {
-    const int lane = threadIdx.x;
+    const int lane = tile_lane();
     const int tile_blocks = job.tile_blocks;
-    const uint64_t bi = first + lane;
-    const bool active = lane < tile_blocks && bi < job.block_count;
+    const bool active = lane < tile_blocks && first + lane < job.block_count;
     const Plan p = make_plan(job.level, job.tcm != 0, job.tcb != 0);
     const int dec = job.decoder;
     const f3 cw = mk(job.cw0, job.cw1, job.cw2);
     const DecoderWeights dw = k_dweights[dec];
-    const uint64_t safe = active ? bi : 0;
-    const TexelsOf<kRgba8> tx = block_texels<kRgba8>(job, safe);
+    const TileTexels<kRgba8> tx = tile_texels<kRgba8>(job, first, active ? lane : 0);
 
     Code out{0u, 0u};
     bool written = false;
@@ -688,9 +734,9 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
     if (p.cluster) {  // icbc.h:3788-3801, 3537-3571
         // workspace slices: SATs for the 4-cluster and 3-cluster fits, and the
         // reduced colours parked in their original order for skip_blacks
-        float4* sat4 = ws + lane * kSatRows;
-        float4* sat3 = ws + (kTile + lane) * kSatRows;
-        float4* park = ws + 2 * kTile * kSatRows + lane * 16;
+        float4* sat4 = ws + lane * kLaneRows;
+        float4* sat3 = sat4 + kSatRows;
+        float4* park = sat4 + 2 * kSatRows;
         const bool fit = !done;
         const f3 metric = cw * cw;
         const bool use_black = fit && any_black && job.tcb != 0;
@@ -721,7 +767,7 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
         // 3-cluster SATs: the skip_blacks table where one was built, else the 4-cluster one
         const int win4 = search4<kUnit>(sh, ws, tile_blocks, fit ? n : 0, cw, metric, dw,
                                         job.prune_min ? max(job.prune_min, kMinPruneColors) : kDefaultPruneMinColors);
-        const int win3 = search3<kUnit>(sh, ws, kTile * kSatRows, skip ? 1 : 0, tile_blocks, use3 ? m : 0, metric);
+        const int win3 = search3<kUnit>(sh, ws, kSatRows, skip ? 1 : 0, tile_blocks, use3 ? m : 0, metric);
 
         if (fit) {
             f3 a4 = mk(0.0f, 0.0f, 0.0f), b4 = mk(0.0f, 0.0f, 0.0f);
@@ -756,6 +802,8 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
     // level 9 (icbc.h:3803-3805): the block goes to the refinement kernel,
     // which starts from the code written here (or, if no fit was written, the
     // one the output buffer held, as the reference refines whatever it holds)
+    // the block index, recomputed from the lane: no 64-bit index is held across the tile
+    const uint32_t out_lane = (uint32_t)tile_lane();
     const bool hand_off = p.refine && !done;
     const uint64_t hand_mask = __ballot(hand_off);
     if (hand_mask) {
@@ -765,14 +813,20 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
         if (hand_off) {
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(hand_mask >> 32),
                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)hand_mask, 0));
-            rl.items[at + rank] = RefineItem{job_index, (uint32_t)bi, err, 0u};
+            rl.items[at + rank] = RefineItem{job_index, (uint32_t)(first + out_lane), err, 0u};
         }
     }
 
     if (active) {
-        if (written) job.codes[bi] = make_uint2(out.cols, out.idx);
-        if (job.errors) job.errors[bi] = err;
-        if (job.stats) job.stats[bi] = make_int4(n, n3, steps, 0);
+        if (written) st_global(uniform_ptr(job.codes + first) + out_lane, make_uint2(out.cols, out.idx));
+        if (job.errors) st_global(uniform_ptr(job.errors + first) + out_lane, err);
+        if (job.stats) {  // four word stores: an int4 value would tie n into a register tuple from the start
+            int* st = reinterpret_cast<int*>(job.stats + first + out_lane);
+            st_global(st, n);
+            st_global(st + 1, n3);
+            st_global(st + 2, steps);
+            st_global(st + 3, 0);
+        }
     }
 }
 
@@ -950,7 +1004,7 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
                 cw = mk(jd.cw0, jd.cw1, jd.cw2);
                 tcm = jd.tcm != 0;
                 weighted = jd.weights != nullptr;
-                const uint2 v = jd.codes[block];
+                const uint2 v = ld_global(jd.codes + block);
                 out = Code{v.x, v.y};
                 best = it.error;
                 last = 0;
@@ -1020,9 +1074,9 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
                 if (sub == 0) {
                     const JobDev& jd = jobs[job];
                     const int steps = acc ? 256 : limit + 1;
-                    jd.codes[block] = make_uint2(out.cols, out.idx);
-                    if (jd.errors) jd.errors[block] = best;
-                    if (jd.stats) reinterpret_cast<int32_t*>(jd.stats + block)[2] = steps;
+                    st_global(jd.codes + block, make_uint2(out.cols, out.idx));
+                    if (jd.errors) st_global(jd.errors + block, best);
+                    if (jd.stats) st_global(reinterpret_cast<int32_t*>(jd.stats + block) + 2, steps);
                 }
                 has = false;
                 stale = false;

@@ -76,9 +76,12 @@ int encode_workgroups_per_cu();
 //   then claims further tiles of its XCD's share from that XCD's queue (reset
 //   by this call); otherwise workgroup g takes tiles g, g + grid, ...
 // `workspace` holds kWorkspaceBytesPerWorkgroup bytes per workgroup of `grid`.
-// Per workgroup: two SAT slices (4- and 3-cluster fits) of 64 blocks x 18
-// rows, and the tile's reduced colours (64 x 16 float4).
-constexpr size_t kWorkspaceBytesPerWorkgroup = ((size_t)2 * kTile * kSatRows + (size_t)kTile * 16) * 16;
+// Per workgroup: kLaneRows float4 rows for each of the tile's 64 blocks, in
+// block order: the 4-cluster SAT (18 rows), the 3-cluster SAT (18 rows) and
+// the block's reduced colours (16 rows).  A lane's rows span 832 bytes, so
+// every row is its base address plus an instruction's immediate offset.
+constexpr int kLaneRows = 2 * kSatRows + 16;
+constexpr size_t kWorkspaceBytesPerWorkgroup = (size_t)kTile * kLaneRows * 16;
 
 // Tile queues: one counter per XCD (workgroups are dispatched round-robin
 // over the 8 XCDs, and XCD x takes tiles x, x + 8, x + 16, ... of the list),
