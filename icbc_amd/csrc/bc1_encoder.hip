@@ -676,8 +676,9 @@ __device__ __forceinline__ void encode_box_tile(const JobDev& job, uint64_t firs
 // 0-1 (or invalid levels), kModeCluster launches without level 9, kModeAll
 // everything (the refinement walk's registers weigh only on that one).
 // kRgba8: the job's colours are RGBA8 words (one instance per texel format,
-// chosen per tile, so neither pays for the other's loads).
-template <int kMode, bool kRgba8, bool kUnit>
+// chosen per tile, so neither pays for the other's loads).  Without
+// kBoxStage every job is at level 4-9 (no box-fit stage).
+template <int kMode, bool kRgba8, bool kUnit, bool kBoxStage>
 __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_index, uint64_t first, Shared& sh,
                                             float4* ws, const RefineList& rl) {
     const int lane = tile_lane();
@@ -712,7 +713,7 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
         done = true;
     }
 
-    if (p.box && !done) {  // icbc.h:3769-3786
+    if (kBoxStage && p.box && !done) {  // icbc.h:3769-3786
         f3 c0, c1;
         if (p.cluster) box_fit([&](int i) { return rc[i]; }, n, c0, c1);
         else box_fit([&](int i) { return tx.color(i); }, 16, c0, c1);
@@ -847,7 +848,8 @@ __device__ __forceinline__ void load_tables(S& sh) {
 // workgroup takes tiles blockIdx.x, blockIdx.x + gridDim.x, ...  With `tiles`
 // null the launch covers one job, tile k = blocks [64 k, 64 k + 64).
 // kUnit: every job of the launch has channel weights (1, 1, 1).
-template <int kMode, int kTex, bool kUnit>
+// kBoxStage: some job of the launch is at level 0-3 (cluster instances).
+template <int kMode, int kTex, bool kUnit, bool kBoxStage = true>
 __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) void bc1_encode_kernel(
     const JobDev* __restrict__ jobs, const uint2* __restrict__ tiles, uint32_t ntiles,
     uint32_t* __restrict__ counter, float4* __restrict__ workspace, RefineList refine) {
@@ -877,10 +879,10 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         }
         if constexpr (kMode == kModeBox) encode_box_tile<kUnit>(jobs[job], first, sh);  // either format, per tile
         else if constexpr (kTex == kTexMixed) {
-            if (jobs[job].rgba8) encode_tile<kMode, true, kUnit>(jobs[job], job, first, sh, ws, refine);
-            else encode_tile<kMode, false, kUnit>(jobs[job], job, first, sh, ws, refine);
+            if (jobs[job].rgba8) encode_tile<kMode, true, kUnit, kBoxStage>(jobs[job], job, first, sh, ws, refine);
+            else encode_tile<kMode, false, kUnit, kBoxStage>(jobs[job], job, first, sh, ws, refine);
         } else {
-            encode_tile<kMode, kTex == kTexRgba8, kUnit>(jobs[job], job, first, sh, ws, refine);
+            encode_tile<kMode, kTex == kTexRgba8, kUnit, kBoxStage>(jobs[job], job, first, sh, ws, refine);
         }
         if (queue) {
             uint32_t v = 0;
@@ -1259,6 +1261,7 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
     }
     const int tex = (mode >> 4) & 3;
     const bool unit = (mode & kUnitWeights) != 0;
+    const bool box = (mode & kBoxStage) != 0;
     mode &= 15;
     if (mode == kModeAll) {
         hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, refine.counters, 2u, stride);
@@ -1268,20 +1271,24 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
     // every schedule covers all tiles with any grid (queue claims or a
     // grid-stride walk), so the level 0-1 instance runs at its own residency
     const dim3 g(grid), bg(std::min(grid, box_grid())), blk(kTile);
-#define ENC(M, T, U) hipLaunchKernelGGL((bc1_encode_kernel<M, T, U>), (M) == kModeBox ? bg : g, blk, 0, stream, jobs, \
-                                        tiles, ntiles, counter, workspace, refine)
+#define ENC(M, T, U, B) hipLaunchKernelGGL((bc1_encode_kernel<M, T, U, B>), (M) == kModeBox ? bg : g, blk, 0, stream, \
+                                           jobs, tiles, ntiles, counter, workspace, refine)
     // The unit-weight instances run the level 0-1 kernel and the refinement;
     // the cluster kernel has only the general one (its unit-weight build was
-    // measured slower: the compiler schedules the search worse).
+    // measured slower: the compiler schedules the search worse).  Launches
+    // of one texel format at levels 4-9 run cluster instances without the
+    // box-fit stage.
     if (mode == kModeBox) {
-        if (unit) ENC(kModeBox, kTexFloat, true);
-        else ENC(kModeBox, kTexFloat, false);
+        if (unit) ENC(kModeBox, kTexFloat, true, true);
+        else ENC(kModeBox, kTexFloat, false, true);
     } else if (tex == kTexRgba8) {
-        ENC(kModeCluster, kTexRgba8, false);
+        if (box) ENC(kModeCluster, kTexRgba8, false, true);
+        else ENC(kModeCluster, kTexRgba8, false, false);
     } else if (tex == kTexMixed) {
-        ENC(kModeCluster, kTexMixed, false);
+        ENC(kModeCluster, kTexMixed, false, true);
     } else {
-        ENC(kModeCluster, kTexFloat, false);
+        if (box) ENC(kModeCluster, kTexFloat, false, true);
+        else ENC(kModeCluster, kTexFloat, false, false);
     }
 #undef ENC
     hipError_t e = hipGetLastError();

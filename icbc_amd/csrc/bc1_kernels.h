@@ -131,8 +131,13 @@ constexpr int kUnitWeights = 1 << 6;
 
 inline bool unit_weights(const JobDev& j) { return j.cw0 == 1.0f && j.cw1 == 1.0f && j.cw2 == 1.0f; }
 
+// Some job of a launch is at level 0-3, whose box-fit stage the cluster
+// instance then carries; launches at levels 4-9 only run an instance without
+// it (fewer live registers across the per-block stages).
+constexpr int kBoxStage = 1 << 7;
+
 // `mode` selects the kernel instance (level mode | texel format << 4 |
-// kUnitWeights); encode_mode() gives the one a set of jobs needs.
+// kUnitWeights | kBoxStage); encode_mode() gives the one a set of jobs needs.
 // With kModeAll, `refine` holds room for the launch's level-9 blocks.
 hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, uint32_t* counter,
                          float4* workspace, uint32_t grid, int mode, RefineList refine, hipStream_t stream);
@@ -160,10 +165,12 @@ inline int encode_mode(const JobDev* jobs, size_t n) {
         else any_float = true;
     }
     const int tex = any_rgba8 ? (any_float ? kTexMixed : kTexRgba8) : kTexFloat;
-    bool unit = true;
-    for (size_t k = 0; k < n; k++)
+    bool unit = true, box = false;
+    for (size_t k = 0; k < n; k++) {
         if (!unit_weights(jobs[k])) unit = false;
-    return mode | (tex << 4) | (unit ? kUnitWeights : 0);
+        if (jobs[k].level >= 0 && jobs[k].level <= 3) box = true;
+    }
+    return mode | (tex << 4) | (unit ? kUnitWeights : 0) | (box ? kBoxStage : 0);
 }
 
 // The one-workgroup-per-tile schedule gives every tile its own workspace
