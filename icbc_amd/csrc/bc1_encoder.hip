@@ -906,14 +906,15 @@ __global__ void zero_words_kernel(uint32_t* p, uint32_t n, uint32_t stride) {
 // step is accepted) and keeps the step if its error is strictly lower; a walk
 // ends 33 steps after its last improvement or after 256 steps.
 //
-// A walk runs on a group of K lanes (K a power of two, 1..16): since every
+// A walk runs on a group of K lanes (K a power of two, 1..64): since every
 // step until the next acceptance starts from the same best code, the group
 // evaluates steps i .. i+K-1 at once and keeps the first accepted one (the
 // later ones are discarded, as the sequential walk would never have tried
 // them from that code), or, with none accepted, moves on K steps -- never past
 // the step where the walk ends.  Results are those of the sequential walk.
 // K grows when there are few walks for the lanes, so short launches are not
-// bound by their longest walk.  A group whose walk has ended takes the next
+// bound by their longest walk, and again, up to 64, once no block is left to
+// start (the wavefront's last walks then share all of its lanes).  A group whose walk has ended takes the next
 // block from the list (claims are gathered per wavefront).
 constexpr int kRefineThreads = 256;
 
@@ -952,9 +953,9 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
     const uint64_t lanes = (uint64_t)gridDim.x * kRefineThreads;
     int K = 1;
     while (K < 16 && (uint64_t)total * (uint64_t)(2 * K) <= 4 * lanes) K *= 2;
-    const int sub = lane & (K - 1);
-    const int leader = lane - sub;
-    const uint64_t group_bits = (K == 64 ? ~0ull : ((1ull << K) - 1)) << leader;
+    int sub = lane & (K - 1);
+    int leader = lane - sub;
+    uint64_t group_bits = ((1ull << K) - 1) << leader;
     uint64_t leaders = 0;
     for (int l = 0; l < kTile; l += K) leaders |= 1ull << l;
 
@@ -979,6 +980,57 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
     const int refill_groups = max(1, (kTile / 4) / K);
     const int all_groups = kTile / K;
     for (;;) {
+        // Tail: no block is left to start (the list is drained everywhere at
+        // about the same time, so every wavefront is here together), and the
+        // wavefront's remaining walks would finish on a few lanes.  They spread
+        // over the whole wavefront instead: walk w moves to lanes
+        // [w K', (w + 1) K') for the largest K' <= 64 with walks * K' <= 64,
+        // its state copied from its leader lane and its texels read again.
+        if (drained && lo == hi && K < kTile) {
+            const uint64_t walks = __ballot(has) & leaders;
+            const int nwalks = __popcll(walks);
+            int K2 = K, log2K2 = __builtin_ctz((uint32_t)K);
+            while (K2 < kTile && nwalks * 2 * K2 <= kTile) {
+                K2 *= 2;
+                log2K2++;
+            }
+            if (nwalks > 0 && K2 > K) {
+                int src = -1;  // the leader lane of this lane's new walk
+                uint64_t left = walks;
+                for (int w = 0; left; w++) {
+                    const int l = __builtin_ctzll(left);
+                    left &= left - 1;
+                    if ((lane >> log2K2) == w) src = l;
+                }
+                const int from = src >= 0 ? src : lane;
+                const bool moved = src >= 0;
+                const uint32_t cols = (uint32_t)__shfl((int)out.cols, from, kTile);
+                const uint32_t oidx = (uint32_t)__shfl((int)out.idx, from, kTile);
+                best = __shfl(best, from, kTile);
+                last = __shfl(last, from, kTile);
+                i = __shfl(i, from, kTile);
+                idx = (uint32_t)__shfl((int)idx, from, kTile);
+                job = (uint32_t)__shfl((int)job, from, kTile);
+                block = (uint32_t)__shfl((int)block, from, kTile);
+                stale = __shfl(stale ? 1 : 0, from, kTile) != 0 && moved;
+                out = Code{cols, oidx};
+                has = moved;
+                if (has) {
+                    const JobDev& jd = jobs[job];
+                    if (jd.rgba8) r.load(block_texels<true>(jd, block));
+                    else r.load(block_texels<false>(jd, block));
+                    cw = mk(jd.cw0, jd.cw1, jd.cw2);
+                    tcm = jd.tcm != 0;
+                    weighted = jd.weights != nullptr;
+                }
+                K = K2;
+                sub = lane & (K - 1);
+                leader = lane - sub;
+                group_bits = (K == kTile ? ~0ull : ((1ull << K) - 1)) << leader;
+                leaders = 0;
+                for (int l = 0; l < kTile; l += K) leaders |= 1ull << l;
+            }
+        }
         // free groups (their leader lanes) start the wavefront's next blocks
         uint64_t free_mask = __ballot(!has) & leaders;
         const int nfree = __popcll(free_mask);
