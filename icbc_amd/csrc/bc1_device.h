@@ -407,21 +407,65 @@ __device__ __forceinline__ float select3(const Tx& t, f3 w, bool allow_black, co
 // `pal` (the candidate block's palette).
 // Indices of the 16 texels against `sel` (palette of the current best block):
 // nearest entry under the colour weights, lowest index on ties.
+// Index of colour x against the weighted palette entries p0..p3 (the
+// entries already multiplied by the colour weights w).
+template <bool kUnit>
+__device__ __forceinline__ uint32_t nearest_index(f3 x, f3 w, f3 p0, f3 p1, f3 p2, f3 p3) {
+    const f3 c = wmul<kUnit>(x, w);
+    const f3 e0 = c - p0, e1 = c - p1, e2 = c - p2, e3 = c - p3;
+    const float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2), d3 = dot3(e3, e3);
+    const uint32_t i1 = (d1 <= d0) & (d1 < d2) & (d1 < d3);
+    const uint32_t i2 = (d2 <= d0) & (d2 <= d1) & (d2 < d3);
+    const uint32_t i3 = (d3 <= d0) & (d3 <= d1) & (d3 <= d2);
+    return (i1 | i3) | ((i2 | i3) << 1);
+}
+
 template <bool kUnit, typename Src>
 __device__ __forceinline__ uint32_t select_indices(const Src& t, f3 w, const f3 sel[4]) {
     const f3 p0 = wmul<kUnit>(sel[0], w), p1 = wmul<kUnit>(sel[1], w), p2 = wmul<kUnit>(sel[2], w),
              p3 = wmul<kUnit>(sel[3], w);
     uint32_t out = 0;
 #pragma unroll TexelUnroll<Src>::value
-    for (int i = 0; i < 16; i++) {
-        const f3 c = wmul<kUnit>(t.color(i), w);
-        const f3 e0 = c - p0, e1 = c - p1, e2 = c - p2, e3 = c - p3;
-        const float d0 = dot3(e0, e0), d1 = dot3(e1, e1), d2 = dot3(e2, e2), d3 = dot3(e3, e3);
-        const uint32_t i1 = (d1 <= d0) & (d1 < d2) & (d1 < d3);
-        const uint32_t i2 = (d2 <= d0) & (d2 <= d1) & (d2 < d3);
-        const uint32_t i3 = (d3 <= d0) & (d3 <= d1) & (d3 <= d2);
-        out |= ((i1 | i3) | ((i2 | i3) << 1)) << (2 * i);
+    for (int i = 0; i < 16; i++) out |= nearest_index<kUnit>(t.color(i), w, p0, p1, p2, p3) << (2 * i);
+    return out;
+}
+
+// The indices of select_indices computed by a group of K lanes (K a power of
+// two, wavefront-uniform) that hold the same texels: with K = 2 each lane
+// takes 8 texels, with K >= 4 lanes 0-3 of the group take 4 each, and the
+// lanes' index bits are combined over the group.  Every lane of the group
+// must be active.
+template <bool kUnit, bool kWeighted>
+__device__ __forceinline__ uint32_t group_select_indices(const TexelRegs<kWeighted>& r, f3 w, const f3 sel[4], int K,
+                                                         int sub) {
+    const f3 p0 = wmul<kUnit>(sel[0], w), p1 = wmul<kUnit>(sel[1], w), p2 = wmul<kUnit>(sel[2], w),
+             p3 = wmul<kUnit>(sel[3], w);
+    uint32_t out = 0;
+    if (K == 1) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) out |= nearest_index<kUnit>(r.c[i], w, p0, p1, p2, p3) << (2 * i);
+        return out;
     }
+    if (K == 2) {
+        const bool hi = sub != 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const f3 a = r.c[j], b = r.c[8 + j];
+            const f3 x = mk(hi ? b.x : a.x, hi ? b.y : a.y, hi ? b.z : a.z);
+            out |= nearest_index<kUnit>(x, w, p0, p1, p2, p3) << (2 * j);
+        }
+        out <<= hi ? 16 : 0;
+    } else if (sub < 4) {
+        const bool b0 = (sub & 1) != 0, b1 = (sub & 2) != 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const f3 a = b0 ? r.c[4 + j] : r.c[j], b = b0 ? r.c[12 + j] : r.c[8 + j];
+            const f3 x = b1 ? b : a;
+            out |= nearest_index<kUnit>(x, w, p0, p1, p2, p3) << (2 * j);
+        }
+        out <<= 8 * sub;
+    }
+    for (int o = 1; o < K; o <<= 1) out |= (uint32_t)__shfl_xor((int)out, o, 64);
     return out;
 }
 

@@ -1077,7 +1077,7 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
             if (has && stale) {
                 f3 sel[4];
                 palette_f(col0(out), col1(out), decoder, sel);
-                idx = select_indices<kUnit>(r, cw, sel);
+                idx = group_select_indices<kUnit>(r, cw, sel, K, sub);
                 stale = false;
             }
             continue;
