@@ -899,6 +899,30 @@ __device__ __forceinline__ float4 bound_row(const f3 (&c)[16], const float (&w)[
     return make_float4(margin, ok ? 1.0f : 0.0f, 0.0f, 0.0f);
 }
 
+// Lower bound of a four-cluster candidate's error (as candidate4 computes it,
+// without the constant sum(w x^2) term) from its clusters alone.  All texels
+// of a cluster share one palette point p, and a cluster's share of the
+// quadratic, sum_c m_c (W p_c^2 - 2 p_c X_c) (X, W its colour and weight
+// sums), is at least -|X|_m^2 / W; the sum over the four clusters bounds the
+// candidate's error whatever its endpoints.  No system is solved, so it also
+// decides candidates whose system is nearly singular.  Each cluster's sums are
+// one subtraction of SAT rows (relative error u) and the terms are non-
+// negative with a sum at most sum_c m_c P_c, so the rounding is far inside
+// the block's margin, which also covers the deviation of candidate4's
+// arithmetic (rounded palette constants, its own cluster sums) from the
+// quadratic.
+template <bool kUnit>
+__device__ __forceinline__ float cluster_bound(float4 s0, float4 s1, float4 s2, float4 tot, f3 metric) {
+    auto term = [&](float4 lo, float4 hi) {
+        const f3 x = mk(hi.x - lo.x, hi.y - lo.y, hi.z - lo.z);
+        const float w = hi.w - lo.w;
+        const float nrm = metric_dot<kUnit>(x * x, metric);
+        return w > 0.0f ? nrm * __builtin_amdgcn_rcpf(w) : 0.0f;  // an empty cluster has X = 0
+    };
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return -(((term(zero, s0) + term(s0, s1)) + term(s1, s2)) + term(s2, tot));
+}
+
 template <bool kUnit>
 __device__ __forceinline__ float candidate3(float4 s0, float4 s1, float4 tot, f3 metric, f3& a, f3& b) {
     f3 x0 = mk(s0.x, s0.y, s0.z), x1 = mk(s1.x, s1.y, s1.z);

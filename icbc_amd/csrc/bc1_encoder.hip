@@ -34,6 +34,9 @@ __constant__ int k_three_total[16];
 __constant__ uint16_t k_match5[4][256];
 __constant__ uint16_t k_match6[4][256];
 __constant__ DecoderWeights k_dweights[4];
+__constant__ uint16_t k_singular[4][64];
+__constant__ uint16_t k_singular_mask[4][64];
+__constant__ int k_singular_count[4];
 
 hipError_t upload_tables(const HostTables& t, hipStream_t stream) {
     hipError_t e;
@@ -50,6 +53,9 @@ hipError_t upload_tables(const HostTables& t, hipStream_t stream) {
     UP(k_match5, t.match5);
     UP(k_match6, t.match6);
     UP(k_dweights, t.dweights);
+    UP(k_singular, t.singular);
+    UP(k_singular_mask, t.singular_mask);
+    UP(k_singular_count, t.singular_count);
 #undef UP
     return hipStreamSynchronize(stream);
 }
@@ -105,11 +111,13 @@ __device__ __forceinline__ uint32_t four_c1(uint32_t e) { return (e >> 26) & 31u
 __device__ __forceinline__ uint32_t four_c2(uint32_t e) { return (e >> 16) & 31u; }
 
 // Per-workgroup LDS: candidate tables and two SAT buffers (the block being
-// searched and the next one).  8,672 bytes: at most 10 KiB keeps 16
+// searched and the next one).  8,928 bytes: at most 10 KiB keeps 16
 // workgroups (4 waves per SIMD) resident per CU.
 struct Shared {
     uint32_t four[1024];          // 976 entries, padded so 16 rounds of 64 lanes stay inside
     uint32_t three[160];
+    uint16_t singular[64];        // the launch decoder's nearly singular candidates (k_singular)
+    uint16_t singular_mask[64];
     float4 sat[2][kSatRows];
     float4 centred[kSatRows];     // four-cluster search: the block's rows centred for the bound pass
                                   // (one buffer: the wavefront's LDS accesses complete in order)
@@ -292,7 +300,7 @@ __device__ __forceinline__ int next_block(int my_n, int tile_blocks, int from, i
 // level).
 template <bool kUnit>
 __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile_blocks, int my_n, f3 cw, f3 metric,
-                                       const DecoderWeights& dw, int prune_min) {
+                                       const DecoderWeights& dw, int prune_min, bool unweighted, int singular_count) {
     constexpr int kMinN = 2;
     const int lane = tile_lane();
     const float4* ws = uniform_ptr(ws_in);  // the workgroup's workspace: scalar base
@@ -340,7 +348,7 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
             // its key is the one with the lowest bound (largest q)
             float q[16];
             float qkey = -INFINITY;
-            uint32_t pkey = sh.four[lane];
+            int kkey = 0;  // the round of the lane's key candidate
             if (nb == 16) {
                 // 968 candidates: 15 full rounds and 8 lanes of a 16th, as
                 // straight-line code: no per-round branch, so the rounds form
@@ -354,12 +362,12 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
                     q[k] = valid ? v : __builtin_nanf("");
                     const bool better = valid & (v > qkey);
                     qkey = better ? v : qkey;
-                    pkey = better ? pk : pkey;
+                    kkey = better ? k : kkey;
                 }
             } else {
                 // software pipeline: round k computes while round k + 1's pair
                 // sums and row are read, and round k + 2's table entry
-                uint32_t pk = pkey;
+                uint32_t pk = sh.four[lane];
                 float4 pc = pair_c(sh, pk);
                 float pa = pair_a(sh, pk);
                 float4 r2 = row_at(cen, pk);
@@ -383,7 +391,7 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
                         q[k] = valid ? v : q[k];
                         const bool better = valid & (v > qkey);
                         qkey = better ? v : qkey;
-                        pkey = better ? pk : pkey;
+                        kkey = better ? k : kkey;
                         pk = pk_next;
                         pc = pc_n;
                         pa = pa_n;
@@ -393,15 +401,35 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
                 }
             }
             // threshold: the best true error among the lanes' key candidates;
-            // a candidate is pruned when -K0 - W q - margin > threshold
+            // a candidate is pruned when -K0 - W q - margin > threshold.  The
+            // keys' errors also enter the argmin (the winner rule below), and
+            // the keys leave the survivor list.
             float thr;
             {
+                const int ikey = lane + kTile * kkey;
+                const uint32_t pkey = sh.four[ikey];
                 f3 a, e;
                 float err = candidate4<kUnit>(sat[four_c0(pkey)], sat[four_c1(pkey)], sat[four_c2(pkey)], tot, metric,
                                        dw, a, e);
                 thr = wave_min(err == err ? err : FLT_MAX);
+                if (err < best) {  // the lane's first candidate: best is FLT_MAX, no winner yet
+                    best = err;
+                    besti = ikey;
+                }
             }
             const float qthr = (-K0 - margin - thr) / W;
+            // Blocks of 16 unit-weight colours: the candidates whose system is
+            // nearly singular (q NaN, never pruned by q) are a fixed set, decided
+            // instead by the cluster bound, one lane each
+            const bool singular_set = unweighted && nb == 16;
+            bool singular_keep = false;
+            int singular_index = 0;
+            if (singular_set && lane < singular_count) {
+                singular_index = sh.singular[lane];
+                const uint32_t pk = sh.four[singular_index];
+                const float lb = cluster_bound<kUnit>(sat[four_c0(pk)], sat[four_c1(pk)], sat[four_c2(pk)], tot, metric);
+                singular_keep = !(lb - margin > thr);  // NaN: keep (a key evaluated twice is harmless)
+            }
             __syncthreads();  // the pair sums are dead: their space takes the survivor list
             // survivors to LDS, lane-major: each lane marks its kept rounds in a
             // mask, a wave prefix sum of the counts gives its first slot
@@ -411,15 +439,25 @@ __device__ __forceinline__ int search4(Shared& sh, const float4* ws_in, int tile
 #pragma unroll
             for (int k = 0; k < 16; k++) pruned |= (q[k] < qthr ? 1u : 0u) << k;
             const int mine_rounds = (total - lane + kTile - 1) / kTile;
-            uint32_t keep = ((1u << mine_rounds) - 1u) & ~pruned;
+            uint32_t keep = ((1u << mine_rounds) - 1u) & ~pruned & ~(1u << kkey);
+            if (singular_set) keep &= ~(uint32_t)sh.singular_mask[lane];
             const int mine = __popc(keep);
             const int incl = wave_inclusive_sum(mine);
-            const int count = __builtin_amdgcn_readlane(incl, kTile - 1);
+            int count = __builtin_amdgcn_readlane(incl, kTile - 1);
             int slot = incl - mine;
             while (keep) {
                 const int k = __builtin_ctz(keep);
                 keep &= keep - 1;
                 sh.survivors[slot++] = (uint16_t)(lane + kTile * k);
+            }
+            if (singular_set) {  // the singular candidates the cluster bound keeps follow
+                const uint64_t kept = __ballot(singular_keep);
+                if (singular_keep) {
+                    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(kept >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)kept, 0));
+                    sh.survivors[count + rank] = (uint16_t)singular_index;
+                }
+                count += __popcll(kept);
             }
             __syncthreads();
             // the survivors get the exact evaluation; every candidate that could
@@ -767,7 +805,8 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
         __syncthreads();
         // 3-cluster SATs: the skip_blacks table where one was built, else the 4-cluster one
         const int win4 = search4<kUnit>(sh, ws, tile_blocks, fit ? n : 0, cw, metric, dw,
-                                        job.prune_min ? max(job.prune_min, kMinPruneColors) : kDefaultPruneMinColors);
+                                        job.prune_min ? max(job.prune_min, kMinPruneColors) : kDefaultPruneMinColors,
+                                        job.weights == nullptr, k_singular_count[dec & 3]);
         const int win3 = search3<kUnit>(sh, ws, kSatRows, skip ? 1 : 0, tile_blocks, use3 ? m : 0, metric);
 
         if (fit) {
@@ -835,10 +874,12 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
 // kernels
 
 template <int kMode, typename S>
-__device__ __forceinline__ void load_tables(S& sh) {
+__device__ __forceinline__ void load_tables(S& sh, int decoder) {
     if constexpr (kMode != kModeBox) {
         for (int i = threadIdx.x; i < 1024; i += kTile) sh.four[i] = k_four[i < 976 ? i : 975];
         for (int i = threadIdx.x; i < 160; i += kTile) sh.three[i] = k_three[i];
+        sh.singular[threadIdx.x] = k_singular[decoder][threadIdx.x];
+        sh.singular_mask[threadIdx.x] = k_singular_mask[decoder][threadIdx.x];
         __syncthreads();
     }
 }
@@ -854,7 +895,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) v
     const JobDev* __restrict__ jobs, const uint2* __restrict__ tiles, uint32_t ntiles,
     uint32_t* __restrict__ counter, float4* __restrict__ workspace, RefineList refine) {
     __shared__ std::conditional_t<kMode == kModeBox, SharedBox, Shared> sh;
-    load_tables<kMode>(sh);
+    load_tables<kMode>(sh, jobs[0].decoder & 3);  // one decoder per device (init)
     float4* ws = workspace + (size_t)blockIdx.x * (kWorkspaceBytesPerWorkgroup / sizeof(float4));
     // queue of this workgroup's XCD: its share is tiles x, x + 8, ...; the
     // first claim of each workgroup is static, later ones come from the counter

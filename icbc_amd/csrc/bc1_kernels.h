@@ -59,6 +59,13 @@ struct HostTables {
     uint16_t match5[4][256];
     uint16_t match6[4][256];
     float dweights[4][5];
+    // Per decoder, the four-cluster candidates of a block of 16 colours of
+    // weight 1 whose least-squares system is nearly singular (the pruning
+    // bound cannot decide them): their table indices (at most 64) and, per
+    // lane l, the bit mask of rounds k with candidate l + 64 k in the set.
+    uint16_t singular[4][64];
+    uint16_t singular_mask[4][64];
+    int32_t singular_count[4];
 };
 
 hipError_t upload_tables(const HostTables& t, hipStream_t stream);

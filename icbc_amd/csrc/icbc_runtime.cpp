@@ -169,6 +169,27 @@ void build_tables(HostTables& t) {
         t.dweights[d][2] = w1_3 * w1_3;
         t.dweights[d][3] = w2_3 * w2_3;
         t.dweights[d][4] = w1_3 * w2_3;
+        // Nearly singular systems of 16 unit-weight colours: with cumulative
+        // weights c0 <= c1 <= c2 at the boundaries, A W - Wa^2 < A W / 16
+        // (A = sum w alpha^2, Wa = sum w alpha, W = 16; the device's guard).
+        // Which candidates are listed only affects speed: a listed candidate is
+        // decided by the cluster bound, an unlisted singular one is evaluated.
+        const double a0 = 1.0 - t.dweights[d][3], a1 = (double)t.dweights[d][3] - t.dweights[d][2],
+                     a2 = t.dweights[d][2];
+        const double k0 = 1.0 - w2_3, k1 = (double)w2_3 - w1_3, k2 = w1_3;
+        int count = 0;
+        for (int k = 0; k < 64; k++) t.singular_mask[d][k] = 0;
+        for (int k = 0; k < 64; k++) t.singular[d][k] = 0;
+        for (int i = 0; i < t.four_total[15] && count < 64; i++) {
+            const uint32_t e = t.four[i];
+            const double s0 = (e >> 21) & 31u, s1 = (e >> 26) & 31u, s2 = (e >> 16) & 31u;
+            const double A = a0 * s0 + a1 * s1 + a2 * s2, Wa = k0 * s0 + k1 * s1 + k2 * s2, W = 16.0;
+            if (A * W - Wa * Wa < A * W / 16.0 * 1.001) {
+                t.singular[d][count++] = (uint16_t)i;
+                t.singular_mask[d][i & 63] |= (uint16_t)(1u << (i >> 6));
+            }
+        }
+        t.singular_count[d] = count;
     }
 }
 
