@@ -60,49 +60,6 @@ hipError_t upload_tables(const HostTables& t, hipStream_t stream) {
     return hipStreamSynchronize(stream);
 }
 
-// ---------------------------------------------------------------------------
-// Wave-wide argmin over (error, candidate index): smaller error wins, equal
-// errors (including -0 == +0) resolve to the smaller index.  Each lane's pair
-// becomes one 64-bit key, the error's bits mapped to an unsigned order
-// (after -0 + 0 = +0, so that equal errors tie) above the index, and the
-// minimum key is carried to lane 63 by DPP moves: quad swaps, half-row and
-// row mirrors (every lane has a source in its row), then the row broadcasts.
-// Errors here are never NaN (a lane's best only takes values below FLT_MAX).
-
-__device__ __forceinline__ uint64_t argmin_key(float e, int i) {
-    const uint32_t u = __float_as_uint(e + 0.0f);
-    const uint32_t k = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-    return ((uint64_t)k << 32) | (uint32_t)i;
-}
-
-template <int kCtrl>
-__device__ __forceinline__ uint64_t key_min_in_row(uint64_t k) {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)k, kCtrl, 0xf, 0xf, false);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(k >> 32), kCtrl, 0xf, 0xf, false);
-    const uint64_t o = ((uint64_t)hi << 32) | lo;
-    return o < k ? o : k;
-}
-
-template <int kCtrl, int kRowMask>
-__device__ __forceinline__ uint64_t key_min_bcast(uint64_t k) {  // rows outside kRowMask keep theirs
-    const int lo0 = (int)(uint32_t)k, hi0 = (int)(uint32_t)(k >> 32);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(lo0, lo0, kCtrl, kRowMask, 0xf, false);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(hi0, hi0, kCtrl, kRowMask, 0xf, false);
-    const uint64_t o = ((uint64_t)hi << 32) | lo;
-    return o < k ? o : k;
-}
-
-__device__ __forceinline__ int wave_argmin(float e, int i) {
-    uint64_t k = argmin_key(e, i);
-    k = key_min_in_row<0xB1>(k);         // quad_perm [1,0,3,2]
-    k = key_min_in_row<0x4E>(k);         // quad_perm [2,3,0,1]
-    k = key_min_in_row<0x141>(k);        // row_half_mirror
-    k = key_min_in_row<0x140>(k);        // row_mirror
-    k = key_min_bcast<0x142, 0xa>(k);    // row_bcast:15 into rows 1 and 3
-    k = key_min_bcast<0x143, 0xc>(k);    // row_bcast:31 into rows 2 and 3
-    return __builtin_amdgcn_readlane((int)(uint32_t)k, 63);
-}
-
 constexpr int kNoWinner = 0x7fffffff;
 
 // Fields of a four-cluster table entry (four_entry).
@@ -191,6 +148,34 @@ __device__ __forceinline__ float wave_min(float e) {
     e = min_step<0x142, 0xa>(e);
     e = min_step<0x143, 0xc>(e);
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e), 63));
+}
+
+// The same reduction over ints.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ int imin_step(int v) {
+    return min(v, __builtin_amdgcn_update_dpp(v, v, kCtrl, kRowMask, 0xf, false));
+}
+__device__ __forceinline__ int wave_min_int(int v) {
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xf, 0xf, false));   // quad_perm [1,0,3,2]
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xf, 0xf, false));   // quad_perm [2,3,0,1]
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xf, 0xf, false));  // row_half_mirror
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xf, 0xf, false));  // row_mirror
+    v = imin_step<0x142, 0xa>(v);                                     // row_bcast:15 into rows 1 and 3
+    v = imin_step<0x143, 0xc>(v);                                     // row_bcast:31 into rows 2 and 3
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Wave-wide argmin over (error, candidate index): smaller error wins, equal
+// errors (including -0 == +0) resolve to the smaller index.  The minimum
+// error is reduced alone; when one lane holds it (the usual case) that lane's
+// index is read out, and only ties reduce the tied lanes' indices.  Errors
+// here are never NaN (a lane's best only takes values below FLT_MAX; a lane
+// without a winner holds FLT_MAX and kNoWinner).
+__device__ __forceinline__ int wave_argmin(float e, int i) {
+    const float m = wave_min(e);
+    const uint64_t at = __ballot(e == m);
+    if (__popcll(at) == 1) return __builtin_amdgcn_readlane(i, __builtin_ctzll(at));
+    return wave_min_int(e == m ? i : kNoWinner);
 }
 
 // Inclusive prefix sum of v over the wave's lanes (lane l gets the sum over
@@ -860,13 +845,7 @@ __device__ __forceinline__ void encode_tile(const JobDev& job, uint32_t job_inde
     if (active) {
         if (written) st_global(uniform_ptr(job.codes + first) + out_lane, make_uint2(out.cols, out.idx));
         if (job.errors) st_global(uniform_ptr(job.errors + first) + out_lane, err);
-        if (job.stats) {  // four word stores: an int4 value would tie n into a register tuple from the start
-            int* st = reinterpret_cast<int*>(job.stats + first + out_lane);
-            st_global(st, n);
-            st_global(st + 1, n3);
-            st_global(st + 2, steps);
-            st_global(st + 3, 0);
-        }
+        if (job.stats) st_global(job.stats + first + out_lane, make_int4(n, n3, steps, 0));
     }
 }
 
