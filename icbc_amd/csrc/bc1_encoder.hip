@@ -126,56 +126,51 @@ struct TexelUnroll<TexelsLds<kWeighted>> {
 };
 
 
-// Wave-wide minimum (all lanes get it); NaN inputs must be mapped out first.
-// Which of -0 and +0 survives is unspecified (the value serves as a bound).
+// Wave-wide minima.  Floats are reduced as order-preserving unsigned keys
+// (the sign bit flipped for positive values, every bit for negative ones;
+// -0 is first mapped to +0, so that the two tie): an integer min takes its
+// DPP operand directly, where a float min also quiets its operands.  The
+// inputs must not be NaN.
+__device__ __forceinline__ uint32_t order_key(float e) {
+    const uint32_t u = __float_as_uint(e + 0.0f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// One step: the DPP source's value where the lane has one, else the min's
+// identity (all ones), so that the move folds into the min instruction.
 template <int kCtrl, int kRowMask>
-__device__ __forceinline__ float min_step(float e) {
-    return fminf(e, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(e), __float_as_int(e), kCtrl, kRowMask,
-                                                               0xf, false)));
+__device__ __forceinline__ uint32_t umin_step(uint32_t v) {
+    return min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, kCtrl, kRowMask, 0xf, false));
+}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    v = umin_step<0xB1, 0xf>(v);   // quad_perm [1,0,3,2]
+    v = umin_step<0x4E, 0xf>(v);   // quad_perm [2,3,0,1]
+    v = umin_step<0x141, 0xf>(v);  // row_half_mirror
+    v = umin_step<0x140, 0xf>(v);  // row_mirror
+    v = umin_step<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v = umin_step<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
-// Steps whose every lane has a source in its row need no old value.
-template <int kCtrl>
-__device__ __forceinline__ float min_step_in_row(float e) {
-    return fminf(e, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(e), kCtrl, 0xf, 0xf, false)));
-}
-
+// Minimum error over the wave (all lanes get it; of -0 and +0 either).
 __device__ __forceinline__ float wave_min(float e) {
-    e = min_step_in_row<0xB1>(e);
-    e = min_step_in_row<0x4E>(e);
-    e = min_step_in_row<0x141>(e);
-    e = min_step_in_row<0x140>(e);
-    e = min_step<0x142, 0xa>(e);
-    e = min_step<0x143, 0xc>(e);
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e), 63));
-}
-
-// The same reduction over ints.
-template <int kCtrl, int kRowMask>
-__device__ __forceinline__ int imin_step(int v) {
-    return min(v, __builtin_amdgcn_update_dpp(v, v, kCtrl, kRowMask, 0xf, false));
-}
-__device__ __forceinline__ int wave_min_int(int v) {
-    v = min(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xf, 0xf, false));   // quad_perm [1,0,3,2]
-    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xf, 0xf, false));   // quad_perm [2,3,0,1]
-    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xf, 0xf, false));  // row_half_mirror
-    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xf, 0xf, false));  // row_mirror
-    v = imin_step<0x142, 0xa>(v);                                     // row_bcast:15 into rows 1 and 3
-    v = imin_step<0x143, 0xc>(v);                                     // row_bcast:31 into rows 2 and 3
-    return __builtin_amdgcn_readlane(v, 63);
+    const uint32_t k = order_key(e);
+    const uint64_t at = __ballot(k == wave_min_u32(k));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e), __builtin_ctzll(at)));
 }
 
 // Wave-wide argmin over (error, candidate index): smaller error wins, equal
-// errors (including -0 == +0) resolve to the smaller index.  The minimum
-// error is reduced alone; when one lane holds it (the usual case) that lane's
-// index is read out, and only ties reduce the tied lanes' indices.  Errors
-// here are never NaN (a lane's best only takes values below FLT_MAX; a lane
-// without a winner holds FLT_MAX and kNoWinner).
+// errors (including -0 == +0) resolve to the smaller index.  The error keys
+// are reduced alone; when one lane holds the minimum (the usual case) that
+// lane's index is read out, and only ties reduce the tied lanes' indices.
+// Errors here are never NaN (a lane's best only takes values below FLT_MAX; a
+// lane without a winner holds FLT_MAX and kNoWinner).
 __device__ __forceinline__ int wave_argmin(float e, int i) {
-    const float m = wave_min(e);
-    const uint64_t at = __ballot(e == m);
+    const uint32_t k = order_key(e);
+    const uint64_t at = __ballot(k == wave_min_u32(k));
     if (__popcll(at) == 1) return __builtin_amdgcn_readlane(i, __builtin_ctzll(at));
-    return wave_min_int(e == m ? i : kNoWinner);
+    return (int)wave_min_u32(k == (uint32_t)__builtin_amdgcn_readlane((int)k, __builtin_ctzll(at)) ? (uint32_t)i
+                                                                                                 : (uint32_t)kNoWinner);
 }
 
 // Inclusive prefix sum of v over the wave's lanes (lane l gets the sum over
