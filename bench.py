@@ -230,19 +230,57 @@ def cpu_baseline(blocks, level, tcm, tcb, threads, budget_cpu_s=20.0, seed=12):
                       f"chunks of 32 blocks; {secs:.2f} s wall"}
 
 
+class ResultBuffers:
+    """Page-locked code buffers for the serving client, `per_texture` for each
+    pool texture: the server's copy kernel writes results into them in place.
+    A request takes a free buffer of its texture (or, with none free, a
+    pageable array, which the server stages) and gives it back once its
+    result has been checked."""
+
+    def __init__(self, g, counts, per_texture):
+        import threading
+        self.holder = g.PinnedArray((int(sum(counts)) * per_texture, 8), np.uint8)
+        self.free = []
+        off = 0
+        for n in counts:
+            bufs = []
+            for _ in range(per_texture):
+                bufs.append(self.holder.array[off:off + n])
+                off += n
+            self.free.append(bufs)
+        self.lock = threading.Lock()
+        self.fallbacks = 0
+
+    def take(self, k, n):
+        with self.lock:
+            if self.free[k]:
+                return self.free[k].pop(), True
+            self.fallbacks += 1
+        return np.zeros((n, 8), np.uint8), False
+
+    def give(self, k, buf, pinned):
+        if pinned:
+            with self.lock:
+                self.free[k].append(buf)
+
+    def close(self):
+        self.free = None
+        self.holder.close()
+
+
 def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64 << 30,
-                  max_inflight=3, parity=None, texels="rgba8"):
+                  max_inflight=3, parity=None, texels="rgba8", saturation_seconds=3.0, window=512):
     """Poisson request stream against the request server: textures with sides
     log-uniform in [16, 2048], level uniform in 1..9, three-colour mode on for
     half of them, block mix 40/30/20/10, stored as RGBA8 (read by the encoder
     as c / 255.0f) or, with texels="float32", as float RGBA (256 B per block).
-    Inputs sit in page-locked host memory and each request's texels cross
-    PCIe inside its latency.  First the whole
-    pool is submitted at once to find the saturation rate; then each QPS
+    Inputs and result buffers sit in page-locked host memory: each request's
+    texels cross PCIe inside its latency, and the server reads and writes them
+    in place.  Saturation is the sustained rate with `window` requests kept
+    outstanding for `saturation_seconds` (cycling over the pool); then each QPS
     fraction runs for `seconds`.  Every served result is compared with the
     same texture encoded alone, and every pool texture encoded alone is
     checked block for block against the oracle (with `parity`)."""
-    # (saturation is measured on a warm server: see the untimed passes below)
     import queue
     import threading
 
@@ -267,73 +305,91 @@ def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64
             parity.check(f"serving pool texture {k}", p["level"], f, None, (1, 1, 1), p["tcm"], p["tcm"],
                          r[0], r[1], sample=1 << 30)
     srv = g.Server(budget, max_inflight=max_inflight)
+    results = ResultBuffers(g, [p["n"] for p in pool], per_texture=16)
 
-    def submit(k, codes):
+    def submit(k):
         p = pool[k]
-        return srv.submit(p["level"], p["blocks"], None, (1, 1, 1), p["tcm"], p["tcm"], codes=codes)
+        buf, pinned = results.take(k, p["n"])
+        t = srv.submit(p["level"], p["blocks"], None, (1, 1, 1), p["tcm"], p["tcm"], codes=buf, with_errors=False)
+        return t, k, buf, pinned
 
-    # result buffers are touched before submission (a fresh np.zeros array maps
-    # its pages on first write, which would land inside the server's copy-out)
-    outs = [np.full((p["n"], 8), 0, np.uint8) for p in pool]
-    # three untimed passes first: the first launches of a server grow its slab
-    # pool, staging and tables (one-time work that a running server has done;
-    # the second pass still grows them when its launches overlap differently)
-    for _ in range(3):
-        for t in [submit(k, outs[k]) for k in range(pool_size)]:
-            srv.wait(t)
-    # saturation: the median of three timed passes (a pass's time depends on
-    # how its requests happen to be split into launches)
-    passes = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        tickets = [submit(k, outs[k]) for k in range(pool_size)]
-        for t in tickets:
-            srv.wait(t)
-        passes.append(time.perf_counter() - t0)
-    sat_s = float(np.median(passes))
-    total_blocks = sum(p["n"] for p in pool)
-    sat_qps = pool_size / sat_s
-    log(f"serving saturation: {sat_qps:.1f} requests/s, {total_blocks / sat_s / 1e6:.1f} Mblocks/s "
-        f"(pool {pool_size} textures, mean {total_blocks / pool_size:.0f} blocks)")
-
-    runs = []
-    for frac in qps_fracs:
-        rate = frac * sat_qps
+    def run(arrivals, stop_after=None):
+        """Submits requests from `arrivals` (yields a pool index when the next
+        request is due, None to stop); a waiter thread collects them in order,
+        checks each result and frees its buffer."""
         q = queue.Queue()
-        infos = []
-        bad = [0]
+        infos, bad = [], [0]
 
         def waiter():
             while True:
                 item = q.get()
                 if item is None:
                     return
-                ticket, k, codes = item
+                ticket, k, buf, pinned = item
                 _, _, info = srv.wait(ticket)
-                if not np.array_equal(codes, alone[k]):
+                if not np.array_equal(buf, alone[k]):
                     bad[0] += 1
+                results.give(k, buf, pinned)
                 infos.append(info)
 
         th = threading.Thread(target=waiter)
         th.start()
-        start = time.perf_counter()
-        t_next = start
-        arrivals = 0
-        while True:
-            t_next += rng.exponential(1.0 / rate)
-            if t_next - start > seconds:
-                break
-            while True:
-                d = t_next - time.perf_counter()
-                if d <= 0:
-                    break
-                time.sleep(d if d > 0.002 else 0)
-            k = int(rng.integers(0, pool_size))
-            codes = np.full((pool[k]["n"], 8), 0, np.uint8)  # pages mapped before submission
-            q.put((submit(k, codes), k, codes))
-            arrivals += 1
+        for k in arrivals(q):
+            q.put(submit(k))
         q.put(None)
         th.join()
+        return infos, bad[0]
+
+    # warm server: two untimed passes over the pool grow its slabs, staging
+    # and tables (one-time work that a running server has done)
+    def pool_pass(q):
+        for k in range(pool_size):
+            yield k
+    for _ in range(2):
+        run(pool_pass)
+
+    # saturation: `window` requests kept outstanding, cycling over the pool
+    order = rng.permutation(np.tile(np.arange(pool_size), 2048))
+
+    def windowed(q):
+        t0 = time.perf_counter()
+        for i, k in enumerate(order):
+            while q.qsize() >= window:
+                time.sleep(0.0002)
+            if time.perf_counter() - t0 > saturation_seconds:
+                return
+            yield int(k)
+
+    infos, bad_sat = run(windowed)
+    t_first = min(i.submit_s for i in infos)
+    t_last = max(i.done_s for i in infos)
+    sat_blocks = sum(i.block_count for i in infos)
+    sat_s = t_last - t_first
+    sat_qps = len(infos) / sat_s
+    sat_mbps = sat_blocks / sat_s / 1e6
+    log(f"serving saturation: {sat_qps:.1f} requests/s, {sat_mbps:.1f} Mblocks/s "
+        f"({len(infos)} requests over {sat_s:.2f} s, window {window}, mean "
+        f"{np.mean([i.batch_size for i in infos]):.1f} requests per launch)")
+
+    runs = []
+    for frac in qps_fracs:
+        rate = frac * sat_qps
+
+        def poisson(q):
+            start = time.perf_counter()
+            t_next = start
+            while True:
+                t_next += rng.exponential(1.0 / rate)
+                if t_next - start > seconds:
+                    return
+                while True:
+                    d = t_next - time.perf_counter()
+                    if d <= 0:
+                        break
+                    time.sleep(d if d > 0.002 else 0)
+                yield int(rng.integers(0, pool_size))
+
+        infos, bad = run(poisson)
         lat = np.array([i.done_s - i.submit_s for i in infos]) * 1e3
         blocks = sum(i.block_count for i in infos)
         span = max(i.done_s for i in infos) - min(i.submit_s for i in infos)
@@ -343,18 +399,23 @@ def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64
                      "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
                      "latency_ms_max": round(float(lat.max()), 3),
                      "mean_batch_requests": round(float(np.mean([i.batch_size for i in infos])), 2),
-                     "results_identical_to_alone": bad[0] == 0})
+                     "results_identical_to_alone": bad == 0})
         log(f"serving @ {rate:.1f} qps: {runs[-1]}")
     srv.close()
+    fallbacks = results.fallbacks
+    results.close()
     for p in pool:
         p["holder"].close()
     return {"saturation_requests_per_s": round(sat_qps, 2),
-            "saturation_mblocks_per_s": round(total_blocks / sat_s / 1e6, 3),
-            "pool_textures": pool_size, "mean_blocks_per_request": round(total_blocks / pool_size, 1),
+            "saturation_mblocks_per_s": round(sat_mbps, 3),
+            "saturation_results_identical_to_alone": bad_sat == 0,
+            "saturation_window_requests": window,
+            "pool_textures": pool_size, "mean_blocks_per_request": round(sum(p["n"] for p in pool) / pool_size, 1),
             "memory_budget_bytes": budget, "max_inflight_launches": max_inflight,
-            "seconds_per_rate": seconds, "runs": runs,
+            "seconds_per_rate": seconds, "runs": runs, "pageable_result_fallbacks": fallbacks,
             "input": ("RGBA8 texels (64 B/block)" if texels == "rgba8" else "float32 RGBA texels (256 B/block)")
-                     + " in page-locked host memory, copied per request"}
+                     + " and codes (8 B/block) in page-locked host memory, read and written by the server "
+                       "in place"}
 
 
 def drop_in_per_block(g, synth, thread_counts, level=8, repeats=3, parity=None):
@@ -565,6 +626,8 @@ def main():
                     help="run only the serving benchmark: Poisson arrivals at 50/80/95%% of saturation, 60 s each")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the short serving run included in the default benchmark")
+    ap.add_argument("--serve-profile-seconds", type=float, default=60.0,
+                    help="length of each load level of the --serve profile")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-drop-in", action="store_true",
                     help="skip the per-block icbc::compress_bc1 run from worker threads")
@@ -618,8 +681,9 @@ def main():
         log(f"calibration copy: {nb} bytes device to device")
 
     if args.serve:
-        res = serving_bench(g, synth, 60.0, (0.5, 0.8, 0.95), parity=parity)
-        res_f = serving_bench(g, synth, 60.0, (0.8,), parity=parity, texels="float32")
+        sec = args.serve_profile_seconds
+        res = serving_bench(g, synth, sec, (0.5, 0.8, 0.95), parity=parity)
+        res_f = serving_bench(g, synth, sec, (0.8,), parity=parity, texels="float32")
         top = res["runs"][-1]
         finish({"metric": "BC1 serving: Mblocks/s and p50/p99 request latency under Poisson load",
                 "value": top["mblocks_per_s"], "unit": "Mblocks/s", "n_gpus": 1,

@@ -732,15 +732,17 @@ class Server:
 
     def submit(self, level, colors, weights=None, color_weights=(1.0, 1.0, 1.0),
                three_color_mode=False, three_color_black=False, codes=None, errors=None,
-               tile_blocks=0):
+               tile_blocks=0, with_errors=True):
         """Queues one texture (block-major host arrays, float32 or uint8 RGBA).
-        Returns a ticket."""
+        Output arrays in page-locked memory (PinnedArray) are written in place
+        by the device; others are staged.  Returns a ticket."""
         rgba8 = colors.dtype == np.uint8
         colors = np.ascontiguousarray(colors, dtype=np.uint8 if rgba8 else np.float32)
         n = colors.shape[0]
         w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
         codes = np.zeros((n, 8), np.uint8) if codes is None else codes
-        errors = np.zeros(n, np.float32) if errors is None else errors
+        if errors is None and with_errors:
+            errors = np.zeros(n, np.float32)
         job = make_job(level, n, colors, codes, w, errors, None, color_weights, three_color_mode,
                        three_color_black, host=True, tile_blocks=tile_blocks, rgba8=rgba8)
         t = C.c_uint64()

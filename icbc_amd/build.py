@@ -29,8 +29,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libicbc_amd.so")
-SOURCES = ["bc1_encoder.hip", "texture_prepass.hip", "icbc_runtime.cpp", "icbc_server.cpp", "containers.cpp",
-           "autotune.cpp", "block_ring.cpp"]
+SOURCES = ["bc1_encoder.hip", "texture_prepass.hip", "transfer.hip", "icbc_runtime.cpp", "icbc_server.cpp",
+           "containers.cpp", "autotune.cpp", "block_ring.cpp"]
 ARCH = "gfx950"
 
 HIPCC_FLAGS = [

@@ -40,6 +40,7 @@
 #include "../../include/icbc_gpu.h"
 #include "bc1_kernels.h"
 #include "icbc_internal.h"
+#include "transfer.h"
 
 using namespace icbc_gpu;
 
@@ -55,6 +56,12 @@ size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 // Fewest resident workgroups a server launches with (the workspace a small
 // budget must still hold).
 constexpr uint64_t kMinServerGrid = 64;
+
+// Blocks merged into one launch at most: a million blocks fill every resident
+// workgroup of the encoder many times over (1-3 ms at levels 2-9), so larger
+// launches gain no throughput, only latency for their first requests and
+// staging memory for pageable buffers.
+constexpr uint64_t kMaxLaunchBlocks = 1u << 20;
 
 // Device memory the runtime itself holds for each launch slot's stream (its
 // queue and signal pools; 4-5 MiB measured on MI355X), charged to the budget.
@@ -188,9 +195,11 @@ private:
     std::vector<void*> raw_;
 };
 
-// Page-locked host staging that grows on demand.
+// Page-locked host staging that grows on demand, with its mapped device
+// address (the copy kernels read and write it directly).
 struct Staging {
     void* p = nullptr;
+    void* dp = nullptr;
     size_t cap = 0;
     ~Staging() {
         if (p) (void)hipHostFree(p);
@@ -198,23 +207,31 @@ struct Staging {
     bool reserve(size_t bytes) {
         if (bytes <= cap) return true;
         if (p) (void)hipHostFree(p);
-        p = nullptr;
+        p = dp = nullptr;
         cap = 0;
         const size_t want = std::max<size_t>(bytes + bytes / 2, 1 << 20);
         if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) return false;
+        if (hipHostGetDevicePointer(&dp, p, 0) != hipSuccess) {
+            (void)hipHostFree(p);
+            p = dp = nullptr;
+            return false;
+        }
         cap = want;
         return true;
     }
 };
 
-// True when `p` is page-locked host memory the DMA engines can read directly.
-bool is_pinned(const void* p) {
+// The device-accessible address of `p` when it is page-locked host memory
+// (mapped into the device's address space), else null: such buffers are read
+// and written by the copy kernels in place, others go through staging.
+void* mapped_address(const void* p) {
+    if (!p) return nullptr;
     hipPointerAttribute_t a;
     if (hipPointerGetAttributes(&a, p) != hipSuccess) {
         (void)hipGetLastError();
-        return false;
+        return nullptr;
     }
-    return a.type == hipMemoryTypeHost;
+    return a.type == hipMemoryTypeHost ? a.devicePointer : nullptr;
 }
 
 enum State { kPending, kAdmitted, kLaunched, kDone };
@@ -225,6 +242,12 @@ struct Request {
     Layout lay;
     size_t cls = 0;      // pool class (also the budget charge)
     void* dev = nullptr;
+    // mapped addresses of the caller's buffers (null: pageable, staged)
+    const void* map_colors = nullptr;
+    const void* map_weights = nullptr;
+    void* map_codes = nullptr;
+    void* map_errors = nullptr;
+    void* map_stats = nullptr;
     State state = kPending;
     int status = ICBC_OK;
     uint32_t batch_size = 0;
@@ -235,6 +258,7 @@ struct Request {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
+    hipEvent_t uploaded = nullptr;  // this launch's inputs are on the device (upload stream)
     void* work = nullptr;  // queue counters + workspace slices (charged at creation)
     // this launch's job table + tile list and level-9 handoff list: pool
     // blocks taken with the batch, given back when it completes
@@ -244,10 +268,10 @@ struct Slot {
     size_t refine_cls = 0;
     std::vector<uint32_t> tile_blocks;  // per request of the batch
     uint64_t ntiles = 0;
-    Staging host_table;  // pinned staging for the table
+    Staging host_table;  // pinned staging for the table and the copy lists
     Staging in;   // inputs of requests whose buffers are pageable
-    Staging out;  // codes / errors / stats on their way back
-    std::vector<size_t> out_off;  // per request: offset of its outputs in `out`
+    Staging out;  // outputs of requests whose buffers are pageable
+    std::vector<size_t> out_off;  // per request: offset of its staged outputs in `out`
     std::vector<Request*> reqs;
     bool busy = false;
 };
@@ -266,6 +290,13 @@ struct icbc_server {
     std::unordered_map<uint64_t, Request*> all;
     Pool pool;
     uint32_t grid = 0;  // resident workgroups of a launch (its workspace size)
+    uint32_t copy_grid = 0;  // workgroups of a copy kernel
+    uint64_t slot_bytes = 0;  // device bytes charged per launch slot (fixed memory and workspace)
+    // Every launch's table copy and input copy kernel run in order on this one
+    // stream; the launch's encode waits for them on its slot's stream.  So the
+    // next launch's inputs cross PCIe while the current one encodes, and
+    // uploads never compete with each other for PCIe.
+    hipStream_t upload = nullptr;
     uint64_t used = 0;  // bytes of pool blocks taken (requests and launch tables)
     bool hold_admission = false;  // a formed batch waits for its launch tables' room
     uint64_t next_id = 1;
@@ -292,8 +323,24 @@ int set_fatal(icbc_server* sv, int code) {
 
 }  // namespace
 
-// Device footprint of one launch besides its requests' buffers: the job table
-// and tile list, and the level-9 handoff list.
+// A launch's table block: job table, tile list, and the copy lists in (each
+// request's colours, weights and prior codes) and out (codes, errors,
+// counters), three segments per request each way.
+constexpr size_t kSegsPerRequest = 3;
+struct TableLayout {
+    size_t tiles = 0, segs_in = 0, segs_out = 0, total = 0;
+};
+TableLayout table_layout(size_t requests, uint64_t tiles) {
+    TableLayout t;
+    t.tiles = align_up(requests * sizeof(JobDev), 256);
+    t.segs_in = t.tiles + align_up(std::max<uint64_t>(1, tiles) * sizeof(uint2), 256);
+    t.segs_out = t.segs_in + requests * kSegsPerRequest * sizeof(CopySeg);
+    t.total = t.segs_out + requests * kSegsPerRequest * sizeof(CopySeg);
+    return t;
+}
+
+// Device footprint of one launch besides its requests' buffers: the table
+// block above and the level-9 handoff list.
 struct LaunchNeeds {
     std::vector<uint32_t> tile_blocks;
     uint64_t tiles = 0;
@@ -311,7 +358,7 @@ LaunchNeeds launch_needs(const std::vector<Request*>& reqs, uint32_t grid) {
         n.tiles += (b + t - 1) / t;
         if (r->job.level == 9) level9 += b;
     }
-    n.table_bytes = align_up(reqs.size() * sizeof(JobDev), 256) + std::max<uint64_t>(1, n.tiles) * sizeof(uint2);
+    n.table_bytes = table_layout(reqs.size(), n.tiles).total;
     n.refine_bytes = level9 ? refine_list_bytes(level9) : 0;
     return n;
 }
@@ -356,12 +403,44 @@ void icbc_server::release_launch_resources(Slot& s) {
 }
 
 int icbc_server::launch(Slot& s, std::vector<Request*>& reqs) {
-    // device job table and tile list for the batch, in the slot's table block
+    // pageable inputs are staged in page-locked memory first; then one table
+    // copy (jobs, tiles, copy lists), one copy kernel in, the encode, one copy
+    // kernel out, and the completion event, all on the slot's stream
+    size_t in_bytes = 0, out_bytes = 0;
+    for (Request* r : reqs) {
+        const uint64_t n = r->job.block_count;
+        if (!r->map_colors) in_bytes += align_up(r->lay.color_bytes, 256);
+        if (r->job.weights && !r->map_weights) in_bytes += align_up(n * 64, 256);
+        if (!r->map_codes) in_bytes += align_up(n * 8, 256);
+        if (!r->map_codes) out_bytes += align_up(n * 8, 256);
+        if (r->job.errors && !r->map_errors) out_bytes += align_up(n * 4, 256);
+        if (r->job.stats && !r->map_stats) out_bytes += align_up(n * 16, 256);
+    }
+    if (!s.in.reserve(std::max<size_t>(in_bytes, 256)) || !s.out.reserve(std::max<size_t>(out_bytes, 256)))
+        return ICBC_ERR_OUT_OF_MEMORY;
     std::vector<JobDev> dev;
+    std::vector<CopySeg> in, out;
+    size_t in_off = 0, out_off = 0;
+    s.out_off.assign(reqs.size(), 0);
+    // a caller buffer's device-side address: mapped in place, or staged
+    auto stage_in = [&](const void* mapped, const void* src, size_t bytes) -> const void* {
+        if (mapped) return mapped;
+        memcpy((char*)s.in.p + in_off, src, bytes);
+        const void* d = (const char*)s.in.dp + in_off;
+        in_off += align_up(bytes, 256);
+        return d;
+    };
+    auto stage_out = [&](void* mapped, size_t bytes) -> void* {
+        if (mapped) return mapped;
+        void* d = (char*)s.out.dp + out_off;
+        out_off += align_up(bytes, 256);
+        return d;
+    };
     for (size_t k = 0; k < reqs.size(); k++) {
         Request* r = reqs[k];
-        icbc_bc1_job dj = r->job;
+        const uint64_t n = r->job.block_count;
         char* base = (char*)r->dev;
+        icbc_bc1_job dj = r->job;
         dj.flags = r->job.flags & ICBC_JOB_COLORS_RGBA8;  // RGBA8 texels are read as they are
         dj.colors = (const float*)(base + r->lay.colors);
         dj.weights = r->job.weights ? (const float*)(base + r->lay.weights) : nullptr;
@@ -371,52 +450,47 @@ int icbc_server::launch(Slot& s, std::vector<Request*>& reqs) {
         JobDev d = job_to_device(dj, decoder);
         d.tile_blocks = (int32_t)s.tile_blocks[k];
         dev.push_back(d);
+        // in: colours, weights, prior codes (blocks the encoder leaves
+        // untouched keep them); out: codes, errors, counters
+        in.push_back({stage_in(r->map_colors, r->job.colors, r->lay.color_bytes), base + r->lay.colors,
+                      n ? r->lay.color_bytes : 0, 0});
+        in.push_back({r->job.weights ? stage_in(r->map_weights, r->job.weights, n * 64) : nullptr,
+                      base + r->lay.weights, r->job.weights ? n * 64 : 0, 0});
+        in.push_back({stage_in(r->map_codes, r->job.codes, n * 8), base + r->lay.codes, n * 8, 0});
+        s.out_off[k] = out_off;
+        out.push_back({base + r->lay.codes, stage_out(r->map_codes, n * 8), n * 8, 0});
+        out.push_back({base + r->lay.errors, r->job.errors ? stage_out(r->map_errors, n * 4) : nullptr,
+                       r->job.errors ? n * 4 : 0, 0});
+        out.push_back({base + r->lay.stats, r->job.stats ? stage_out(r->map_stats, n * 16) : nullptr,
+                       r->job.stats ? n * 16 : 0, 0});
     }
+    const uint64_t chunks_in = number_copy_segments(in.data(), (uint32_t)in.size());
+    const uint64_t chunks_out = number_copy_segments(out.data(), (uint32_t)out.size());
     std::vector<uint2> tiles = tiles_for(dev, true);
     if (tiles.size() != s.ntiles) return ICBC_ERR_DEVICE;  // the table block was sized for s.ntiles
-    const size_t jbytes = align_up(dev.size() * sizeof(JobDev), 256);
-    const size_t need = jbytes + std::max<size_t>(1, tiles.size()) * sizeof(uint2);
-    if (!s.host_table.reserve(need)) return ICBC_ERR_OUT_OF_MEMORY;
-    memcpy(s.host_table.p, dev.data(), dev.size() * sizeof(JobDev));
-    memcpy((char*)s.host_table.p + jbytes, tiles.data(), tiles.size() * sizeof(uint2));
-    if (hipMemcpyAsync(s.table, s.host_table.p, need, hipMemcpyHostToDevice, s.stream) != hipSuccess)
+    const TableLayout tl = table_layout(reqs.size(), tiles.size());
+    if (!s.host_table.reserve(tl.total)) return ICBC_ERR_OUT_OF_MEMORY;
+    char* ht = (char*)s.host_table.p;
+    memcpy(ht, dev.data(), dev.size() * sizeof(JobDev));
+    memcpy(ht + tl.tiles, tiles.data(), tiles.size() * sizeof(uint2));
+    memcpy(ht + tl.segs_in, in.data(), in.size() * sizeof(CopySeg));
+    memcpy(ht + tl.segs_out, out.data(), out.size() * sizeof(CopySeg));
+    char* dt = (char*)s.table;
+    if (hipMemcpyAsync(dt, ht, tl.total, hipMemcpyHostToDevice, upload) != hipSuccess ||
+        launch_copy_segments((const CopySeg*)(dt + tl.segs_in), (uint32_t)in.size(), chunks_in, copy_grid, upload) !=
+            hipSuccess ||
+        hipEventRecord(s.uploaded, upload) != hipSuccess || hipStreamWaitEvent(s.stream, s.uploaded, 0) != hipSuccess)
         return ICBC_ERR_DEVICE;
     const RefineList rl = s.refine ? RefineList{(RefineItem*)((char*)s.refine + kRefineCounterBytes), (uint32_t*)s.refine}
                                    : RefineList{nullptr, nullptr};
-    if (launch_encode((const JobDev*)s.table, (const uint2*)((char*)s.table + jbytes), (uint32_t)tiles.size(),
-                      (uint32_t*)s.work, (float4*)((char*)s.work + kQueueBytes), grid, encode_mode(dev.data(), dev.size()),
-                      rl, s.stream) != hipSuccess)
+    if (launch_encode((const JobDev*)dt, (const uint2*)(dt + tl.tiles), (uint32_t)tiles.size(), (uint32_t*)s.work,
+                      (float4*)((char*)s.work + kQueueBytes), grid, encode_mode(dev.data(), dev.size()), rl,
+                      s.stream) != hipSuccess)
         return ICBC_ERR_DEVICE;
-    // results go to pinned staging (a copy into pageable memory would block
-    // this thread until the kernel ends); the reaper hands them over
-    size_t out_bytes = 0;
-    s.out_off.assign(reqs.size(), 0);
-    for (size_t k = 0; k < reqs.size(); k++) {
-        s.out_off[k] = out_bytes;
-        const uint64_t n = reqs[k]->job.block_count;
-        out_bytes += align_up(n * 8, 256) + (reqs[k]->job.errors ? align_up(n * 4, 256) : 0) +
-                     (reqs[k]->job.stats ? align_up(n * 16, 256) : 0);
-    }
-    if (!s.out.reserve(std::max<size_t>(out_bytes, 256))) return ICBC_ERR_OUT_OF_MEMORY;
-    for (size_t k = 0; k < reqs.size(); k++) {
-        Request* r = reqs[k];
-        const uint64_t n = r->job.block_count;
-        char* base = (char*)r->dev;
-        char* dst = (char*)s.out.p + s.out_off[k];
-        if (n == 0) continue;
-        if (hipMemcpyAsync(dst, base + r->lay.codes, n * 8, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
-            return ICBC_ERR_DEVICE;
-        dst += align_up(n * 8, 256);
-        if (r->job.errors) {
-            if (hipMemcpyAsync(dst, base + r->lay.errors, n * 4, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
-                return ICBC_ERR_DEVICE;
-            dst += align_up(n * 4, 256);
-        }
-        if (r->job.stats &&
-            hipMemcpyAsync(dst, base + r->lay.stats, n * 16, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
-            return ICBC_ERR_DEVICE;
-    }
-    if (hipEventRecord(s.done, s.stream) != hipSuccess) return ICBC_ERR_DEVICE;
+    if (launch_copy_segments((const CopySeg*)(dt + tl.segs_out), (uint32_t)out.size(), chunks_out, copy_grid,
+                             s.stream) != hipSuccess ||
+        hipEventRecord(s.done, s.stream) != hipSuccess)
+        return ICBC_ERR_DEVICE;
     return ICBC_OK;
 }
 
@@ -474,6 +548,15 @@ void icbc_server::run_scheduler() {
         if (!admitted.empty() && free_slot && fatal == ICBC_OK) {
             std::vector<Request*> batch;
             size_t take = opt.max_batch ? std::min<size_t>(opt.max_batch, admitted.size()) : admitted.size();
+            // and at most kMaxLaunchBlocks blocks (a larger request runs alone)
+            uint64_t blocks = admitted[0]->job.block_count;
+            for (size_t k = 1; k < take; k++) {
+                blocks += admitted[k]->job.block_count;
+                if (blocks > kMaxLaunchBlocks) {
+                    take = k;
+                    break;
+                }
+            }
             // the launch's table and handoff list come from the budget too: a
             // batch they do not fit in is halved until they do
             bool ok = false;
@@ -525,46 +608,8 @@ void icbc_server::run_scheduler() {
             free_slot->busy = true;
             free_slot->reqs = batch;
             lk.unlock();
-            int r = ICBC_OK;
-            // upload inputs (and prior codes: blocks the encoder leaves
-            // untouched keep them); pageable buffers go through pinned staging
-            size_t in_bytes = 0;
-            std::vector<char> pinned(batch.size() * 3, 0);
-            for (size_t k = 0; k < batch.size(); k++) {
-                const icbc_bc1_job& j = batch[k]->job;
-                const uint64_t n = j.block_count;
-                if (n == 0) continue;
-                pinned[3 * k + 0] = is_pinned(j.colors);
-                pinned[3 * k + 1] = j.weights ? is_pinned(j.weights) : 1;
-                pinned[3 * k + 2] = is_pinned(j.codes);
-                if (!pinned[3 * k + 0]) in_bytes += align_up(batch[k]->lay.color_bytes, 256);
-                if (!pinned[3 * k + 1]) in_bytes += align_up(n * 64, 256);
-                if (!pinned[3 * k + 2]) in_bytes += align_up(n * 8, 256);
-            }
-            if (!free_slot->in.reserve(std::max<size_t>(in_bytes, 256))) r = ICBC_ERR_OUT_OF_MEMORY;
-            size_t in_off = 0;
-            auto up = [&](void* dev_dst, const void* src, size_t bytes, bool src_pinned) {
-                const void* from = src;
-                if (!src_pinned) {
-                    char* st = (char*)free_slot->in.p + in_off;
-                    memcpy(st, src, bytes);
-                    in_off += align_up(bytes, 256);
-                    from = st;
-                }
-                return hipMemcpyAsync(dev_dst, from, bytes, hipMemcpyHostToDevice, free_slot->stream) == hipSuccess;
-            };
-            for (size_t k = 0; k < batch.size() && r == ICBC_OK; k++) {
-                Request* q = batch[k];
-                const uint64_t n = q->job.block_count;
-                char* base = (char*)q->dev;
-                if (n == 0) continue;
-                if (!up(base + q->lay.colors, q->job.colors, q->lay.color_bytes, pinned[3 * k + 0]) ||
-                    (q->job.weights && !up(base + q->lay.weights, q->job.weights, n * 64, pinned[3 * k + 1])) ||
-                    !up(base + q->lay.codes, q->job.codes, n * 8, pinned[3 * k + 2]))
-                    r = ICBC_ERR_DEVICE;
-            }
             const double t = now_s();
-            if (r == ICBC_OK) r = launch(*free_slot, batch);
+            const int r = launch(*free_slot, batch);
             lk.lock();
             for (Request* q : batch) {
                 q->state = kLaunched;
@@ -596,18 +641,22 @@ void icbc_server::run_reaper() {
         lk.unlock();
         const hipError_t e = hipEventSynchronize(s->done);
         if (e == hipSuccess) {
+            // outputs of pageable buffers wait in staging (mapped ones are
+            // already in place)
             for (size_t k = 0; k < s->reqs.size(); k++) {
                 Request* r = s->reqs[k];
                 const uint64_t n = r->job.block_count;
                 if (n == 0 || r->status != ICBC_OK) continue;
                 const char* src = (const char*)s->out.p + s->out_off[k];
-                memcpy(r->job.codes, src, n * 8);
-                src += align_up(n * 8, 256);
-                if (r->job.errors) {
+                if (!r->map_codes) {
+                    memcpy(r->job.codes, src, n * 8);
+                    src += align_up(n * 8, 256);
+                }
+                if (r->job.errors && !r->map_errors) {
                     memcpy(r->job.errors, src, n * 4);
                     src += align_up(n * 4, 256);
                 }
-                if (r->job.stats) memcpy(r->job.stats, src, n * 16);
+                if (r->job.stats && !r->map_stats) memcpy(r->job.stats, src, n * 16);
             }
         }
         const double t = now_s();
@@ -648,28 +697,43 @@ int icbc_server_create(const icbc_server_options* opt, icbc_server** out) {
     icbc_server_options o = *opt;
     if (o.max_inflight == 0) o.max_inflight = 3;
     if (o.max_inflight > 8) o.max_inflight = 8;
-    // the launch slots (stream allowance and workspace) take at most half
-    // the budget: as many resident workgroups as that allows, up to the
-    // device's residency, and no fewer than kMinServerGrid
+    // fixed device memory of a slot besides its workspace: its stream's
+    // runtime allocations
+    const uint64_t slot_fixed = kStreamAllowance;
+    // the launch slots (fixed memory and workspace) take at most half the
+    // budget: as many resident workgroups as that allows, up to the
+    // device's residency, and no fewer than kMinServerGrid (the upload
+    // stream's allowance comes out of the other half)
     const size_t per_wg = kWorkspaceBytesPerWorkgroup;
     const uint64_t share = o.memory_budget / 2 / o.max_inflight / kGranule * kGranule;  // per slot
-    const uint64_t fit = share > kQueueBytes + kStreamAllowance ? (share - kQueueBytes - kStreamAllowance) / per_wg : 0;
+    const uint64_t fit = share > kQueueBytes + slot_fixed ? (share - kQueueBytes - slot_fixed) / per_wg : 0;
     if (fit < kMinServerGrid)
         return icbc_internal_fail(ICBC_ERR_BUDGET, "memory budget %llu B is below the %llu B that %u launch slots "
                                   "need", (unsigned long long)o.memory_budget,
                                   (unsigned long long)(2ull * o.max_inflight *
-                                                       (kStreamAllowance +
+                                                       (slot_fixed +
                                                         align_up(kQueueBytes + kMinServerGrid * per_wg, kGranule))),
                                   o.max_inflight);
     icbc_server* sv = new icbc_server(o.memory_budget);
     sv->opt = o;
     sv->decoder = decoder;
-    sv->grid = (uint32_t)std::min<uint64_t>(grid, fit);
+    // one workgroup slot per compute unit is left to the copy kernels, so that
+    // a launch's copies can start beside a running encode kernel
+    const uint32_t cus = std::max<uint32_t>(1, grid / (uint32_t)encode_workgroups_per_cu());
+    sv->grid = (uint32_t)std::min<uint64_t>(grid > cus ? grid - cus : grid, fit);
+    sv->copy_grid = cus;
+    sv->slot_bytes = slot_fixed + align_up(kQueueBytes + (uint64_t)sv->grid * per_wg, kGranule);
     sv->slots.resize(sv->opt.max_inflight);
+    if (hipStreamCreateWithFlags(&sv->upload, hipStreamNonBlocking) != hipSuccess ||
+        !sv->pool.charge(kStreamAllowance)) {
+        icbc_server_destroy(sv);
+        return icbc_internal_fail(ICBC_ERR_OUT_OF_MEMORY, "server upload stream creation failed");
+    }
     for (Slot& s : sv->slots) {
         if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
-            !sv->pool.charge(kStreamAllowance) ||
+            hipEventCreateWithFlags(&s.uploaded, hipEventDisableTiming) != hipSuccess ||
+            !sv->pool.charge(slot_fixed) ||
             sv->pool.alloc_raw(kQueueBytes + (size_t)sv->grid * per_wg, &s.work) != hipSuccess) {
             icbc_server_destroy(sv);
             return icbc_internal_fail(ICBC_ERR_OUT_OF_MEMORY, "server launch slot allocation failed");
@@ -691,6 +755,11 @@ int icbc_server_submit(icbc_server* sv, const icbc_bc1_job* job, uint64_t* ticke
     q->job = *job;
     q->lay = layout_of(*job);
     q->cls = Pool::class_of(q->lay.total);
+    q->map_colors = mapped_address(job->colors);
+    q->map_weights = mapped_address(job->weights);
+    q->map_codes = mapped_address(job->codes);
+    q->map_errors = mapped_address(job->errors);
+    q->map_stats = mapped_address(job->stats);
     q->t_submit = now_s();
     std::lock_guard<std::mutex> lk(sv->m);
     if (sv->fatal) {
@@ -708,9 +777,7 @@ int icbc_server_submit(icbc_server* sv, const icbc_bc1_job* job, uint64_t* ticke
         const LaunchNeeds n = launch_needs(one, sv->grid);
         const uint64_t need = q->cls + Pool::class_of(n.table_bytes) +
                               (n.refine_bytes ? Pool::class_of(n.refine_bytes) : 0);
-        const uint64_t room = sv->opt.memory_budget -
-                              sv->slots.size() * (kStreamAllowance +
-                                                  align_up(kQueueBytes + (uint64_t)sv->grid * kWorkspaceBytesPerWorkgroup, kGranule));
+        const uint64_t room = sv->opt.memory_budget - kStreamAllowance - sv->slots.size() * sv->slot_bytes;
         if (need > room) {
             delete q;
             return icbc_internal_fail(ICBC_ERR_BUDGET, "request needs %llu bytes, the budget leaves %llu for requests",
@@ -800,7 +867,12 @@ int icbc_server_destroy(icbc_server* sv) {
     for (Slot& s : sv->slots) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
         if (s.done) (void)hipEventDestroy(s.done);
+        if (s.uploaded) (void)hipEventDestroy(s.uploaded);
         if (s.stream) (void)hipStreamDestroy(s.stream);
+    }
+    if (sv->upload) {
+        (void)hipStreamSynchronize(sv->upload);
+        (void)hipStreamDestroy(sv->upload);
     }
     for (auto& kv : sv->all) delete kv.second;
     delete sv;

@@ -77,10 +77,15 @@ def _warm_runtime(gpu):
     streams = [gpu.Stream() for _ in range(8)]
     try:
         for s in streams:
-            for lv, wt in ((1, None), (1, w), (8, None), (9, w)):
-                codes = np.zeros((len(colors), 8), np.uint8)
-                gpu.compress_batch([gpu.make_job(lv, len(colors), colors, codes, wt, None, None, (1, 1, 1), True, True,
-                                                 host=True)], stream=s)
+            # one launch per set of levels: the level 0-1 instance, the cluster
+            # instances without and with the box-fit stage (a launch mixing
+            # levels 0-3 with 4-9 runs the latter, which spills most), and the
+            # refinement kernel
+            for lvs, wt in (((1,), None), ((1,), w), ((8,), None), ((9,), w), ((2, 8), None), ((3, 9), w)):
+                outs = [np.zeros((len(colors), 8), np.uint8) for _ in lvs]
+                jobs = [gpu.make_job(lv, len(colors), colors, out, wt, None, None, (1, 1, 1), True, True, host=True)
+                        for lv, out in zip(lvs, outs)]
+                gpu.compress_batch(jobs, stream=s)
             s.synchronize()
     finally:
         for s in streams:
@@ -173,7 +178,7 @@ def test_oversize_request_rejected(gpu):
 def test_cancel_pending_request(gpu):
     # a budget that holds one request at a time keeps the later ones pending
     blocks, _ = synth.make_image_blocks(3, 1024, 1024, synth.MIX_DEFAULT)   # 65536 blocks, ~17 MiB
-    srv = gpu.Server(memory_budget=80 << 20, max_inflight=1)
+    srv = gpu.Server(memory_budget=88 << 20, max_inflight=1)
     try:
         t1 = srv.submit(9, blocks, None, (1, 1, 1), True, True)
         t2 = srv.submit(9, blocks, None, (1, 1, 1), True, True)
@@ -250,6 +255,51 @@ def test_identity_under_load(gpu):
         srv.close()
         for p in pinned:
             p.close()
+
+
+def test_mapped_and_staged_buffers(gpu, oracle):
+    # one batch mixing page-locked buffers (read and written in place by the
+    # copy kernels through their device mapping) with pageable ones (staged),
+    # at 16-byte aligned and unaligned addresses, with odd block counts
+    reqs = _requests(12, 24, max_side=160)
+    rng = np.random.default_rng(12)
+    holders = []
+
+    def place(a, pinned, shift):
+        if not pinned:
+            return a.copy()
+        h = gpu.PinnedArray((a.nbytes + 16,), np.uint8)
+        holders.append(h)
+        v = h.array[shift:shift + a.nbytes].view(a.dtype).reshape(a.shape)
+        v[...] = a
+        return v
+
+    srv = gpu.Server(memory_budget=96 << 20, max_inflight=2)
+    try:
+        tickets = []
+        for k, (lv, b, w, tcm) in enumerate(reqs):
+            if k % 3 == 0 and len(b) % 2 == 0:  # an odd block count
+                b = b[: len(b) - 1]
+                w = None if w is None else w[: len(b)]
+            pin = rng.integers(0, 2, size=4).astype(bool)
+            shift = int(rng.choice([0, 4, 8]))
+            n = len(b)
+            colors = place(np.ascontiguousarray(b, np.float32), pin[0], shift)
+            wts = None if w is None else place(np.ascontiguousarray(w, np.float32), pin[1], shift)
+            codes = place(np.zeros((n, 8), np.uint8), pin[2], shift)
+            errs = place(np.zeros(n, np.float32), pin[3], shift)
+            tickets.append((srv.submit(lv, colors, wts, (1, 1, 1), tcm, tcm, codes=codes, errors=errs),
+                            lv, np.array(colors), None if wts is None else np.array(wts), tcm))
+        for t, lv, b, w, tcm in tickets:
+            codes, errs, info = srv.wait(t)
+            assert info.status == 0
+            want = _oracle(oracle, lv, b, w, tcm)
+            assert np.array_equal(codes, want[0]), f"request {t} codes"
+            assert np.array_equal(errs.view(np.uint32), want[1].view(np.uint32)), f"request {t} errors"
+    finally:
+        srv.close()
+        for h in holders:
+            h.close()
 
 
 def test_two_waits_on_one_ticket(gpu):
