@@ -167,7 +167,7 @@ typedef struct icbc_server icbc_server;
 typedef struct icbc_server_options {
     uint64_t memory_budget; /* every device byte the server holds: request buffers, launch
                                tables and level-9 lists, the launch slots' workspaces and an
-                               allowance for each slot's stream and the upload stream (the
+                               allowance for each slot's stream and the two upload streams (the
                                slots take at most half of it; at least 20 MiB per slot) */
     uint32_t max_inflight;  /* launches overlapping on the device (0 = 3, at most 8) */
     uint32_t max_batch;     /* requests merged into one launch (0 = no limit); a launch also

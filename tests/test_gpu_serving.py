@@ -165,7 +165,7 @@ def test_oversize_request_rejected(gpu):
     with pytest.raises(gpu.IcbcError) as e:
         gpu.Server(memory_budget=1 << 20)
     assert e.value.code == gpu.ERR_BUDGET
-    srv = gpu.Server(memory_budget=24 << 20, max_inflight=1)
+    srv = gpu.Server(memory_budget=32 << 20, max_inflight=1)
     try:
         big = np.zeros((131072, 16, 4), np.float32)   # 32 MiB of texels
         with pytest.raises(gpu.IcbcError) as e:
@@ -178,7 +178,7 @@ def test_oversize_request_rejected(gpu):
 def test_cancel_pending_request(gpu):
     # a budget that holds one request at a time keeps the later ones pending
     blocks, _ = synth.make_image_blocks(3, 1024, 1024, synth.MIX_DEFAULT)   # 65536 blocks, ~17 MiB
-    srv = gpu.Server(memory_budget=88 << 20, max_inflight=1)
+    srv = gpu.Server(memory_budget=112 << 20, max_inflight=1)
     try:
         t1 = srv.submit(9, blocks, None, (1, 1, 1), True, True)
         t2 = srv.submit(9, blocks, None, (1, 1, 1), True, True)
@@ -196,7 +196,7 @@ def test_cancel_pending_request(gpu):
 
 def test_invalid_level_request(gpu):
     blocks, _ = synth.make_image_blocks(4, 32, 32, synth.MIX_DEFAULT)
-    srv = gpu.Server(memory_budget=24 << 20, max_inflight=1)
+    srv = gpu.Server(memory_budget=32 << 20, max_inflight=1)
     try:
         prior = np.full((len(blocks), 8), 0xab, np.uint8)
         t = srv.submit(11, blocks, codes=prior)
@@ -262,6 +262,10 @@ def test_mapped_and_staged_buffers(gpu, oracle):
     # copy kernels through their device mapping) with pageable ones (staged),
     # at 16-byte aligned and unaligned addresses, with odd block counts
     reqs = _requests(12, 24, max_side=160)
+    # and buffers of 1 MiB or more (copied by DMA), page-locked and pageable
+    for k in range(3):
+        big, _ = synth.make_image_blocks(40 + k, 320, 256, synth.MIX_DEFAULT)   # 5,120 blocks, 1.25 MiB
+        reqs.append((8 + k % 2, big, None, bool(k % 2)))
     rng = np.random.default_rng(12)
     holders = []
 
