@@ -230,59 +230,24 @@ def cpu_baseline(blocks, level, tcm, tcb, threads, budget_cpu_s=20.0, seed=12):
                       f"chunks of 32 blocks; {secs:.2f} s wall"}
 
 
-class ResultBuffers:
-    """Page-locked code buffers for the serving client, `per_texture` for each
-    pool texture: the server's copy kernel writes results into them in place.
-    A request takes a free buffer of its texture (or, with none free, a
-    pageable array, which the server stages) and gives it back once its
-    result has been checked."""
-
-    def __init__(self, g, counts, per_texture):
-        import threading
-        self.holder = g.PinnedArray((int(sum(counts)) * per_texture, 8), np.uint8)
-        self.free = []
-        off = 0
-        for n in counts:
-            bufs = []
-            for _ in range(per_texture):
-                bufs.append(self.holder.array[off:off + n])
-                off += n
-            self.free.append(bufs)
-        self.lock = threading.Lock()
-        self.fallbacks = 0
-
-    def take(self, k, n):
-        with self.lock:
-            if self.free[k]:
-                return self.free[k].pop(), True
-            self.fallbacks += 1
-        return np.zeros((n, 8), np.uint8), False
-
-    def give(self, k, buf, pinned):
-        if pinned:
-            with self.lock:
-                self.free[k].append(buf)
-
-    def close(self):
-        self.free = None
-        self.holder.close()
-
-
 def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64 << 30,
                   max_inflight=3, parity=None, texels="rgba8", saturation_seconds=3.0, window=512):
-    """Poisson request stream against the request server: textures with sides
-    log-uniform in [16, 2048], level uniform in 1..9, three-colour mode on for
-    half of them, block mix 40/30/20/10, stored as RGBA8 (read by the encoder
-    as c / 255.0f) or, with texels="float32", as float RGBA (256 B per block).
-    Inputs and result buffers sit in page-locked host memory: each request's
-    texels cross PCIe inside its latency, and the server reads and writes them
-    in place.  Saturation is the sustained rate with `window` requests kept
-    outstanding for `saturation_seconds` (cycling over the pool); then each QPS
-    fraction runs for `seconds`.  Every served result is compared with the
-    same texture encoded alone, and every pool texture encoded alone is
-    checked block for block against the oracle (with `parity`)."""
-    import queue
-    import threading
+    """Request streams against the request server, driven by the C++ client
+    icbc_amd/serve_bench (a Python client's own cost per request would bound
+    the rate it can offer): textures with sides log-uniform in [16, 2048],
+    level uniform in 1..9, three-colour mode on for half of them, block mix
+    40/30/20/10, stored as RGBA8 (read by the encoder as c / 255.0f) or, with
+    texels="float32", as float RGBA (256 B per block).  Inputs and result
+    buffers sit in page-locked host memory, which the server reads and writes
+    in place, so each request's texels cross PCIe inside its latency.
+    Saturation is the sustained rate with `window` requests kept outstanding
+    for `saturation_seconds`; then each QPS fraction runs a Poisson stream for
+    `seconds`.  Every served result is compared with the same texture encoded
+    alone (by the client), and every pool texture encoded alone is checked
+    block for block against the oracle (with `parity`)."""
+    import subprocess
+    import tempfile
+    from icbc_amd import build as hip_build
 
     rng = np.random.default_rng(seed)
     pool = []
@@ -292,130 +257,44 @@ def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64
         blocks = synth.image_to_blocks(synth.make_image(seed * 10000 + k, w, h, synth.MIX_DEFAULT))
         if texels == "rgba8":
             blocks = np.clip(np.round(blocks * 255), 0, 255).astype(np.uint8)   # RGBA8 texels
-        holder = g.pinned_copy(blocks)
-        pool.append({"blocks": holder.array, "holder": holder, "n": len(blocks),
+        pool.append({"blocks": np.ascontiguousarray(blocks), "n": len(blocks),
                      "level": int(rng.integers(1, 10)), "tcm": bool(rng.integers(0, 2))})
-    alone_runs = [g.compress_many(p["level"], p["blocks"], None, (1, 1, 1), p["tcm"], p["tcm"]) for p in pool]
-    alone = [r[0] for r in alone_runs]
+    total = sum(p["n"] for p in pool)
+    with tempfile.TemporaryDirectory() as tmp:
+        src, alone_path = os.path.join(tmp, "pool.bin"), os.path.join(tmp, "alone.bin")
+        with open(src, "wb") as f:
+            f.write(np.array([pool_size, 0 if texels == "rgba8" else 1], np.uint32).tobytes())
+            for p in pool:
+                f.write(np.array([p["n"]], np.uint64).tobytes() + np.array([p["level"], int(p["tcm"])],
+                                                                           np.int32).tobytes())
+            for p in pool:
+                f.write(p["blocks"].tobytes())
+        cmd = [hip_build.SERVE_BIN, src, alone_path, str(seconds), ",".join(str(x) for x in qps_fracs),
+               str(budget), str(max_inflight), str(window), str(saturation_seconds), str(seed)]
+        res = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        for line in res.stderr.strip().splitlines():
+            log(line)
+        if res.returncode != 0:
+            raise RuntimeError(f"serve_bench failed ({res.returncode})")
+        out = json.loads(res.stdout.strip().splitlines()[-1])
+        raw = np.fromfile(alone_path, np.uint8)
+        alone, alone_err = raw[:8 * total].reshape(total, 8), raw[8 * total:].view(np.float32)
     if parity is not None:
         # the oracle reads RGBA8 texels as c / 255.0f, as the encoder does
-        for k, (p, r) in enumerate(zip(pool, alone_runs)):
+        off = 0
+        for k, p in enumerate(pool):
             f = (np.asarray(p["blocks"], np.float32) / np.float32(255) if texels == "rgba8"
                  else np.asarray(p["blocks"], np.float32))
             parity.check(f"serving pool texture {k}", p["level"], f, None, (1, 1, 1), p["tcm"], p["tcm"],
-                         r[0], r[1], sample=1 << 30)
-    srv = g.Server(budget, max_inflight=max_inflight)
-    results = ResultBuffers(g, [p["n"] for p in pool], per_texture=16)
-
-    def submit(k):
-        p = pool[k]
-        buf, pinned = results.take(k, p["n"])
-        t = srv.submit(p["level"], p["blocks"], None, (1, 1, 1), p["tcm"], p["tcm"], codes=buf, with_errors=False)
-        return t, k, buf, pinned
-
-    def run(arrivals, stop_after=None):
-        """Submits requests from `arrivals` (yields a pool index when the next
-        request is due, None to stop); a waiter thread collects them in order,
-        checks each result and frees its buffer."""
-        q = queue.Queue()
-        infos, bad = [], [0]
-
-        def waiter():
-            while True:
-                item = q.get()
-                if item is None:
-                    return
-                ticket, k, buf, pinned = item
-                _, _, info = srv.wait(ticket)
-                if not np.array_equal(buf, alone[k]):
-                    bad[0] += 1
-                results.give(k, buf, pinned)
-                infos.append(info)
-
-        th = threading.Thread(target=waiter)
-        th.start()
-        for k in arrivals(q):
-            q.put(submit(k))
-        q.put(None)
-        th.join()
-        return infos, bad[0]
-
-    # warm server: two untimed passes over the pool grow its slabs, staging
-    # and tables (one-time work that a running server has done)
-    def pool_pass(q):
-        for k in range(pool_size):
-            yield k
-    for _ in range(2):
-        run(pool_pass)
-
-    # saturation: `window` requests kept outstanding, cycling over the pool
-    order = rng.permutation(np.tile(np.arange(pool_size), 2048))
-
-    def windowed(q):
-        t0 = time.perf_counter()
-        for i, k in enumerate(order):
-            while q.qsize() >= window:
-                time.sleep(0.0002)
-            if time.perf_counter() - t0 > saturation_seconds:
-                return
-            yield int(k)
-
-    infos, bad_sat = run(windowed)
-    t_first = min(i.submit_s for i in infos)
-    t_last = max(i.done_s for i in infos)
-    sat_blocks = sum(i.block_count for i in infos)
-    sat_s = t_last - t_first
-    sat_qps = len(infos) / sat_s
-    sat_mbps = sat_blocks / sat_s / 1e6
-    log(f"serving saturation: {sat_qps:.1f} requests/s, {sat_mbps:.1f} Mblocks/s "
-        f"({len(infos)} requests over {sat_s:.2f} s, window {window}, mean "
-        f"{np.mean([i.batch_size for i in infos]):.1f} requests per launch)")
-
-    runs = []
-    for frac in qps_fracs:
-        rate = frac * sat_qps
-
-        def poisson(q):
-            start = time.perf_counter()
-            t_next = start
-            while True:
-                t_next += rng.exponential(1.0 / rate)
-                if t_next - start > seconds:
-                    return
-                while True:
-                    d = t_next - time.perf_counter()
-                    if d <= 0:
-                        break
-                    time.sleep(d if d > 0.002 else 0)
-                yield int(rng.integers(0, pool_size))
-
-        infos, bad = run(poisson)
-        lat = np.array([i.done_s - i.submit_s for i in infos]) * 1e3
-        blocks = sum(i.block_count for i in infos)
-        span = max(i.done_s for i in infos) - min(i.submit_s for i in infos)
-        runs.append({"qps_target": round(rate, 2), "fraction_of_saturation": frac, "requests": len(infos),
-                     "mblocks_per_s": round(blocks / span / 1e6, 3),
-                     "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
-                     "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
-                     "latency_ms_max": round(float(lat.max()), 3),
-                     "mean_batch_requests": round(float(np.mean([i.batch_size for i in infos])), 2),
-                     "results_identical_to_alone": bad == 0})
-        log(f"serving @ {rate:.1f} qps: {runs[-1]}")
-    srv.close()
-    fallbacks = results.fallbacks
-    results.close()
-    for p in pool:
-        p["holder"].close()
-    return {"saturation_requests_per_s": round(sat_qps, 2),
-            "saturation_mblocks_per_s": round(sat_mbps, 3),
-            "saturation_results_identical_to_alone": bad_sat == 0,
-            "saturation_window_requests": window,
-            "pool_textures": pool_size, "mean_blocks_per_request": round(sum(p["n"] for p in pool) / pool_size, 1),
-            "memory_budget_bytes": budget, "max_inflight_launches": max_inflight,
-            "seconds_per_rate": seconds, "runs": runs, "pageable_result_fallbacks": fallbacks,
-            "input": ("RGBA8 texels (64 B/block)" if texels == "rgba8" else "float32 RGBA texels (256 B/block)")
-                     + " and codes (8 B/block) in page-locked host memory, read and written by the server "
-                       "in place"}
+                         alone[off:off + p["n"]], alone_err[off:off + p["n"]], sample=1 << 30)
+            off += p["n"]
+    log(f"serving saturation: {out['saturation_requests_per_s']:.1f} requests/s, "
+        f"{out['saturation_mblocks_per_s']:.1f} Mblocks/s ({out['saturation_requests']} requests, window {window}, "
+        f"mean {out['saturation_mean_batch_requests']:.1f} requests per launch)")
+    out["client"] = "icbc_amd/serve_bench (C++), one submitting thread and one waiting thread"
+    out["input"] = (("RGBA8 texels (64 B/block)" if texels == "rgba8" else "float32 RGBA texels (256 B/block)")
+                    + " and codes (8 B/block) in page-locked host memory, read and written by the server in place")
+    return out
 
 
 def drop_in_per_block(g, synth, thread_counts, level=8, repeats=3, parity=None):
@@ -472,7 +351,7 @@ def single_image_levels(g, synth, levels=(1, 5, 9), steps=10, seed=7, parity=Non
         for lv in levels:
             job = g.make_job(lv, n, dcol, dcode, None, derr, None, (1, 1, 1), True, True)
             plan = g.Plan([job], stream=s)
-            for _ in range(2):
+            for _ in range(5):   # the device's clocks settle over the first launches
                 plan.execute(s)
             e0.record(s)
             for _ in range(steps):

@@ -49,18 +49,22 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-# The drop-in per-block benchmark program (a C++ caller of the icbc:: API,
-# run by bench.py).
+# Benchmark programs run by bench.py: the drop-in per-block benchmark (a C++
+# caller of the icbc:: API) and the serving client (a C++ caller of the
+# request server).
 BENCH_SRC = os.path.join(CSRC, "drop_in_bench.cpp")
 BENCH_BIN = os.path.join(HERE, "drop_in_bench")
+SERVE_SRC = os.path.join(CSRC, "serve_bench.cpp")
+SERVE_BIN = os.path.join(HERE, "serve_bench")
 
 
 def build_drop_in_bench(verbose=False):
-    cmd = ["g++", "-std=c++17", "-O2", "-pthread", "-I", os.path.join(ROOT, "include"), BENCH_SRC,
-           "-L", HERE, "-licbc_amd", "-Wl,-rpath,$ORIGIN", "-o", BENCH_BIN]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.check_call(cmd)
+    for src, out in ((BENCH_SRC, BENCH_BIN), (SERVE_SRC, SERVE_BIN)):
+        cmd = ["g++", "-std=c++17", "-O2", "-pthread", "-I", os.path.join(ROOT, "include"), src,
+               "-L", HERE, "-licbc_amd", "-Wl,-rpath,$ORIGIN", "-o", out]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
     return BENCH_BIN
 
 

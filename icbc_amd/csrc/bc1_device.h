@@ -363,7 +363,16 @@ __device__ __forceinline__ float select4(const Src& t, f3 w, const f3 pal[4], ui
         const uint32_t lo = (d0 > d3) & (d2 > d3);
         const uint32_t k = lo | (hi << 1);
         out |= k << (2 * i);
-        e += t.weight(i) * texel_error<kUnit>(pick(pal, k), x, w);
+        if constexpr (kUnit) {
+            // with unit channel weights c = x, and the error's difference
+            // p - x is the chosen entry's c - p negated, exactly (IEEE
+            // subtraction is antisymmetric): the error squares the same values
+            const f3 lo3 = (k & 1) ? e1 : e0, hi3 = (k & 1) ? e3 : e2;
+            const f3 d = ((k & 2) ? hi3 : lo3) * 255.0f;
+            e += t.weight(i) * dot3(d, d);
+        } else {
+            e += t.weight(i) * texel_error<kUnit>(pick(pal, k), x, w);
+        }
     }
     indices = out;
     return e;
