@@ -230,6 +230,31 @@ def cpu_baseline(blocks, level, tcm, tcb, threads, budget_cpu_s=20.0, seed=12):
                       f"chunks of 32 blocks; {secs:.2f} s wall"}
 
 
+def write_serving_pool(path, synth, seed=5, pool_size=128, texels="rgba8"):
+    """The serving workload's request pool, written for icbc_amd/serve_bench:
+    textures with sides log-uniform in [16, 2048], level uniform in 1..9,
+    three-colour mode on for half of them, block mix 40/30/20/10, as RGBA8 or
+    float32 texels.  Returns the pool (blocks, block count, level, mode)."""
+    rng = np.random.default_rng(seed)
+    pool = []
+    for k in range(pool_size):
+        w = int(round(np.exp(rng.uniform(np.log(16), np.log(2048)))))
+        h = int(round(np.exp(rng.uniform(np.log(16), np.log(2048)))))
+        blocks = synth.image_to_blocks(synth.make_image(seed * 10000 + k, w, h, synth.MIX_DEFAULT))
+        if texels == "rgba8":
+            blocks = np.clip(np.round(blocks * 255), 0, 255).astype(np.uint8)   # RGBA8 texels
+        pool.append({"blocks": np.ascontiguousarray(blocks), "n": len(blocks),
+                     "level": int(rng.integers(1, 10)), "tcm": bool(rng.integers(0, 2))})
+    with open(path, "wb") as f:
+        f.write(np.array([pool_size, 0 if texels == "rgba8" else 1], np.uint32).tobytes())
+        for p in pool:
+            f.write(np.array([p["n"]], np.uint64).tobytes() + np.array([p["level"], int(p["tcm"])],
+                                                                       np.int32).tobytes())
+        for p in pool:
+            f.write(p["blocks"].tobytes())
+    return pool
+
+
 def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64 << 30,
                   max_inflight=3, parity=None, texels="rgba8", saturation_seconds=3.0, window=512):
     """Request streams against the request server, driven by the C++ client
@@ -249,26 +274,10 @@ def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64
     import tempfile
     from icbc_amd import build as hip_build
 
-    rng = np.random.default_rng(seed)
-    pool = []
-    for k in range(pool_size):
-        w = int(round(np.exp(rng.uniform(np.log(16), np.log(2048)))))
-        h = int(round(np.exp(rng.uniform(np.log(16), np.log(2048)))))
-        blocks = synth.image_to_blocks(synth.make_image(seed * 10000 + k, w, h, synth.MIX_DEFAULT))
-        if texels == "rgba8":
-            blocks = np.clip(np.round(blocks * 255), 0, 255).astype(np.uint8)   # RGBA8 texels
-        pool.append({"blocks": np.ascontiguousarray(blocks), "n": len(blocks),
-                     "level": int(rng.integers(1, 10)), "tcm": bool(rng.integers(0, 2))})
-    total = sum(p["n"] for p in pool)
     with tempfile.TemporaryDirectory() as tmp:
         src, alone_path = os.path.join(tmp, "pool.bin"), os.path.join(tmp, "alone.bin")
-        with open(src, "wb") as f:
-            f.write(np.array([pool_size, 0 if texels == "rgba8" else 1], np.uint32).tobytes())
-            for p in pool:
-                f.write(np.array([p["n"]], np.uint64).tobytes() + np.array([p["level"], int(p["tcm"])],
-                                                                           np.int32).tobytes())
-            for p in pool:
-                f.write(p["blocks"].tobytes())
+        pool = write_serving_pool(src, synth, seed, pool_size, texels)
+        total = sum(p["n"] for p in pool)
         cmd = [hip_build.SERVE_BIN, src, alone_path, str(seconds), ",".join(str(x) for x in qps_fracs),
                str(budget), str(max_inflight), str(window), str(saturation_seconds), str(seed)]
         res = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
@@ -507,6 +516,9 @@ def main():
                     help="length of the short serving run included in the default benchmark")
     ap.add_argument("--serve-profile-seconds", type=float, default=60.0,
                     help="length of each load level of the --serve profile")
+    ap.add_argument("--write-serving-pool", metavar="PATH",
+                    help="only write the serving workload's RGBA8 request pool to PATH (the input of "
+                         "icbc_amd/serve_bench, for profiling the server on its own) and exit")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-drop-in", action="store_true",
                     help="skip the per-block icbc::compress_bc1 run from worker threads")
@@ -534,6 +546,11 @@ def main():
 
     import icbc_amd as g
     from icbc_amd import synth
+
+    if args.write_serving_pool:
+        pool = write_serving_pool(args.write_serving_pool, synth)
+        log(f"wrote {len(pool)} textures, {sum(p['n'] for p in pool)} blocks to {args.write_serving_pool}")
+        return
 
     g.init(g.Decoder.D3D10)
     dev = g.device_name()
