@@ -280,12 +280,14 @@ def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64
         total = sum(p["n"] for p in pool)
         cmd = [hip_build.SERVE_BIN, src, alone_path, str(seconds), ",".join(str(x) for x in qps_fracs),
                str(budget), str(max_inflight), str(window), str(saturation_seconds), str(seed)]
-        res = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
-        for line in res.stderr.strip().splitlines():
-            log(line)
-        if res.returncode != 0:
-            raise RuntimeError(f"serve_bench failed ({res.returncode})")
-        out = json.loads(res.stdout.strip().splitlines()[-1])
+        # the client's progress lines are passed on as they come
+        proc = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+        for line in proc.stderr:
+            log(line.rstrip())
+        stdout = proc.stdout.read()
+        if proc.wait(timeout=600) != 0:
+            raise RuntimeError(f"serve_bench failed ({proc.returncode})")
+        out = json.loads(stdout.strip().splitlines()[-1])
         raw = np.fromfile(alone_path, np.uint8)
         alone, alone_err = raw[:8 * total].reshape(total, 8), raw[8 * total:].view(np.float32)
     if parity is not None:

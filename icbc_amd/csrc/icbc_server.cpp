@@ -5,7 +5,9 @@
 //     buffers, each launch's job table, tile list and level-9 handoff list,
 //     and the launch slots' workspaces are all charged against it;
 //   - admits requests in arrival order while their device footprint fits the
-//     budget (a request that could never fit is rejected at submission);
+//     budget and the admitted, not yet launched ones hold fewer than two
+//     launches' worth of blocks (a request that could never fit is rejected
+//     at submission);
 //   - keeps request buffers resident: device memory comes from a pool of
 //     size-classed blocks carved from slabs and reused; a slab goes back to
 //     the driver only when all its blocks are free and another class needs
@@ -62,6 +64,15 @@ constexpr uint64_t kMinServerGrid = 64;
 // launches gain no throughput, only latency for their first requests and
 // staging memory for pageable buffers.
 constexpr uint64_t kMaxLaunchBlocks = 1u << 20;
+
+// Requests are admitted (given device memory) only while the admitted, not
+// yet launched ones hold fewer blocks than this: enough for the next two
+// launches.  Under overload the backlog then waits in the host-side queue
+// instead of taking device memory request by request: admitting everything
+// a large budget allowed made the pool map tens of gigabytes of slabs in the
+// scheduler thread (and release them again as size classes shifted), which
+// halved the rate the server sustained once a backlog had formed.
+constexpr uint64_t kAdmitAheadBlocks = 2 * kMaxLaunchBlocks;
 
 // Input buffers of at least this size go by DMA (1 MiB copies run at 38 GB/s
 // on MI355X, 128 KiB ones at 11 GB/s: smaller ones go through the copy
@@ -544,9 +555,12 @@ void icbc_server::run_scheduler() {
             cv_done.notify_all();
         }
         // admit in arrival order while the budget allows (the pool refuses
-        // blocks the budget has no room for); not while a batch waits for the
+        // blocks the budget has no room for) and the admitted requests hold
+        // fewer than kAdmitAheadBlocks blocks; not while a batch waits for the
         // room its launch tables need, which admissions would take again
-        while (!pending.empty() && !hold_admission) {
+        uint64_t ahead = 0;
+        for (const Request* r : admitted) ahead += r->job.block_count;
+        while (!pending.empty() && !hold_admission && ahead < kAdmitAheadBlocks) {
             Request* r = pending.front();
             // the pool is only touched under the lock (the reaper returns blocks)
             const hipError_t te = pool.take(r->cls, &r->dev);
@@ -564,6 +578,7 @@ void icbc_server::run_scheduler() {
             r->state = kAdmitted;
             r->t_admit = now_s();
             admitted.push_back(r);
+            ahead += r->job.block_count;
         }
         Slot* free_slot = nullptr;
         for (Slot& s : slots)

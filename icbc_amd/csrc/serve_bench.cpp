@@ -287,10 +287,18 @@ int main(int argc, char** argv) {
         const double rate = frac * sat_qps;
         std::exponential_distribution<double> gap(rate);
         const auto t0 = clk::now();
-        double t_next = 0;
+        double t_next = 0, t_report = 10.0;
+        uint64_t submitted = 0;
         const Run r = stream([&] {
             t_next += gap(rng);
             if (t_next > seconds) return -1;
+            if (t_next >= t_report) {  // a progress line every 10 s of a long run
+                std::fprintf(stderr, "serving @ %.0f req/s: %.0f s, %llu requests submitted\n", rate, t_report,
+                             (unsigned long long)submitted);
+                std::fflush(stderr);
+                t_report += 10.0;
+            }
+            submitted++;
             const auto due = t0 + std::chrono::duration_cast<clk::duration>(std::chrono::duration<double>(t_next));
             if (due - clk::now() > std::chrono::microseconds(200)) std::this_thread::sleep_until(due -
                                                                                    std::chrono::microseconds(100));

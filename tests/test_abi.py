@@ -111,3 +111,13 @@ def test_c_header_compiles_as_c_and_cpp_header_as_cpp(tmp_path):
                    'unsigned char o[8];(void)&icbc::compress_bc1;(void)c;(void)w;(void)cw;(void)o;return 0;}\n')
     subprocess.check_call(["g++", "-std=c++11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
                            "-c", str(cpp), "-o", str(tmp_path / "t2.o")])
+
+
+def test_benchmark_programs_report_usage():
+    # the C++ benchmark callers are built beside the library and check their
+    # arguments before touching a device
+    import subprocess
+    from icbc_amd import build as hip_build
+    for prog in (hip_build.BENCH_BIN, hip_build.SERVE_BIN):
+        res = subprocess.run([prog], capture_output=True, text=True, timeout=60)
+        assert res.returncode == 2 and "usage" in res.stderr, (prog, res.returncode, res.stderr)

@@ -329,3 +329,35 @@ def test_two_waits_on_one_ticket(gpu):
     finally:
         srv.close()
     assert sorted(outcome) == [("error", gpu.ERR_INVALID_ARGUMENT), ("ok", 0)], outcome
+
+
+def test_serve_bench_program(gpu, oracle, tmp_path):
+    # the C++ serving client (icbc_amd/serve_bench) on a small RGBA8 pool:
+    # every served result equals the texture encoded alone, which equals the oracle
+    import json
+    import subprocess
+    from icbc_amd import build as hip_build
+    reqs = _requests(14, 12, max_side=160)
+    pool = [(lv, np.clip(np.round(b * 255), 0, 255).astype(np.uint8), tcm) for lv, b, _, tcm in reqs]
+    src, alone = tmp_path / "pool.bin", tmp_path / "alone.bin"
+    with open(src, "wb") as f:
+        f.write(np.array([len(pool), 0], np.uint32).tobytes())
+        for lv, b, tcm in pool:
+            f.write(np.array([len(b)], np.uint64).tobytes() + np.array([lv, int(tcm)], np.int32).tobytes())
+        for _, b, _ in pool:
+            f.write(np.ascontiguousarray(b).tobytes())
+    res = subprocess.run([hip_build.SERVE_BIN, str(src), str(alone), "0.3", "0.5", str(256 << 20), "2", "32", "0.3",
+                          "3"], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    r = json.loads(res.stdout.strip().splitlines()[-1])
+    assert r["saturation_results_identical_to_alone"] and r["saturation_requests"] > 0
+    assert all(run["results_identical_to_alone"] for run in r["runs"])
+    total = sum(len(b) for _, b, _ in pool)
+    raw = np.fromfile(alone, np.uint8)
+    codes, errs = raw[:8 * total].reshape(total, 8), raw[8 * total:].view(np.float32)
+    off = 0
+    for lv, b, tcm in pool:
+        want = _oracle(oracle, lv, b.astype(np.float32) / np.float32(255), None, tcm)
+        assert np.array_equal(codes[off:off + len(b)], want[0])
+        assert np.array_equal(errs[off:off + len(b)].view(np.uint32), want[1].view(np.uint32))
+        off += len(b)
