@@ -361,3 +361,51 @@ def test_serve_bench_program(gpu, oracle, tmp_path):
         assert np.array_equal(codes[off:off + len(b)], want[0])
         assert np.array_equal(errs[off:off + len(b)].view(np.uint32), want[1].view(np.uint32))
         off += len(b)
+
+
+def test_backlog_waits_in_the_host_queue(gpu):
+    # a burst far above what the device holds at once, with a budget that
+    # could hold all of it: requests are admitted only while fewer than two
+    # launches' worth of blocks (2^21) wait, so the memory in use stays near
+    # that bound and the backlog waits in the host-side queue
+    blocks, _ = synth.make_image_blocks(21, 512, 512, synth.MIX_DEFAULT)   # 16,384 blocks
+    pinned = gpu.pinned_copy(np.clip(np.round(blocks * 255), 0, 255).astype(np.uint8))
+    srv = gpu.Server(memory_budget=16 << 30, max_inflight=3)
+    peak = {"in_use": 0, "pending": 0}
+    stop = threading.Event()
+
+    def watch():
+        while not stop.is_set():
+            st = srv.stats()
+            peak["in_use"] = max(peak["in_use"], st["memory_in_use"])
+            peak["pending"] = max(peak["pending"], st["pending"])
+            time.sleep(0.0002)
+
+    th = threading.Thread(target=watch)
+    th.start()
+    try:
+        outs = [np.empty((len(blocks), 8), np.uint8) for _ in range(600)]   # 9.8 M blocks in all
+        tickets = [srv.submit(9, pinned.array, None, (1, 1, 1), True, True, codes=o, with_errors=False)
+                   for o in outs]
+        for t in tickets:
+            srv.wait(t)
+    finally:
+        stop.set()
+        th.join()
+        srv.close()
+        pinned.close()
+    alone = gpu.compress_many(9, np.clip(np.round(blocks * 255), 0, 255).astype(np.uint8), None, (1, 1, 1), True,
+                              True)[0]
+    assert all(np.array_equal(o, alone) for o in outs)
+    # in use: admitted requests (admission stops once 2^21 blocks wait, so at
+    # most one request past that), 3 launches of at most 2^20 blocks each,
+    # every request's 64 B texels and 8 B codes in a power-of-two class, each
+    # launch's level-9 list (16 B per block, a 32 MiB class) and small tables:
+    # about 0.75 GiB, against the 1.2 GiB all 600 requests would take at once
+    per_request = 2 << 20   # 16,384 x 72 B = 1.1 MiB -> its 2 MiB class
+    waiting = (1 << 21) // len(blocks) + 1
+    launched = 3 * ((1 << 20) // len(blocks))
+    bound = (waiting + launched) * per_request + 3 * (32 << 20) + (16 << 20)
+    assert bound < len(outs) * per_request
+    assert 0 < peak["in_use"] <= bound, (peak, bound)
+    assert peak["pending"] > 50, peak
