@@ -362,8 +362,14 @@ def single_image_levels(g, synth, levels=(1, 5, 9), steps=10, seed=7, parity=Non
         for lv in levels:
             job = g.make_job(lv, n, dcol, dcode, None, derr, None, (1, 1, 1), True, True)
             plan = g.Plan([job], stream=s)
-            for _ in range(5):   # the device's clocks settle over the first launches
-                plan.execute(s)
+            # untimed launches for 0.2 s first: the device's clocks settle over
+            # the first launches (a level-1 step takes 0.112 ms on the first
+            # few, 0.100 ms once settled)
+            t_warm = time.perf_counter()
+            while time.perf_counter() - t_warm < 0.2:
+                for _ in range(10):
+                    plan.execute(s)
+                s.synchronize()
             e0.record(s)
             for _ in range(steps):
                 plan.execute(s)

@@ -901,7 +901,16 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         }
         if (queue) {
             uint32_t v = 0;
-            if (threadIdx.x == 0) v = atomicAdd(queue, 1u);
+            if (threadIdx.x == 0) {
+                v = atomicAdd(queue, 1u);
+                // A queue takes exactly `share` claims in a launch: each
+                // workgroup with a static tile claims after every tile, and
+                // the claims past the share end one workgroup each.  The
+                // claim that returns share - 1 is the last one, so it puts the
+                // counter back to zero for the next launch (a plan needs no
+                // reset kernel).
+                if (v == share - 1) atomicExch(queue, 0u);
+            }
             j = static_claims + __builtin_amdgcn_readlane(v, 0);
         } else {
             t += gridDim.x;
@@ -912,7 +921,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) v
 // Zeroes `n` words `stride` words apart (queue counters; a kernel rather
 // than a memset node, so that captured graphs replay it correctly).
 __global__ void zero_words_kernel(uint32_t* p, uint32_t n, uint32_t stride) {
-    if (threadIdx.x < n) p[threadIdx.x * stride] = 0u;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i * stride] = 0u;
 }
 
 // refine_endpoints (icbc.h:3574-3651) for the level-9 blocks handed over by
@@ -988,6 +997,8 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
     // blocks claimed by this wavefront and not started yet: [lo, hi); the
     // list counter is touched once per kTile / K blocks, not per free group
     const uint32_t chunk = (uint32_t)(kTile / K);
+    const uint32_t below = total > chunk ? (total + chunk - 1) / chunk - 1 : 0;  // claims not draining the list
+    const uint32_t last_claim = (below + gridDim.x * (kRefineThreads / kTile) - 1) * chunk;
     uint32_t lo = 0, hi = 0;
     // starting a walk reads its block's texels from HBM, which stalls the
     // whole wavefront: free groups start together, once at least a quarter of
@@ -1052,7 +1063,19 @@ __global__ __launch_bounds__(kRefineThreads) __attribute__((amdgpu_waves_per_eu(
         if (nfree < refill_groups && nfree < all_groups) free_mask = 0;
         if (free_mask && lo == hi && !drained) {
             uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(claims, chunk);
+            if (lane == 0) {
+                base = atomicAdd(claims, chunk);
+                // The list takes exactly (m + W) claims in a launch: m below
+                // total - chunk, then one per wavefront (the claim that drains
+                // it; every wavefront claims until drained, then never again).
+                // The last one puts both counters back to zero for the next
+                // launch's encode and refinement (a plan needs no reset
+                // kernel); every wavefront read `total` before its first claim.
+                if (base == last_claim) {
+                    atomicExch(rl.counters, 0u);
+                    atomicExch(claims, 0u);
+                }
+            }
             base = __builtin_amdgcn_readlane(base, 0);
             lo = min(base, total);
             hi = min(base + chunk, total);
@@ -1317,12 +1340,13 @@ uint32_t refine_grid() {
 }  // namespace
 
 hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, uint32_t* counter,
-                         float4* workspace, uint32_t grid, int mode, RefineList refine, hipStream_t stream) {
+                         float4* workspace, uint32_t grid, int mode, RefineList refine, hipStream_t stream,
+                         bool counters_zeroed) {
     if (ntiles == 0) return hipSuccess;
     if (grid > ntiles) grid = ntiles;
-    const uint32_t stride = (uint32_t)(kQueueStride / sizeof(uint32_t));
-    if (counter) {
-        hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, counter, kQueues, stride);
+    if (counter && !counters_zeroed) {
+        hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, counter, kQueues,
+                           (uint32_t)(kQueueStride / sizeof(uint32_t)));
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -1330,8 +1354,9 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
     const bool unit = (mode & kUnitWeights) != 0;
     const bool box = (mode & kBoxStage) != 0;
     mode &= 15;
-    if (mode == kModeAll) {
-        hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, refine.counters, 2u, stride);
+    if (mode == kModeAll && !counters_zeroed) {
+        hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, refine.counters, 2u,
+                           (uint32_t)(kQueueStride / sizeof(uint32_t)));
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

@@ -146,8 +146,14 @@ constexpr int kBoxStage = 1 << 7;
 // `mode` selects the kernel instance (level mode | texel format << 4 |
 // kUnitWeights | kBoxStage); encode_mode() gives the one a set of jobs needs.
 // With kModeAll, `refine` holds room for the launch's level-9 blocks.
+// The queue counters (kQueueBytes at `counter`) and the handoff list's
+// counters are zeroed by a small kernel first, unless `counters_zeroed`: the
+// kernels leave them at zero when they finish (the last claim on each counter
+// resets it), so a resource that is reused launch after launch (a plan)
+// zeroes them once, when it is made.
 hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles, uint32_t* counter,
-                         float4* workspace, uint32_t grid, int mode, RefineList refine, hipStream_t stream);
+                         float4* workspace, uint32_t grid, int mode, RefineList refine, hipStream_t stream,
+                         bool counters_zeroed = false);
 
 inline int encode_mode_of_level(int level) {
     return level == 9 ? kModeAll : (level >= 2 && level <= 8) ? kModeCluster : kModeBox;

@@ -1012,7 +1012,9 @@ int plan_launch(icbc_bc1_plan* pl, hipStream_t s) {
     uint32_t* counter = pl->opt.schedule == ICBC_SCHEDULE_PERSISTENT ? (uint32_t*)pl->dwork : nullptr;
     float4* slices = (float4*)((char*)pl->dwork + kQueueBytes);
     const RefineList rl = refine_list_at((char*)pl->dwork + kQueueBytes + (size_t)pl->grid * kWorkspaceBytesPerWorkgroup);
-    HIP_TRY(launch_encode(pl->djobs, pl->dtiles, pl->ntiles, counter, slices, pl->grid, pl->mode, rl, s));
+    // the counters were zeroed when the plan was made, and every launch leaves
+    // them at zero: no reset kernel in the plan's launch
+    HIP_TRY(launch_encode(pl->djobs, pl->dtiles, pl->ntiles, counter, slices, pl->grid, pl->mode, rl, s, true));
     return ICBC_OK;
 }
 
@@ -1102,6 +1104,11 @@ int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_pla
         (e = hipMalloc(&pl->dwork, kQueueBytes + (size_t)pl->grid * kWorkspaceBytesPerWorkgroup +
                                        refine_list_bytes(level9_blocks(dev.data(), dev.size())))) != hipSuccess)
         return cleanup(hip_fail(e, "hipMalloc (plan)"));
+    // the queue and handoff-list counters start at zero; the kernels leave them so
+    if ((e = hipMemset(pl->dwork, 0, kQueueBytes)) != hipSuccess ||
+        (e = hipMemset((char*)pl->dwork + kQueueBytes + (size_t)pl->grid * kWorkspaceBytesPerWorkgroup, 0,
+                       kRefineCounterBytes)) != hipSuccess)
+        return cleanup(hip_fail(e, "hipMemset (plan counters)"));
     if (!dev.empty() &&
         (e = hipMemcpy(pl->djobs, dev.data(), dev.size() * sizeof(JobDev), hipMemcpyHostToDevice)) != hipSuccess)
         return cleanup(hip_fail(e, "hipMemcpy (plan jobs)"));
