@@ -754,6 +754,67 @@ std::vector<uint2> make_tiles(const std::vector<JobDev>& dev, bool by_cost) {
     return tiles;
 }
 
+// One launch over part of a job table: its kernel instance (encode_mode) and
+// its range of the tile list.
+struct SubLaunch {
+    int mode;
+    uint32_t first, count;
+};
+
+// A launch runs one kernel instance for all its jobs: levels 0-1 beside
+// cluster levels run in the cluster instance's box-fit stage, 3.5x slower per
+// block than the level 0-1 instance, which stages its tiles' texels in LDS
+// (level 1 on the 4096x4096 image: 0.36 against 0.10 ms), and levels 4-9
+// beside levels 2-3 run the instance that carries the box-fit stage (more
+// live registers, 1-2 % slower).  So jobs are grouped by the instance their
+// level needs -- levels 4-9, 2-3, 0-1 (invalid levels with 0-1) -- and a
+// group of at least kSplitMinWaves waves of resident workgroups gets a launch
+// of its own; a smaller one joins another group's launch instead, where it
+// adds no launch tail of its own.  Returns the tile list (the launches' tiles
+// one after another, each in job_order) and the launches, in `out`.
+constexpr uint32_t kSplitMinWaves = 2;
+
+std::vector<uint2> make_launches(const std::vector<JobDev>& dev, bool by_cost, uint32_t resident,
+                                 std::vector<SubLaunch>& out) {
+    auto group_of = [](int level) { return (level >= 4 && level <= 9) ? 0 : (level >= 2 && level <= 3) ? 1 : 2; };
+    std::vector<uint32_t> group(dev.size());
+    uint64_t tiles[3] = {0, 0, 0};
+    for (size_t k = 0; k < dev.size(); k++) {
+        group[k] = (uint32_t)group_of(dev[k].level);
+        tiles[group[k]] += tiles_of(dev[k].block_count, dev[k].tile_blocks);
+    }
+    const uint64_t big = (uint64_t)kSplitMinWaves * resident;
+    auto merge = [&](int from, int into) {
+        for (auto& g : group)
+            if (g == (uint32_t)from) g = (uint32_t)into;
+        tiles[into] += tiles[from];
+        tiles[from] = 0;
+    };
+    const int present = (tiles[0] > 0) + (tiles[1] > 0) + (tiles[2] > 0);
+    if (present > 1) {
+        if (tiles[2] && tiles[2] < big) merge(2, 1);             // levels 0-1 into the box-stage launch
+        if (tiles[1] && tiles[1] < big && tiles[0]) merge(1, 0);  // and that into the levels 4-9 launch
+        if (tiles[0] && tiles[0] < big && tiles[1]) merge(0, 1);
+    }
+    std::vector<uint2> list;
+    list.reserve(tile_count(dev));
+    out.clear();
+    const std::vector<uint32_t> order = job_order(dev, by_cost);
+    for (uint32_t g = 0; g < 3; g++) {
+        const uint32_t first = (uint32_t)list.size();
+        std::vector<JobDev> members;
+        for (uint32_t k : order) {
+            if (group[k] != g) continue;
+            members.push_back(dev[k]);
+            for (uint32_t t = 0; t < tiles_of(dev[k].block_count, dev[k].tile_blocks); t++)
+                list.push_back(make_uint2(k, t * (uint32_t)dev[k].tile_blocks));
+        }
+        if (list.size() > first)
+            out.push_back(SubLaunch{encode_mode(members.data(), members.size()), first, (uint32_t)list.size() - first});
+    }
+    return list;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -835,16 +896,18 @@ int icbc_gpu_compress_bc1_batch(const icbc_bc1_job* jobs, size_t n_jobs, void* s
         r = upload_blob(p.dev.data(), p.dev.size() * sizeof(JobDev), s, &djobs);
         if (r) return r;
         // one job: its tiles are implied (tile k = blocks [T k, T k + T)), no list
+        std::vector<SubLaunch> subs{{encode_mode(p.dev.data(), p.dev.size()), 0, (uint32_t)ntiles}};
         if (p.dev.size() > 1) {
-            const std::vector<uint2> tiles = make_tiles(p.dev, true);
+            const std::vector<uint2> tiles = make_launches(p.dev, true, resident, subs);
             r = upload_blob(tiles.data(), tiles.size() * sizeof(uint2), s, &dtiles);
             if (r) return r;
         }
         Workspace w;
         r = workspace_alloc(grid, level9_blocks(p.dev.data(), p.dev.size()), s, w);
         if (r) return r;
-        HIP_TRY(launch_encode((const JobDev*)djobs, (const uint2*)dtiles, (uint32_t)ntiles, w.counter(),
-                              w.slices(), grid, encode_mode(p.dev.data(), p.dev.size()), w.refine(), s));
+        for (const SubLaunch& sl : subs)
+            HIP_TRY(launch_encode((const JobDev*)djobs, dtiles ? (const uint2*)dtiles + sl.first : nullptr, sl.count,
+                                  w.counter(), w.slices(), std::min(grid, sl.count), sl.mode, w.refine(), s));
         HIP_TRY(hipFreeAsync(w.base, s));
         HIP_TRY(hipFreeAsync(djobs, s));
         if (dtiles) HIP_TRY(hipFreeAsync(dtiles, s));
@@ -1000,7 +1063,7 @@ struct icbc_bc1_plan {
     std::vector<void*> convert_bufs;
     uint32_t ntiles = 0;
     uint32_t grid = 0;
-    int mode = kModeBox;  // kernel instance (encode_mode)
+    std::vector<SubLaunch> launches;  // kernel instance and tile range of each launch
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -1013,8 +1076,10 @@ int plan_launch(icbc_bc1_plan* pl, hipStream_t s) {
     float4* slices = (float4*)((char*)pl->dwork + kQueueBytes);
     const RefineList rl = refine_list_at((char*)pl->dwork + kQueueBytes + (size_t)pl->grid * kWorkspaceBytesPerWorkgroup);
     // the counters were zeroed when the plan was made, and every launch leaves
-    // them at zero: no reset kernel in the plan's launch
-    HIP_TRY(launch_encode(pl->djobs, pl->dtiles, pl->ntiles, counter, slices, pl->grid, pl->mode, rl, s, true));
+    // them at zero: no reset kernel in the plan's launches
+    for (const SubLaunch& sl : pl->launches)
+        HIP_TRY(launch_encode(pl->djobs, pl->dtiles + sl.first, sl.count, counter, slices,
+                              std::min(pl->grid, sl.count), sl.mode, rl, s, true));
     return ICBC_OK;
 }
 
@@ -1067,7 +1132,6 @@ int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_pla
     }
     icbc_bc1_plan* pl = new icbc_bc1_plan();
     pl->opt = o;
-    pl->mode = encode_mode(dev.data(), dev.size());
     (void)hipGetDevice(&pl->device);
     hipDeviceProp_t prop;
     hipError_t e = hipGetDeviceProperties(&prop, pl->device);
@@ -1091,7 +1155,7 @@ int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_pla
                                                : encode_workgroups_per_cu();
     const uint32_t resident = (uint32_t)prop.multiProcessorCount * (uint32_t)per_cu;
     if (o.schedule == ICBC_SCHEDULE_PERSISTENT && o.order_by_cost) split_tail(dev, resident);
-    const std::vector<uint2> tiles = make_tiles(dev, o.order_by_cost != 0);
+    const std::vector<uint2> tiles = make_launches(dev, o.order_by_cost != 0, resident, pl->launches);
     pl->ntiles = (uint32_t)tiles.size();
     if (o.schedule == ICBC_SCHEDULE_GRID) {
         // one workgroup per tile, no queue
