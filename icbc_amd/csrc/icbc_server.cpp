@@ -319,6 +319,10 @@ struct icbc_server {
     bool hold_admission = false;  // a formed batch waits for its launch tables' room
     uint64_t next_id = 1;
     bool stop = false;
+    // the scheduler has returned: no launch can follow (the reaper stops
+    // once this is set and nothing is in flight; while the scheduler enqueues
+    // a launch, its batch is in no queue, so empty queues do not suffice)
+    bool sched_done = false;
     int fatal = ICBC_OK;
     std::thread sched, reap;
     // counters
@@ -669,13 +673,15 @@ void icbc_server::run_scheduler() {
         if (stop && pending.empty() && admitted.empty()) break;
         cv_sched.wait(lk);
     }
+    sched_done = true;
+    cv_reap.notify_all();
 }
 
 void icbc_server::run_reaper() {
     std::unique_lock<std::mutex> lk(m);
     for (;;) {
         while (inflight.empty()) {
-            if (stop && pending.empty() && admitted.empty()) return;
+            if (sched_done) return;
             cv_reap.wait(lk);
         }
         Slot* s = inflight.front();

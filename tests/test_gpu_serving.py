@@ -160,6 +160,23 @@ def test_launches_overlap(gpu):
     assert any(i.launch_s < big_info.done_s for i in later), "no launch overlapped the long one: " + detail
 
 
+def test_destroy_drains_outstanding_requests(gpu):
+    # requests still queued, admitted or on the device when the server is
+    # destroyed complete first: their (staged) outputs are written before
+    # destroy returns, as each would be encoded alone
+    reqs = _requests(12, 24, max_side=128)
+    srv = gpu.Server(memory_budget=64 << 20, max_inflight=2)
+    try:
+        tickets = [srv.submit(lv, b, w, (1, 1, 1), tcm, tcm) for lv, b, w, tcm in reqs]
+        kept = [srv._keep[t] for t in tickets]
+    finally:
+        srv.close()
+    for (lv, b, w, tcm), (_, _, codes, errs) in zip(reqs, kept):
+        alone = gpu.compress_many(lv, b, w, (1, 1, 1), tcm, tcm)
+        assert np.array_equal(codes, alone[0]), f"level {lv}, {len(b)} blocks: codes not written"
+        assert np.array_equal(errs.view(np.uint32), alone[1].view(np.uint32))
+
+
 def test_oversize_request_rejected(gpu):
     # a budget below the launch slots' minimum workspace is refused at creation
     with pytest.raises(gpu.IcbcError) as e:
