@@ -153,7 +153,11 @@ typedef struct icbc_plan_options {
  * must stay valid while the plan exists. */
 int icbc_gpu_plan_create(const icbc_bc1_job* jobs, size_t n_jobs, const icbc_plan_options* opt,
                          void* stream, icbc_bc1_plan** plan);
-/* Runs the plan on `stream` (asynchronous). */
+/* Runs the plan on `stream` (asynchronous).  Executions of one plan must not
+ * overlap on the device (issue them on one stream, or order them with
+ * events): they share the plan's tile queues, workspace and hand-off list,
+ * whose counters each execution leaves at zero for the next, and the jobs'
+ * outputs.  After a failed execution (a device error) destroy the plan. */
 int icbc_gpu_plan_execute(icbc_bc1_plan* plan, void* stream);
 int icbc_gpu_plan_destroy(icbc_bc1_plan* plan);
 /* Number of tiles in the plan. */
