@@ -308,6 +308,13 @@ def serving_bench(g, synth, seconds, qps_fracs, seed=5, pool_size=128, budget=64
     return out
 
 
+def drop_in_blocks(synth):
+    """The per-block workload: the 512x512 plain-content image (16,384 blocks)
+    as block-major float32 RGBA texels, the input file of icbc_amd/drop_in_bench."""
+    colors, _ = synth.make_image_blocks(1, 512, 512, synth.MIX_PLAIN)
+    return np.ascontiguousarray(colors, np.float32)
+
+
 def drop_in_per_block(g, synth, thread_counts, level=8, repeats=3, parity=None):
     """The reference's own call pattern through the drop-in C++ API: the
     harness's loop (icbc_test.cpp:389-413) compiled against libicbc_amd.so
@@ -318,12 +325,12 @@ def drop_in_per_block(g, synth, thread_counts, level=8, repeats=3, parity=None):
     import subprocess
     import tempfile
     from icbc_amd import build as hip_build
-    colors, _ = synth.make_image_blocks(1, 512, 512, synth.MIX_PLAIN)
+    colors = drop_in_blocks(synth)
     n = len(colors)
     runs = []
     with tempfile.TemporaryDirectory() as tmp:
         src = os.path.join(tmp, "blocks.f32")
-        np.ascontiguousarray(colors, np.float32).tofile(src)
+        colors.tofile(src)
         for t in thread_counts:
             out = os.path.join(tmp, f"codes{t}.bin")
             res = subprocess.run([hip_build.BENCH_BIN, src, str(n), str(t), str(level), str(repeats), out],
@@ -531,6 +538,8 @@ def main():
                     help="length of the short serving run included in the default benchmark")
     ap.add_argument("--serve-profile-seconds", type=float, default=60.0,
                     help="length of each load level of the --serve profile")
+    ap.add_argument("--write-drop-in-blocks", metavar="PATH",
+                    help="write the per-block workload's input file for icbc_amd/drop_in_bench and exit")
     ap.add_argument("--write-serving-pool", metavar="PATH",
                     help="only write the serving workload's RGBA8 request pool to PATH (the input of "
                          "icbc_amd/serve_bench, for profiling the server on its own) and exit")
@@ -562,6 +571,11 @@ def main():
     import icbc_amd as g
     from icbc_amd import synth
 
+    if args.write_drop_in_blocks:
+        blocks = drop_in_blocks(synth)
+        blocks.tofile(args.write_drop_in_blocks)
+        log(f"wrote {len(blocks)} blocks to {args.write_drop_in_blocks}")
+        return
     if args.write_serving_pool:
         pool = write_serving_pool(args.write_serving_pool, synth)
         log(f"wrote {len(pool)} textures, {sum(p['n'] for p in pool)} blocks to {args.write_serving_pool}")
