@@ -658,8 +658,13 @@ __device__ __forceinline__ void stage_tile(const JobDev& job, uint64_t first, in
     __syncthreads();
 }
 
-// A tile of a level 0-1 (or invalid-level) job.
-template <bool kUnit>
+// A tile of a level 0-1 (or invalid-level) job.  kStaged: the tile's texels
+// are staged through LDS (texel weights, or channel weights other than 1,
+// which need registers the texels would take); otherwise each lane reads its
+// block's 16 texels straight into registers (12-byte RGB loads, or one
+// 64-byte block of RGBA8 words), and the instance holds no LDS, so more of
+// its workgroups fit on a compute unit.
+template <bool kUnit, bool kStaged>
 __device__ __forceinline__ void encode_box_tile(const JobDev& job, uint64_t first, SharedBox& sh) {
     const int lane = threadIdx.x;
     const int nblk = (int)min<uint64_t>((uint64_t)job.tile_blocks, job.block_count - first);
@@ -667,7 +672,20 @@ __device__ __forceinline__ void encode_box_tile(const JobDev& job, uint64_t firs
     const uint64_t bi = first + lane;
     const Plan p = make_plan(job.level, job.tcm != 0, job.tcb != 0);
     float err = FLT_MAX;  // invalid level: FLT_MAX, output untouched (icbc.h:3766)
-    if (p.box) {
+    if constexpr (!kStaged) {
+        if (p.box && active) {
+            const f3 cw = mk(job.cw0, job.cw1, job.cw2);
+            Code out{0u, 0u};
+            if (job.rgba8) {
+                const TexelRegs<false> t(block_texels<true>(job, bi));
+                err = encode_box<kUnit>(t, p.lsq, cw, job.decoder, out);
+            } else {
+                const TexelRegs<false> t(block_texels<false>(job, bi));
+                err = encode_box<kUnit>(t, p.lsq, cw, job.decoder, out);
+            }
+            st_global(job.codes + bi, make_uint2(out.cols, out.idx));
+        }
+    } else if (p.box) {
         stage_tile(job, first, nblk, sh);
         if (active) {
             const f3 cw = mk(job.cw0, job.cw1, job.cw2);
@@ -864,7 +882,8 @@ __device__ __forceinline__ void load_tables(S& sh, int decoder) {
 // null the launch covers one job, tile k = blocks [64 k, 64 k + 64).
 // kUnit: every job of the launch has channel weights (1, 1, 1).
 // kBoxStage: some job of the launch is at level 0-3 (cluster instances).
-template <int kMode, int kTex, bool kUnit, bool kBoxStage = true>
+// kStaged (level 0-1 instance): see encode_box_tile.
+template <int kMode, int kTex, bool kUnit, bool kBoxStage = true, bool kStaged = true>
 __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) void bc1_encode_kernel(
     const JobDev* __restrict__ jobs, const uint2* __restrict__ tiles, uint32_t ntiles,
     uint32_t* __restrict__ counter, float4* __restrict__ workspace, RefineList refine) {
@@ -892,7 +911,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) v
             job = tile.x;
             first = tile.y;
         }
-        if constexpr (kMode == kModeBox) encode_box_tile<kUnit>(jobs[job], first, sh);  // either format, per tile
+        if constexpr (kMode == kModeBox) encode_box_tile<kUnit, kStaged>(jobs[job], first, sh);  // either format
         else if constexpr (kTex == kTexMixed) {
             if (jobs[job].rgba8) encode_tile<kMode, true, kUnit, kBoxStage>(jobs[job], job, first, sh, ws, refine);
             else encode_tile<kMode, false, kUnit, kBoxStage>(jobs[job], job, first, sh, ws, refine);
@@ -1296,25 +1315,29 @@ int encode_workgroups_per_cu() {
 
 namespace {
 
-// Resident workgroups of the level 0-1 instance on the current device (its
-// LDS staging area allows fewer per CU than the cluster instance, whose
-// residency sizes the launches and workspaces).
-uint32_t box_grid() {
+// Resident workgroups of a level 0-1 instance on the current device: the
+// staged one's LDS area allows fewer per CU than the cluster instance, whose
+// residency sizes the launches and workspaces; the register one (no LDS) as
+// many as its registers allow.
+uint32_t box_grid(bool staged) {
     static std::mutex m;
-    static std::vector<std::pair<int, uint32_t>> cache;
+    static std::vector<std::pair<int, uint32_t>> cache;  // (device * 2 + staged, grid)
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 256;
+    const int key = dev * 2 + (staged ? 1 : 0);
     std::lock_guard<std::mutex> lock(m);
     for (auto& c : cache)
-        if (c.first == dev) return c.second;
+        if (c.first == key) return c.second;
     int cus = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bc1_encode_kernel<kModeBox, kTexFloat, false>, kTile, 0) !=
-            hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
+    const hipError_t e =
+        staged ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bc1_encode_kernel<kModeBox, kTexFloat, false>,
+                                                              kTile, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                     &per_cu, bc1_encode_kernel<kModeBox, kTexFloat, true, true, false>, kTile, 0);
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
     const uint32_t g = (uint32_t)(cus * per_cu);
-    cache.push_back({dev, g});
+    cache.push_back({key, g});
     return g;
 }
 
@@ -1353,6 +1376,7 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
     const int tex = (mode >> 4) & 3;
     const bool unit = (mode & kUnitWeights) != 0;
     const bool box = (mode & kBoxStage) != 0;
+    const bool staged = !unit || (mode & kTexelWeights) != 0;  // level 0-1 instance
     mode &= 15;
     if (mode == kModeAll && !counters_zeroed) {
         hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, refine.counters, 2u,
@@ -1362,7 +1386,7 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
     }
     // every schedule covers all tiles with any grid (queue claims or a
     // grid-stride walk), so the level 0-1 instance runs at its own residency
-    const dim3 g(grid), bg(std::min(grid, box_grid())), blk(kTile);
+    const dim3 g(grid), bg(std::min(grid, box_grid(staged))), blk(kTile);
 #define ENC(M, T, U, B) hipLaunchKernelGGL((bc1_encode_kernel<M, T, U, B>), (M) == kModeBox ? bg : g, blk, 0, stream, \
                                            jobs, tiles, ntiles, counter, workspace, refine)
     // The unit-weight instances run the level 0-1 kernel and the refinement;
@@ -1371,7 +1395,10 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
     // of one texel format at levels 4-9 run cluster instances without the
     // box-fit stage.
     if (mode == kModeBox) {
-        if (unit) ENC(kModeBox, kTexFloat, true, true);
+        if (!staged)
+            hipLaunchKernelGGL((bc1_encode_kernel<kModeBox, kTexFloat, true, true, false>), bg, blk, 0, stream, jobs,
+                               tiles, ntiles, counter, workspace, refine);
+        else if (unit) ENC(kModeBox, kTexFloat, true, true);
         else ENC(kModeBox, kTexFloat, false, true);
     } else if (tex == kTexRgba8) {
         if (box) ENC(kModeCluster, kTexRgba8, false, true);
@@ -1391,7 +1418,8 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
 }
 
 hipError_t warm_launch_caches() {
-    (void)box_grid();
+    (void)box_grid(true);
+    (void)box_grid(false);
     (void)refine_grid();
     return hipGetLastError();
 }

@@ -138,13 +138,20 @@ constexpr int kUnitWeights = 1 << 6;
 
 inline bool unit_weights(const JobDev& j) { return j.cw0 == 1.0f && j.cw1 == 1.0f && j.cw2 == 1.0f; }
 
+// Some job of a launch has texel weights.  A level 0-1 launch whose jobs all
+// have unit channel weights and no texel weights runs the instance that reads
+// each block's texels straight into registers (no LDS staging area, so more
+// workgroups per compute unit); the others stage the tile's texels in LDS.
+constexpr int kTexelWeights = 1 << 8;
+
 // Some job of a launch is at level 0-3, whose box-fit stage the cluster
 // instance then carries; launches at levels 4-9 only run an instance without
 // it (fewer live registers across the per-block stages).
 constexpr int kBoxStage = 1 << 7;
 
 // `mode` selects the kernel instance (level mode | texel format << 4 |
-// kUnitWeights | kBoxStage); encode_mode() gives the one a set of jobs needs.
+// kUnitWeights | kBoxStage | kTexelWeights); encode_mode() gives the one a set
+// of jobs needs.
 // With kModeAll, `refine` holds room for the launch's level-9 blocks.
 // The queue counters (kQueueBytes at `counter`) and the handoff list's
 // counters are zeroed by a small kernel first, unless `counters_zeroed`: the
@@ -178,12 +185,13 @@ inline int encode_mode(const JobDev* jobs, size_t n) {
         else any_float = true;
     }
     const int tex = any_rgba8 ? (any_float ? kTexMixed : kTexRgba8) : kTexFloat;
-    bool unit = true, box = false;
+    bool unit = true, box = false, weighted = false;
     for (size_t k = 0; k < n; k++) {
         if (!unit_weights(jobs[k])) unit = false;
         if (jobs[k].level >= 0 && jobs[k].level <= 3) box = true;
+        if (jobs[k].weights) weighted = true;
     }
-    return mode | (tex << 4) | (unit ? kUnitWeights : 0) | (box ? kBoxStage : 0);
+    return mode | (tex << 4) | (unit ? kUnitWeights : 0) | (box ? kBoxStage : 0) | (weighted ? kTexelWeights : 0);
 }
 
 // The one-workgroup-per-tile schedule gives every tile its own workspace
