@@ -556,6 +556,29 @@ def test_nonfinite_blocks_every_level(gpu, oracle):
                 assert_same(got[0], got[1], want[0], want[1], f"non-finite L{level} tcm={tcm}")
 
 
+def test_levels_0_1_register_instance_edge_blocks(gpu, oracle):
+    # level 0-1 launches with unit channel weights and no texel weights run the
+    # instance that reads texels into registers: edge-case and non-finite
+    # colours, float and RGBA8, with random prior output bytes, against the
+    # oracle (the weighted tests above reach the LDS-staged instance)
+    ec, _ = synth.make_edge_blocks(61, 1500)
+    nc, _ = synth.make_nonfinite_blocks(62, 500)
+    colors = np.concatenate([ec, nc])
+    prior = np.random.default_rng(63).integers(0, 256, size=(len(colors), 8), dtype=np.uint8)
+    c8 = np.clip(np.nan_to_num(colors[:len(ec)], nan=0.0, posinf=1.0, neginf=0.0) * 255, 0, 255).round().astype(np.uint8)
+    cf8 = c8.astype(np.float32) / np.float32(255)
+    with np.errstate(all="ignore"):
+        for level in (0, 1):
+            for tcm in (False, True):
+                _say(f"register instance L{level} tcm={tcm}")
+                want = _oracle_run(oracle, level, colors, None, (1, 1, 1), tcm, tcm, prior)
+                got = gpu.compress_many(level, colors, None, (1, 1, 1), tcm, tcm, prior=prior)
+                assert_same(got[0], got[1], want[0], want[1], f"float L{level} tcm={tcm}")
+                want8 = _oracle_run(oracle, level, cf8, None, (1, 1, 1), tcm, tcm, prior[:len(ec)])
+                got8 = gpu.compress_many(level, c8, None, (1, 1, 1), tcm, tcm, prior=prior[:len(ec)])
+                assert_same(got8[0], got8[1], want8[0], want8[1], f"rgba8 L{level} tcm={tcm}")
+
+
 @pytest.mark.parametrize("prune", [0, 9])
 @pytest.mark.parametrize("cw", [(1, 1, 1), (0.3, 0.59, 0.11), (0, 1, 0), (5, 0.001, 1)])
 def test_pruned_search_adversarial_blocks(gpu, oracle, cw, prune):
