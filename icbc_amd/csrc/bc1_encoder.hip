@@ -101,7 +101,7 @@ __device__ __forceinline__ float4 row_at(const float4* rows, uint32_t e) {
     return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(rows) + ((e >> 12) & 0x1ffu));
 }
 
-// LDS of the level 0-1 kernel: the tile's texel colours, RGB, one row of 49
+// LDS of the staged level 0-1 instance: the tile's texel colours, RGB, one row of 49
 // floats per block (48 + 1 pad: an odd stride keeps the 64 lanes' reads of
 // texel i in distinct banks); 12,544 bytes.
 constexpr int kBoxRow = 49;
@@ -1385,11 +1385,11 @@ hipError_t launch_encode(const JobDev* jobs, const uint2* tiles, uint32_t ntiles
         if (e != hipSuccess) return e;
     }
     // every schedule covers all tiles with any grid (queue claims or a
-    // grid-stride walk), so the level 0-1 instance runs at its own residency
+    // grid-stride walk), so a level 0-1 instance runs at its own residency
     const dim3 g(grid), bg(std::min(grid, box_grid(staged))), blk(kTile);
 #define ENC(M, T, U, B) hipLaunchKernelGGL((bc1_encode_kernel<M, T, U, B>), (M) == kModeBox ? bg : g, blk, 0, stream, \
                                            jobs, tiles, ntiles, counter, workspace, refine)
-    // The unit-weight instances run the level 0-1 kernel and the refinement;
+    // The unit-weight instances run the level 0-1 kernels and the refinement;
     // the cluster kernel has only the general one (its unit-weight build was
     // measured slower: the compiler schedules the search worse).  Launches
     // of one texel format at levels 4-9 run cluster instances without the

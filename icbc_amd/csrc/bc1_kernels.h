@@ -71,8 +71,8 @@ struct HostTables {
 hipError_t upload_tables(const HostTables& t, hipStream_t stream);
 
 // Workgroups of the cluster-fit encode instance that fit on one compute unit
-// at once (sizes launches and workspaces; the level 0-1 instance, which fits
-// fewer, is launched at its own residency).
+// at once (sizes launches and workspaces; a level 0-1 instance is launched at
+// its own residency, at most this grid).
 int encode_workgroups_per_cu();
 
 // Encodes `ntiles` tiles with `grid` resident workgroups (clamped to ntiles).

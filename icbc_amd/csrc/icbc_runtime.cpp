@@ -763,8 +763,8 @@ struct SubLaunch {
 
 // A launch runs one kernel instance for all its jobs: levels 0-1 beside
 // cluster levels run in the cluster instance's box-fit stage, 3.5x slower per
-// block than the level 0-1 instance, which stages its tiles' texels in LDS
-// (level 1 on the 4096x4096 image: 0.36 against 0.10 ms), and levels 4-9
+// block than a level 0-1 instance (level 1 on the 4096x4096 image: 0.36
+// against 0.10 ms with the LDS-staged one), and levels 4-9
 // beside levels 2-3 run the instance that carries the box-fit stage (more
 // live registers, 1-2 % slower).  So jobs are grouped by the instance their
 // level needs -- levels 4-9, 2-3, 0-1 (invalid levels with 0-1) -- and a
