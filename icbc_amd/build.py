@@ -9,11 +9,12 @@ Compiler flags that the encoder's bit-exactness depends on:
   (HIP's default correctly-rounded fp32 division is left on)
 Packed-fp32 instruction selection is disabled for device code.
 
-Device code uses the compiler's default instruction scheduling.  The
-max-ILP strategy (-mllvm -amdgpu-sched-strategy=max-ilp) was measured against
-it on the current kernels and was slower on every workload (the level-8
-texture batch by 2.5 %, level 9 by 3 %); scheduling reorders instructions
-only and changes no arithmetic.
+Device code uses the compiler's default instruction scheduling.  The other
+strategies (-mllvm -amdgpu-sched-strategy=...) were measured against it on the
+current kernels: max-ilp was slower on every workload (the level-8 texture
+batch by 2.5 %, level 9 by 3 %), max-memory-clause by 1 %, iterative-ilp by
+2 % and iterative-minreg by 3 % (level 1 three times slower).  Scheduling
+reorders instructions only and changes no arithmetic.
 
 The library exports only the public API (csrc/exports.map).
 """
