@@ -66,6 +66,10 @@ double pct(std::vector<double> v, double p) {
 
 }  // namespace
 
+// Seconds between progress lines of a Poisson run (SERVE_REPORT_SECONDS
+// overrides, for a finer trace of a long run).
+static double kReportSeconds = 10.0;
+
 int main(int argc, char** argv) {
     if (argc != 10) {
         std::fprintf(stderr,
@@ -159,6 +163,7 @@ int main(int argc, char** argv) {
     std::memset(&so, 0, sizeof so);
     so.memory_budget = budget;
     so.max_inflight = inflight;
+    if (const char* e = std::getenv("SERVE_REPORT_SECONDS")) kReportSeconds = std::max(0.1, std::atof(e));
     icbc_server* srv = nullptr;
     if (icbc_server_create(&so, &srv) != ICBC_OK) {
         std::fprintf(stderr, "server: %s\n", icbc_gpu_last_error());
@@ -170,8 +175,15 @@ int main(int argc, char** argv) {
     // submission order, checks them and frees their buffers.
     std::atomic<size_t> outstanding{0};
     std::atomic<int> failed{0};
-    auto stream = [&](auto next) {
+    // `expected`: about how many requests the stream will submit.  The waiter
+    // records every result; its record vector is sized up front, because a
+    // reallocation mid-run (about 100 MB at 2^21 records) pauses the waiter,
+    // completed requests then hold their page-locked result buffers, and the
+    // submitter falls back to fresh pageable ones, whose results the server
+    // copies out with page faults on its completion thread.
+    auto stream = [&](auto next, size_t expected) {
         Run run;
+        run.infos.reserve(expected + expected / 4 + 1024);
         std::mutex qm;
         std::condition_variable qcv;
         std::deque<Pending> q;
@@ -247,7 +259,7 @@ int main(int argc, char** argv) {
     // warm server: two passes over the pool grow its slabs, staging and tables
     for (int pass = 0; pass < 2; pass++) {
         uint32_t i = 0;
-        stream([&] { return i < count ? (int)i++ : -1; });
+        stream([&] { return i < count ? (int)i++ : -1; }, count);
     }
 
     // saturation
@@ -255,10 +267,12 @@ int main(int argc, char** argv) {
     Run sat;
     {
         const auto t0 = clk::now();
-        sat = stream([&] {
-            while (outstanding.load() >= window) std::this_thread::yield();
-            return since(t0) < sat_seconds ? (int)pick(rng) : -1;
-        });
+        sat = stream(
+            [&] {
+                while (outstanding.load() >= window) std::this_thread::yield();
+                return since(t0) < sat_seconds ? (int)pick(rng) : -1;
+            },
+            (size_t)(sat_seconds * 100000));
     }
     auto span_of = [](const Run& r) {
         double a = 1e300, b = -1e300;
@@ -287,16 +301,23 @@ int main(int argc, char** argv) {
         const double rate = frac * sat_qps;
         std::exponential_distribution<double> gap(rate);
         const auto t0 = clk::now();
-        double t_next = 0, t_report = 10.0;
+        double t_next = 0, t_report = kReportSeconds;
         uint64_t submitted = 0;
         const Run r = stream([&] {
             t_next += gap(rng);
             if (t_next > seconds) return -1;
             if (t_next >= t_report) {  // a progress line every 10 s of a long run
-                std::fprintf(stderr, "serving @ %.0f req/s: %.0f s, %llu requests submitted\n", rate, t_report,
-                             (unsigned long long)submitted);
+                icbc_server_stats_t st{};
+                (void)icbc_server_stats(srv, &st);
+                std::fprintf(stderr,
+                             "serving @ %.0f req/s: %.0f s, %llu requests submitted, %llu completed, %llu queued, "
+                             "%llu launches in flight, %llu launches, device memory %.1f MB in use / %.1f MB "
+                             "held\n",
+                             rate, t_report, (unsigned long long)submitted, (unsigned long long)st.completed,
+                             (unsigned long long)st.pending, (unsigned long long)st.inflight,
+                             (unsigned long long)st.launches, st.memory_in_use / 1e6, st.memory_committed / 1e6);
                 std::fflush(stderr);
-                t_report += 10.0;
+                t_report += kReportSeconds;
             }
             submitted++;
             const auto due = t0 + std::chrono::duration_cast<clk::duration>(std::chrono::duration<double>(t_next));
@@ -305,8 +326,9 @@ int main(int argc, char** argv) {
             while (clk::now() < due) {
             }
             return (int)pick(rng);
-        });
+        }, (size_t)(rate * seconds));
         std::vector<double> lat;
+        lat.reserve(r.infos.size());
         for (const auto& i : r.infos) lat.push_back((i.done_s - i.submit_s) * 1e3);
         char line[1024];
         std::snprintf(line, sizeof line,
